@@ -1,0 +1,279 @@
+// C ABI of libddgx (declarations and reference correspondences in
+// include/ddgx.h).  Handles own their device memory; every compute entry is
+// stream-ordered on the caller's stream.
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ddgx_internal.h"
+
+struct ddgx_op_s {
+  ddgx::Op op;
+};
+struct ddgx_pc_s {
+  ddgx::PC pc;
+};
+
+namespace ddgx {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+
+int hip_fail(hipError_t e, const char* what) {
+  g_err = std::string(what) + ": " + hipGetErrorString(e);
+  return e == hipErrorOutOfMemory ? DDGX_ERR_NOMEM : DDGX_ERR_HIP;
+}
+
+int device_cu_count() {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 256;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus <= 0)
+    return 256;
+  return cus;
+}
+
+int apply_grid(int N, int dtype);
+
+static int table_size(int N) {
+  const int Np = (N + 1) * (N + 2) / 2, Nfp = N + 1;
+  return 5 * Np * Np + 3 * Nfp * Nfp + 6 * Np * Nfp;
+}
+
+template <typename T>
+static int upload(const double* src, size_t n, void** dst) {
+  std::vector<T> tmp(n);
+  for (size_t i = 0; i < n; ++i) tmp[i] = (T)src[i];
+  DDGX_HIP(hipMalloc(dst, n * sizeof(T)));
+  DDGX_HIP(hipMemcpy(*dst, tmp.data(), n * sizeof(T), hipMemcpyHostToDevice));
+  return DDGX_OK;
+}
+
+static int upload_as(int dtype, const double* src, size_t n, void** dst) {
+  return dtype == DDGX_F64 ? upload<double>(src, n, dst) : upload<float>(src, n, dst);
+}
+
+static void free_op(Op& o) {
+  void* ptrs[] = {o.tab, o.rec, o.nbr, o.fsj, o.r, o.p, o.z, o.Ap, o.partials, o.scal};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (o.scal_host) (void)hipHostFree(o.scal_host);
+  o = Op{};
+}
+
+}  // namespace ddgx
+
+using namespace ddgx;
+
+extern "C" {
+
+const char* ddgx_last_error(void) { return g_err.c_str(); }
+
+int ddgx_table_size(int N) { return table_size(N); }
+
+int ddgx_op_create(int N, int64_t K, int dtype, const double* tables, const double* rec,
+                   const int32_t* nbr, const double* fsj, ddgx_op* out) {
+  if (out == nullptr || tables == nullptr || rec == nullptr || nbr == nullptr || fsj == nullptr) {
+    set_error("ddgx_op_create: null argument");
+    return DDGX_ERR_INVALID;
+  }
+  *out = nullptr;
+  if (N < 1 || N > 6) {
+    set_error("ddgx_op_create: polynomial order " + std::to_string(N) + " not built (1..6)");
+    return DDGX_ERR_UNSUPPORTED;
+  }
+  if (dtype != DDGX_F64 && dtype != DDGX_F32) {
+    set_error("ddgx_op_create: dtype must be DDGX_F64 or DDGX_F32");
+    return DDGX_ERR_INVALID;
+  }
+  if (K < 1 || K >= (int64_t(1) << 28)) {
+    set_error("ddgx_op_create: element count out of range [1, 2^28)");
+    return DDGX_ERR_INVALID;
+  }
+  // validate neighbour codes on the host: they index device memory
+  for (int64_t k = 0; k < K; ++k)
+    for (int f = 0; f < 3; ++f) {
+      const int c = nbr[k * 4 + f];
+      if (c >= 0 && (nbr_elem(c) >= K || nbr_face(c) > 2)) {
+        set_error("ddgx_op_create: face code out of range at element " + std::to_string(k));
+        return DDGX_ERR_INVALID;
+      }
+      if (c < 0 && c != CODE_DIRICHLET && c != CODE_NONE) {
+        set_error("ddgx_op_create: invalid negative face code");
+        return DDGX_ERR_INVALID;
+      }
+    }
+  auto* h = new ddgx_op_s();
+  Op& o = h->op;
+  o.N = N;
+  o.K = K;
+  o.dtype = dtype;
+  o.Np = (N + 1) * (N + 2) / 2;
+  o.Nfp = N + 1;
+  int rc = DDGX_OK;
+  do {
+    if (hipGetDevice(&o.device) != hipSuccess) { rc = hip_fail(hipGetLastError(), "hipGetDevice"); break; }
+    if ((rc = upload_as(dtype, tables, table_size(N), &o.tab)) != DDGX_OK) break;
+    if ((rc = upload_as(dtype, rec, (size_t)K * DDGX_REC_WORDS, &o.rec)) != DDGX_OK) break;
+    if ((rc = upload_as(dtype, fsj, (size_t)K * 4, &o.fsj)) != DDGX_OK) break;
+    hipError_t e = hipMalloc((void**)&o.nbr, (size_t)K * 4 * sizeof(int));
+    if (e != hipSuccess) { rc = hip_fail(e, "hipMalloc(nbr)"); break; }
+    e = hipMemcpy(o.nbr, nbr, (size_t)K * 4 * sizeof(int), hipMemcpyHostToDevice);
+    if (e != hipSuccess) { rc = hip_fail(e, "hipMemcpy(nbr)"); break; }
+    o.grid_apply = apply_grid(N, dtype);
+    const int64_t nvec = K * o.Np;
+    int64_t gv = (nvec + 255) / 256;
+    const int64_t cap = 4 * (int64_t)device_cu_count();
+    o.grid_vec = (int)(gv < cap ? gv : cap);
+  } while (false);
+  if (rc != DDGX_OK) {
+    free_op(o);
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return DDGX_OK;
+}
+
+int ddgx_op_destroy(ddgx_op op) {
+  if (op == nullptr) return DDGX_OK;
+  free_op(op->op);
+  delete op;
+  return DDGX_OK;
+}
+
+int ddgx_op_info(ddgx_op op, int* N, int64_t* K, int* dtype) {
+  if (op == nullptr) {
+    set_error("ddgx_op_info: null handle");
+    return DDGX_ERR_INVALID;
+  }
+  if (N) *N = op->op.N;
+  if (K) *K = op->op.K;
+  if (dtype) *dtype = op->op.dtype;
+  return DDGX_OK;
+}
+
+int ddgx_sipg_apply(ddgx_op op, const void* u, void* y, const void* u_dir, double alpha,
+                    double shift, void* stream) {
+  if (op == nullptr || u == nullptr || y == nullptr) {
+    set_error("ddgx_sipg_apply: null argument");
+    return DDGX_ERR_INVALID;
+  }
+  if (u == y) {
+    set_error("ddgx_sipg_apply: u and y must not alias");
+    return DDGX_ERR_INVALID;
+  }
+  return launch_apply(op->op, u, y, u_dir, alpha, shift, 0, 0, nullptr, nullptr,
+                      (hipStream_t)stream);
+}
+
+int ddgx_sipg_rhs(ddgx_op op, const void* f, const void* u_dir, const void* q_neu, void* rhs,
+                  void* stream) {
+  if (op == nullptr || rhs == nullptr) {
+    set_error("ddgx_sipg_rhs: null argument");
+    return DDGX_ERR_INVALID;
+  }
+  return launch_rhs(op->op, f, u_dir, q_neu, rhs, (hipStream_t)stream);
+}
+
+int ddgx_block_jacobi_create(ddgx_op op64, double alpha, int dtype, void* stream, ddgx_pc* out) {
+  if (op64 == nullptr || out == nullptr) {
+    set_error("ddgx_block_jacobi_create: null argument");
+    return DDGX_ERR_INVALID;
+  }
+  if (op64->op.dtype != DDGX_F64) {
+    set_error("ddgx_block_jacobi_create: blocks are built from a float64 operator");
+    return DDGX_ERR_INVALID;
+  }
+  if (dtype != DDGX_F64 && dtype != DDGX_F32) {
+    set_error("ddgx_block_jacobi_create: bad dtype");
+    return DDGX_ERR_INVALID;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  Op& o = op64->op;
+  auto* h = new ddgx_pc_s();
+  PC& pc = h->pc;
+  pc.op = &o;
+  pc.dtype = dtype;
+  const size_t nb = (size_t)o.K * o.Np * o.Np;
+  int rc = DDGX_OK;
+  do {
+    hipError_t e = hipMalloc((void**)&pc.blocks, nb * sizeof(double));
+    if (e != hipSuccess) { rc = hip_fail(e, "hipMalloc(blocks)"); break; }
+    e = hipMalloc(&pc.binvT, nb * (dtype == DDGX_F64 ? 8 : 4));
+    if (e != hipSuccess) { rc = hip_fail(e, "hipMalloc(binvT)"); break; }
+    for (int j = 0; j < o.Np && rc == DDGX_OK; ++j)
+      rc = launch_apply(o, nullptr, pc.blocks, nullptr, alpha, 0.0, M_PROBE, j, nullptr, nullptr, s);
+    if (rc != DDGX_OK) break;
+    rc = launch_invert_blocks(o.N, o.K, dtype, pc.blocks, pc.binvT, s);
+    if (rc != DDGX_OK) break;
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) { rc = hip_fail(e, "block-Jacobi build"); break; }
+  } while (false);
+  if (rc != DDGX_OK) {
+    if (pc.blocks) (void)hipFree(pc.blocks);
+    if (pc.binvT) (void)hipFree(pc.binvT);
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return DDGX_OK;
+}
+
+int ddgx_block_jacobi_blocks(ddgx_pc pc, void* blocks, void* stream) {
+  if (pc == nullptr || blocks == nullptr) {
+    set_error("ddgx_block_jacobi_blocks: null argument");
+    return DDGX_ERR_INVALID;
+  }
+  const Op& o = *pc->pc.op;
+  DDGX_HIP(hipMemcpyAsync(blocks, pc->pc.blocks, (size_t)o.K * o.Np * o.Np * sizeof(double),
+                          hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  return DDGX_OK;
+}
+
+int ddgx_pc_apply(ddgx_pc pc, const void* r, void* z, void* stream) {
+  if (pc == nullptr || r == nullptr || z == nullptr) {
+    set_error("ddgx_pc_apply: null argument");
+    return DDGX_ERR_INVALID;
+  }
+  const Op& o = *pc->pc.op;
+  return launch_bj_apply(o.N, o.K, pc->pc.dtype, pc->pc.binvT, r, z, (hipStream_t)stream);
+}
+
+int ddgx_pc_destroy(ddgx_pc pc) {
+  if (pc == nullptr) return DDGX_OK;
+  if (pc->pc.blocks) (void)hipFree(pc->pc.blocks);
+  if (pc->pc.binvT) (void)hipFree(pc->pc.binvT);
+  delete pc;
+  return DDGX_OK;
+}
+
+int ddgx_pcg_solve(ddgx_op op, double alpha, double shift, ddgx_pc pc, const void* b, void* x,
+                   const ddgx_cg_cfg* cfg, ddgx_cg_info* info, void* stream) {
+  if (op == nullptr || b == nullptr || x == nullptr || cfg == nullptr) {
+    set_error("ddgx_pcg_solve: null argument");
+    return DDGX_ERR_INVALID;
+  }
+  if (cfg->maxiter < 0) {
+    set_error("ddgx_pcg_solve: maxiter must be >= 0");
+    return DDGX_ERR_INVALID;
+  }
+  if (pc != nullptr && (pc->pc.op->K != op->op.K || pc->pc.op->N != op->op.N)) {
+    set_error("ddgx_pcg_solve: preconditioner built for a different mesh/order");
+    return DDGX_ERR_INVALID;
+  }
+  return pcg_solve(op->op, alpha, shift, pc ? &pc->pc : nullptr, b, x, *cfg, info,
+                   (hipStream_t)stream);
+}
+
+int ddgx_dot(ddgx_op op, const void* a, const void* b, double* out, void* stream) {
+  if (op == nullptr || a == nullptr || b == nullptr || out == nullptr) {
+    set_error("ddgx_dot: null argument");
+    return DDGX_ERR_INVALID;
+  }
+  return dot(op->op, a, b, out, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,411 @@
+// Conjugate gradient on the device, with the stopping rule of
+// jax.scipy.sparse.linalg.cg (called at src/ddg/dg2d/sipg.py:603-609,
+// 909-915):
+//
+//   r0 = b - A x0;  p0 = z0 = M r0;  gamma0 = <r0, z0>
+//   atol2 = max(tol^2 <b,b>, atol^2)
+//   while rs > atol2 and k < maxiter      (rs = gamma without M, <r,r> with M)
+//     alpha = gamma / <p, A p>; x += alpha p; r -= alpha A p; z = M r
+//     gamma' = <r, z>; p = z + (gamma'/gamma) p
+//
+// All scalars live in device memory (CGScalars) and every kernel of an
+// iteration reads them there, so a chunk of iterations is enqueued with no
+// host synchronisation; kernels return at once when the device stop flag is
+// set, and the host polls that flag every `check_every` iterations.
+// Reductions are two-stage and fixed-order (per-workgroup partials in
+// float64, then one workgroup sums them), so results are bitwise repeatable.
+
+#include "ddgx_internal.h"
+
+namespace ddgx {
+
+int apply_grid(int N, int dtype);
+
+enum { F_INIT_BB = 0, F_INIT_GAMMA = 1, F_ALPHA = 2, F_BETA = 3, F_SUM = 4 };
+
+__device__ inline double block_sum_256(double v, double* s_red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
+  __syncthreads();
+  return t;
+}
+
+// partial[b] = sum a*b ; partial[G + b] = sum c*d (if c != nullptr)
+template <typename T>
+__global__ void __launch_bounds__(256)
+dot2_kernel(const T* __restrict__ a, const T* __restrict__ b, const T* __restrict__ c,
+            const T* __restrict__ d, size_t n, double* __restrict__ partial,
+            const int* __restrict__ skip) {
+  if (skip != nullptr && *skip) return;
+  __shared__ double s_red[4];
+  double s0 = 0.0, s1 = 0.0;
+  const size_t stride = (size_t)gridDim.x * 256;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+    s0 += (double)a[i] * (double)b[i];
+    if (c != nullptr) s1 += (double)c[i] * (double)d[i];
+  }
+  s0 = block_sum_256(s0, s_red);
+  s1 = block_sum_256(s1, s_red);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = s0;
+    partial[gridDim.x + blockIdx.x] = s1;
+  }
+}
+
+// One workgroup: sum the partials in fixed order and advance the scalars.
+__global__ void __launch_bounds__(256)
+finalize_kernel(const double* __restrict__ partial, int nparts, CGScalars* sc, int mode) {
+  if (mode >= F_ALPHA && mode <= F_BETA && sc->done) return;
+  __shared__ double s_red[4];
+  double s0 = 0.0, s1 = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += 256) {
+    s0 += partial[i];
+    s1 += partial[nparts + i];
+  }
+  s0 = block_sum_256(s0, s_red);
+  s1 = block_sum_256(s1, s_red);
+  if (threadIdx.x != 0) return;
+  switch (mode) {
+    case F_INIT_BB: {
+      sc->bb = s0;
+      const double t = sc->tol * sc->tol * s0, a = sc->atol * sc->atol;
+      sc->atol2 = t > a ? t : a;
+      break;
+    }
+    case F_INIT_GAMMA:
+      sc->gamma = s0;
+      sc->rr = sc->use_pc ? s1 : s0;
+      sc->rs = sc->rr;
+      sc->k = 0;
+      sc->done = !(sc->rs > sc->atol2 && sc->k < sc->maxiter);
+      break;
+    case F_ALPHA:
+      sc->pAp = s0;
+      sc->alpha = sc->gamma / s0;
+      break;
+    case F_BETA:
+      sc->beta = s0 / sc->gamma;
+      sc->gamma = s0;
+      sc->rr = sc->use_pc ? s1 : s0;
+      sc->rs = sc->rr;
+      sc->k += 1;
+      sc->done = !(sc->rs > sc->atol2 && sc->k < sc->maxiter);
+      break;
+    default:  // F_SUM
+      sc->pAp = s0;
+      sc->rr = s1;
+      break;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+residual_kernel(const T* __restrict__ b, const T* __restrict__ Ax, T* __restrict__ r, size_t n) {
+  const size_t stride = (size_t)gridDim.x * 256;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) r[i] = b[i] - Ax[i];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+copy_kernel(const T* __restrict__ a, T* __restrict__ b, size_t n) {
+  const size_t stride = (size_t)gridDim.x * 256;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) b[i] = a[i];
+}
+
+// x += alpha p ; r -= alpha Ap   (+ partial <r,r> when no preconditioner)
+template <typename T>
+__global__ void __launch_bounds__(256)
+update_kernel(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
+              const T* __restrict__ Ap, const CGScalars* __restrict__ sc, size_t n,
+              double* __restrict__ partial) {
+  if (sc->done) return;
+  __shared__ double s_red[4];
+  const T alpha = (T)sc->alpha;
+  double acc = 0.0;
+  const size_t stride = (size_t)gridDim.x * 256;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+    x[i] += alpha * p[i];
+    const T ri = r[i] - alpha * Ap[i];
+    r[i] = ri;
+    acc += (double)ri * (double)ri;
+  }
+  if (partial != nullptr) {
+    acc = block_sum_256(acc, s_red);
+    if (threadIdx.x == 0) {
+      partial[blockIdx.x] = acc;
+      partial[gridDim.x + blockIdx.x] = acc;
+    }
+  }
+}
+
+// x += alpha p ; r -= alpha Ap ; z = Binv r (per element) ; partial <r,z>, <r,r>
+template <typename T, int N>
+__global__ void __launch_bounds__(256)
+update_bj_kernel(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
+                 const T* __restrict__ Ap, const T* __restrict__ binvT, T* __restrict__ z,
+                 const CGScalars* __restrict__ sc, int K, double* __restrict__ partial) {
+  using S = Shape<N>;
+  constexpr int Np = S::Np, NL = S::NL, EPB = S::EPB;
+  if (sc->done) return;
+  __shared__ T s_r[EPB * Np];
+  __shared__ double s_red[4];
+  const T alpha = (T)sc->alpha;
+  const int tid = threadIdx.x, e = tid / NL, li = tid % NL;
+  double rz = 0.0, rr = 0.0;
+  const int ntiles = (K + EPB - 1) / EPB;
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int k0 = tile * EPB;
+    const int ne = min(EPB, K - k0);
+    __syncthreads();
+    for (int i = tid; i < ne * Np; i += 256) {
+      const size_t g = (size_t)k0 * Np + i;
+      x[g] += alpha * p[g];
+      const T ri = r[g] - alpha * Ap[g];
+      r[g] = ri;
+      s_r[i] = ri;
+    }
+    __syncthreads();
+    if (li < Np && e < ne) {
+      const T* B = binvT + (size_t)(k0 + e) * Np * Np;
+      T acc = 0;
+#pragma unroll
+      for (int j = 0; j < Np; ++j) acc += B[j * Np + li] * s_r[e * Np + j];
+      z[(size_t)(k0 + e) * Np + li] = acc;
+      const double ri = (double)s_r[e * Np + li];
+      rz += ri * (double)acc;
+      rr += ri * ri;
+    }
+  }
+  rz = block_sum_256(rz, s_red);
+  rr = block_sum_256(rr, s_red);
+  if (tid == 0) {
+    partial[blockIdx.x] = rz;
+    partial[gridDim.x + blockIdx.x] = rr;
+  }
+}
+
+// standalone block-Jacobi apply z = Binv r
+template <typename T, int N>
+__global__ void __launch_bounds__(256)
+bj_apply_kernel(const T* __restrict__ binvT, const T* __restrict__ r, T* __restrict__ z, int K) {
+  using S = Shape<N>;
+  constexpr int Np = S::Np, NL = S::NL, EPB = S::EPB;
+  __shared__ T s_r[EPB * Np];
+  const int tid = threadIdx.x, e = tid / NL, li = tid % NL;
+  const int ntiles = (K + EPB - 1) / EPB;
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int k0 = tile * EPB;
+    const int ne = min(EPB, K - k0);
+    __syncthreads();
+    for (int i = tid; i < ne * Np; i += 256) s_r[i] = r[(size_t)k0 * Np + i];
+    __syncthreads();
+    if (li < Np && e < ne) {
+      const T* B = binvT + (size_t)(k0 + e) * Np * Np;
+      T acc = 0;
+#pragma unroll
+      for (int j = 0; j < Np; ++j) acc += B[j * Np + li] * s_r[e * Np + j];
+      z[(size_t)(k0 + e) * Np + li] = acc;
+    }
+  }
+}
+
+// p = z + beta p
+template <typename T>
+__global__ void __launch_bounds__(256)
+pupdate_kernel(T* __restrict__ p, const T* __restrict__ z, const CGScalars* __restrict__ sc,
+               size_t n) {
+  if (sc->done) return;
+  const T beta = (T)sc->beta;
+  const size_t stride = (size_t)gridDim.x * 256;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
+    p[i] = z[i] + beta * p[i];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+template <int N>
+static int bj_apply_t(int64_t K, int dtype, const void* binvT, const void* r, void* z,
+                      hipStream_t s) {
+  const int ntiles = (int)((K + Shape<N>::EPB - 1) / Shape<N>::EPB);
+  int grid = 8 * device_cu_count();
+  grid = ntiles < grid ? ntiles : grid;
+  if (dtype == DDGX_F64)
+    hipLaunchKernelGGL((bj_apply_kernel<double, N>), dim3(grid), dim3(256), 0, s,
+                       (const double*)binvT, (const double*)r, (double*)z, (int)K);
+  else
+    hipLaunchKernelGGL((bj_apply_kernel<float, N>), dim3(grid), dim3(256), 0, s,
+                       (const float*)binvT, (const float*)r, (float*)z, (int)K);
+  DDGX_HIP(hipGetLastError());
+  return DDGX_OK;
+}
+
+int launch_bj_apply(int N, int64_t K, int dtype, const void* binvT, const void* r, void* z,
+                    hipStream_t s) {
+  switch (N) {
+    case 1: return bj_apply_t<1>(K, dtype, binvT, r, z, s);
+    case 2: return bj_apply_t<2>(K, dtype, binvT, r, z, s);
+    case 3: return bj_apply_t<3>(K, dtype, binvT, r, z, s);
+    case 4: return bj_apply_t<4>(K, dtype, binvT, r, z, s);
+    case 5: return bj_apply_t<5>(K, dtype, binvT, r, z, s);
+    case 6: return bj_apply_t<6>(K, dtype, binvT, r, z, s);
+  }
+  set_error("polynomial order not built");
+  return DDGX_ERR_UNSUPPORTED;
+}
+
+static size_t dsize(int dtype) { return dtype == DDGX_F64 ? 8 : 4; }
+
+static int ensure_workspace(Op& op) {
+  if (op.r != nullptr) return DDGX_OK;
+  const size_t n = (size_t)op.K * op.Np * dsize(op.dtype);
+  DDGX_HIP(hipSetDevice(op.device));
+  if (hipMalloc(&op.r, n) != hipSuccess || hipMalloc(&op.p, n) != hipSuccess ||
+      hipMalloc(&op.z, n) != hipSuccess || hipMalloc(&op.Ap, n) != hipSuccess) {
+    set_error("device allocation of CG workspace failed");
+    return DDGX_ERR_NOMEM;
+  }
+  const int g = op.grid_apply > op.grid_vec ? op.grid_apply : op.grid_vec;
+  DDGX_HIP(hipMalloc((void**)&op.partials, 2 * (size_t)g * sizeof(double)));
+  DDGX_HIP(hipMalloc((void**)&op.scal, sizeof(CGScalars)));
+  DDGX_HIP(hipHostMalloc((void**)&op.scal_host, sizeof(CGScalars), hipHostMallocDefault));
+  return DDGX_OK;
+}
+
+template <typename T>
+static int dot2(const Op& op, const void* a, const void* b, const void* c, const void* d,
+                const int* skip, hipStream_t s) {
+  const size_t n = (size_t)op.K * op.Np;
+  hipLaunchKernelGGL((dot2_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (const T*)a,
+                     (const T*)b, (const T*)c, (const T*)d, n, op.partials, skip);
+  DDGX_HIP(hipGetLastError());
+  return DDGX_OK;
+}
+
+static int finalize(const Op& op, int nparts, int mode, hipStream_t s) {
+  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, s, (const double*)op.partials,
+                     nparts, op.scal, mode);
+  DDGX_HIP(hipGetLastError());
+  return DDGX_OK;
+}
+
+template <typename T, int N>
+static int update_bj(const Op& op, void* x, const void* binvT, hipStream_t s) {
+  hipLaunchKernelGGL((update_bj_kernel<T, N>), dim3(op.grid_vec), dim3(256), 0, s, (T*)x,
+                     (T*)op.r, (const T*)op.p, (const T*)op.Ap, (const T*)binvT, (T*)op.z,
+                     (const CGScalars*)op.scal, (int)op.K, op.partials);
+  DDGX_HIP(hipGetLastError());
+  return DDGX_OK;
+}
+
+template <typename T>
+static int update_bj_dispatch(const Op& op, void* x, const void* binvT, hipStream_t s) {
+  switch (op.N) {
+    case 1: return update_bj<T, 1>(op, x, binvT, s);
+    case 2: return update_bj<T, 2>(op, x, binvT, s);
+    case 3: return update_bj<T, 3>(op, x, binvT, s);
+    case 4: return update_bj<T, 4>(op, x, binvT, s);
+    case 5: return update_bj<T, 5>(op, x, binvT, s);
+    case 6: return update_bj<T, 6>(op, x, binvT, s);
+  }
+  set_error("polynomial order not built");
+  return DDGX_ERR_UNSUPPORTED;
+}
+
+#define CHECK(x)                 \
+  do {                           \
+    int _rc = (x);               \
+    if (_rc != DDGX_OK) return _rc; \
+  } while (0)
+
+template <typename T>
+static int pcg_t(Op& op, double alpha, double shift, PC* pc, const void* b, void* x,
+                 const ddgx_cg_cfg& cfg, ddgx_cg_info* info, hipStream_t s) {
+  const size_t n = (size_t)op.K * op.Np;
+  const bool bj = pc != nullptr;
+  CGScalars& init = *op.scal_host;  // pinned staging buffer
+  init = CGScalars{};
+  init.tol = cfg.tol;
+  init.atol = cfg.atol;
+  init.maxiter = cfg.maxiter;
+  init.use_pc = bj ? 1 : 0;
+  DDGX_HIP(hipMemcpyAsync(op.scal, &init, sizeof(CGScalars), hipMemcpyHostToDevice, s));
+  const int* done = &op.scal->done;
+  void* z = bj ? op.z : op.r;  // without a preconditioner z aliases r
+
+  // atol2 from <b,b>
+  CHECK(dot2<T>(op, b, b, nullptr, nullptr, nullptr, s));
+  CHECK(finalize(op, op.grid_vec, F_INIT_BB, s));
+  // r0 = b - A x0
+  CHECK(launch_apply(op, x, op.Ap, nullptr, alpha, shift, 0, 0, nullptr, nullptr, s));
+  hipLaunchKernelGGL((residual_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (const T*)b,
+                     (const T*)op.Ap, (T*)op.r, n);
+  DDGX_HIP(hipGetLastError());
+  if (bj) CHECK(launch_bj_apply(op.N, op.K, op.dtype, pc->binvT, op.r, op.z, s));
+  hipLaunchKernelGGL((copy_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (const T*)z,
+                     (T*)op.p, n);
+  DDGX_HIP(hipGetLastError());
+  CHECK(dot2<T>(op, op.r, z, op.r, op.r, nullptr, s));
+  CHECK(finalize(op, op.grid_vec, F_INIT_GAMMA, s));
+
+  const int every = cfg.check_every > 0 ? cfg.check_every : 16;
+  int launched = 0;
+  while (true) {
+    DDGX_HIP(hipMemcpyAsync(op.scal_host, op.scal, sizeof(CGScalars), hipMemcpyDeviceToHost, s));
+    DDGX_HIP(hipStreamSynchronize(s));
+    if (op.scal_host->done || launched >= cfg.maxiter) break;
+    for (int c = 0; c < every && launched < cfg.maxiter; ++c, ++launched) {
+      // Ap = A p, partial <p, Ap>
+      CHECK(launch_apply(op, op.p, op.Ap, nullptr, alpha, shift, M_DOT, 0, op.partials, done, s));
+      CHECK(finalize(op, op.grid_apply, F_ALPHA, s));
+      if (bj) {
+        CHECK(update_bj_dispatch<T>(op, x, pc->binvT, s));
+      } else {
+        hipLaunchKernelGGL((update_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (T*)x,
+                           (T*)op.r, (const T*)op.p, (const T*)op.Ap,
+                           (const CGScalars*)op.scal, n, op.partials);
+        DDGX_HIP(hipGetLastError());
+      }
+      CHECK(finalize(op, op.grid_vec, F_BETA, s));
+      hipLaunchKernelGGL((pupdate_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (T*)op.p,
+                         (const T*)z, (const CGScalars*)op.scal, n);
+      DDGX_HIP(hipGetLastError());
+    }
+  }
+  const CGScalars& h = *op.scal_host;
+  if (info != nullptr) {
+    info->iterations = h.k;
+    info->residual2 = h.rs;
+    info->atol2 = h.atol2;
+    info->converged = h.rs <= h.atol2 ? 1 : 0;
+  }
+  return DDGX_OK;
+}
+
+int pcg_solve(Op& op, double alpha, double shift, PC* pc, const void* b, void* x,
+              const ddgx_cg_cfg& cfg, ddgx_cg_info* info, hipStream_t s) {
+  CHECK(ensure_workspace(op));
+  if (pc != nullptr && pc->dtype != op.dtype) {
+    set_error("preconditioner dtype differs from operator dtype");
+    return DDGX_ERR_INVALID;
+  }
+  if (op.dtype == DDGX_F64) return pcg_t<double>(op, alpha, shift, pc, b, x, cfg, info, s);
+  return pcg_t<float>(op, alpha, shift, pc, b, x, cfg, info, s);
+}
+
+int dot(Op& op, const void* a, const void* b, double* out, hipStream_t s) {
+  CHECK(ensure_workspace(op));
+  if (op.dtype == DDGX_F64)
+    CHECK(dot2<double>(op, a, b, nullptr, nullptr, nullptr, s));
+  else
+    CHECK(dot2<float>(op, a, b, nullptr, nullptr, nullptr, s));
+  CHECK(finalize(op, op.grid_vec, F_SUM, s));
+  DDGX_HIP(hipMemcpyAsync(op.scal_host, op.scal, sizeof(CGScalars), hipMemcpyDeviceToHost, s));
+  DDGX_HIP(hipStreamSynchronize(s));
+  *out = op.scal_host->pAp;
+  return DDGX_OK;
+}
+
+}  // namespace ddgx

@@ -1,0 +1,265 @@
+"""Device-resident SIPG operator: packs a ``Mesh2D`` + BC tags + penalty
+into per-element records and owns the libddgx handle.
+
+Packing (host, float64) follows the reference's per-face quantities
+(src/ddg/dg2d/sipg.py:117-139, 211-280): for element k and face f,
+
+* ``tau_k,f = tau_scale * (N+1)^2 * Fscale[face node 0]``     (sipg.py:263)
+* interior face (both sides tagged 0, distinct neighbour k', f'):
+  ``pen = (kappa_k tau_k sJ_k + kappa_k' tau_k' sJ_k') / 2``,
+  ``sJw = (sJ_k + sJ_k') / 4``  -- the two half-weighted sides of the
+  reference's face loop (w = 0.5) combined;
+* Dirichlet: ``pen = kappa_k tau_k sJ_k``, ``sJw = sJ_k``     (w = 1);
+* Neumann, or a boundary face tagged interior: no coupling (w = 0 in the
+  reference; a self-neighbour face has zero jump and its averaged terms
+  cancel in the gradient).
+
+Element-constant geometry is read at node 0 of each element (affine
+elements: the reference's per-node arrays are constant up to rounding).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from .reference import ref_element
+
+BC_INTERIOR, BC_DIRICHLET, BC_NEUMANN = 0, 1, 2
+
+
+def _face_rows(a, Nfp, K):
+    """(3*Nfp, K) face array -> (K, 3) values at face node 0."""
+    return np.asarray(a, dtype=np.float64).reshape((Nfp, 3, K), order="F")[0].T
+
+
+def pack(mesh, bc_types, tau_scale=200.0, kappa=None):
+    """Return ``(rec (K,16), nbr (K,4) int32, fsj (K,4))`` float64/int32."""
+    N, K, Nfp = int(mesh.N), int(mesh.K), int(mesh.Nfp)
+    ref = ref_element(N)
+    if not np.array_equal(np.asarray(mesh.Fmask), ref.Fmask):
+        raise ValueError("mesh.Fmask does not match the degree-N reference element")
+    bc = np.asarray(bc_types).astype(np.int64)
+    if bc.shape != (K, 3):
+        raise ValueError(f"bc_types must have shape (K, 3) = {(K, 3)}, got {bc.shape}")
+    if np.any((bc < 0) | (bc > 2)):
+        raise ValueError("bc_types entries must be BC_INTERIOR, BC_DIRICHLET or BC_NEUMANN")
+    kap = np.ones(K) if kappa is None else np.asarray(kappa, dtype=np.float64).reshape(K)
+    J = np.asarray(mesh.J, dtype=np.float64)[0]
+    rx, sx = np.asarray(mesh.rx)[0], np.asarray(mesh.sx)[0]
+    ry, sy = np.asarray(mesh.ry)[0], np.asarray(mesh.sy)[0]
+    nx, ny = _face_rows(mesh.nx, Nfp, K), _face_rows(mesh.ny, Nfp, K)
+    sJ, Fs = _face_rows(mesh.sJ, Nfp, K), _face_rows(mesh.Fscale, Nfp, K)
+    tau = np.broadcast_to(np.asarray(tau_scale, dtype=np.float64), (3, K)).T
+    tau_side = tau * (N + 1) ** 2 * Fs * kap[:, None]             # (K, 3)
+
+    EToE = np.asarray(mesh.EToE, dtype=np.int64)
+    EToF = np.asarray(mesh.EToF, dtype=np.int64)
+    selfn = EToE == np.arange(K)[:, None]
+    interior = (bc == BC_INTERIOR) & ~selfn
+    kn, fn = EToE, EToF
+    if np.any(interior & (bc[kn, fn] != BC_INTERIOR)):
+        raise ValueError("interior faces must be tagged BC_INTERIOR on both sides")
+    # neighbour node order from vmapP: same or reversed face-node order
+    vM = np.asarray(mesh.vmapM).reshape((Nfp, 3, K), order="F")
+    vP = np.asarray(mesh.vmapP).reshape((Nfp, 3, K), order="F")
+    nbM = vM[:, fn, kn]                                           # (Nfp, K, 3)
+    Pk = np.transpose(vP, (0, 2, 1))                              # (Nfp, K, 3)
+    fwd = np.all(Pk == nbM, axis=0)
+    rev = np.all(Pk == nbM[::-1], axis=0)
+    if np.any(interior & ~(fwd | rev)):
+        raise ValueError("vmapP does not match the neighbour face in either node order")
+    r = (interior & rev & ~fwd).astype(np.int64)
+
+    code = np.where(bc == BC_DIRICHLET, -1, -2).astype(np.int64)
+    code = np.where(interior, (kn << 3) | (fn << 1) | r, code)
+    sJn = sJ[kn, fn]
+    tn = tau_side[kn, fn]
+    isd = bc == BC_DIRICHLET
+    sjw = np.where(interior, 0.25 * (sJ + sJn), np.where(isd, sJ, 0.0))
+    pen = np.where(interior, 0.5 * (tau_side * sJ + tn * sJn), np.where(isd, tau_side * sJ, 0.0))
+
+    rec = np.zeros((K, _lib.REC_WORDS))
+    rec[:, 0] = kap * J * (rx * rx + ry * ry)
+    rec[:, 1] = kap * J * (rx * sx + ry * sy)
+    rec[:, 2] = kap * J * (sx * sx + sy * sy)
+    rec[:, 3] = J
+    rec[:, 4:7] = kap[:, None] * (nx * rx[:, None] + ny * ry[:, None])
+    rec[:, 7:10] = kap[:, None] * (nx * sx[:, None] + ny * sy[:, None])
+    rec[:, 10:13] = sjw
+    rec[:, 13:16] = pen
+    nbr = np.zeros((K, 4), dtype=np.int32)
+    nbr[:, :3] = code
+    fsj = np.zeros((K, 4))
+    fsj[:, :3] = np.where(bc == BC_NEUMANN, sJ, 0.0)
+    return rec, nbr, fsj
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+class SipgOperator:
+    """One SIPG operator on the current HIP device, in float64 or float32.
+
+    ``apply`` / ``rhs`` / ``pcg`` take and return torch tensors on the
+    device in the element-major flat layout (``(K*Np,)`` or ``(K, Np)``).
+    """
+
+    def __init__(self, mesh, bc_types, tau_scale=200.0, kappa=None, dtype=torch.float64):
+        if not torch.cuda.is_available():
+            raise RuntimeError("SipgOperator needs a HIP device (no CPU fallback)")
+        self.N, self.K, self.Np, self.Nfp = int(mesh.N), int(mesh.K), int(mesh.Np), int(mesh.Nfp)
+        self.dtype = dtype
+        self.code = _lib.DDGX_F64 if dtype == torch.float64 else _lib.DDGX_F32
+        if dtype not in (torch.float64, torch.float32):
+            raise ValueError("dtype must be torch.float64 or torch.float32")
+        rec, nbr, fsj = pack(mesh, bc_types, tau_scale, kappa)
+        tab = ref_element(self.N).kernel_tables()
+        L = _lib.lib()
+        assert tab.size == L.ddgx_table_size(self.N)
+        h = C.c_void_p()
+        torch.cuda.current_device()  # make sure the HIP context exists
+        _lib.check(L.ddgx_op_create(self.N, self.K, self.code,
+                                    tab.ctypes.data_as(C.c_void_p),
+                                    np.ascontiguousarray(rec).ctypes.data_as(C.c_void_p),
+                                    np.ascontiguousarray(nbr).ctypes.data_as(C.c_void_p),
+                                    np.ascontiguousarray(fsj).ctypes.data_as(C.c_void_p),
+                                    C.byref(h)), "ddgx_op_create")
+        self._h = h
+        self.device = torch.device("cuda", torch.cuda.current_device())
+        self._bj = {}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            try:
+                for pc in self._bj.values():
+                    pc.close()
+                _lib.lib().ddgx_op_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def _vec(self, t, name):
+        if t is None:
+            return None
+        if not (isinstance(t, torch.Tensor) and t.is_cuda):
+            raise TypeError(f"{name} must be a torch tensor on the HIP device")
+        if t.dtype != self.dtype or t.numel() != self.K * self.Np or not t.is_contiguous():
+            raise ValueError(f"{name} must be contiguous {self.dtype} with K*Np = "
+                             f"{self.K * self.Np} entries (got {t.dtype}, {t.numel()})")
+        return t
+
+    def new(self):
+        return torch.empty(self.K * self.Np, dtype=self.dtype, device=self.device)
+
+    def apply(self, u, y=None, *, u_dir=None, alpha=0.0, shift=0.0):
+        u = self._vec(u, "u")
+        y = self.new() if y is None else self._vec(y, "y")
+        u_dir = self._vec(u_dir, "u_dir")
+        _lib.check(_lib.lib().ddgx_sipg_apply(self._h, _ptr(u), _ptr(y), _ptr(u_dir),
+                                              float(alpha), float(shift), _stream()),
+                   "ddgx_sipg_apply")
+        return y
+
+    def rhs(self, f=None, *, u_dir=None, q_neu=None, out=None):
+        f = self._vec(f, "f")
+        u_dir = self._vec(u_dir, "u_dir")
+        if q_neu is not None:
+            if q_neu.dtype != self.dtype or q_neu.numel() != self.K * 3 * self.Nfp:
+                raise ValueError("q_neu must hold K*3*Nfp values of the operator dtype")
+        out = self.new() if out is None else self._vec(out, "out")
+        _lib.check(_lib.lib().ddgx_sipg_rhs(self._h, _ptr(f), _ptr(u_dir), _ptr(q_neu),
+                                            _ptr(out), _stream()), "ddgx_sipg_rhs")
+        return out
+
+    def dot(self, a, b):
+        out = C.c_double()
+        _lib.check(_lib.lib().ddgx_dot(self._h, _ptr(self._vec(a, "a")),
+                                       _ptr(self._vec(b, "b")), C.byref(out), _stream()),
+                   "ddgx_dot")
+        return out.value
+
+    def block_jacobi(self, alpha=0.0, dtype=None):
+        """Block-Jacobi preconditioner of A + alpha*M (needs a float64 operator)."""
+        dtype = self.dtype if dtype is None else dtype
+        key = (float(alpha), dtype)
+        if key not in self._bj:
+            self._bj[key] = BlockJacobi(self, alpha, dtype)
+        return self._bj[key]
+
+    def pcg(self, b, x=None, *, alpha=0.0, shift=0.0, pc=None, tol=1e-9, atol=0.0,
+            maxiter=1000, check_every=16):
+        """CG with JAX's stopping rule; returns ``(x, info_dict)``."""
+        b = self._vec(b, "b")
+        x = torch.zeros_like(b) if x is None else self._vec(x, "x")
+        cfg = _lib.CGConfig(float(tol), float(atol), int(maxiter), int(check_every))
+        info = _lib.CGInfo()
+        pch = None
+        if pc is not None:
+            if not isinstance(pc, BlockJacobi):
+                raise TypeError("pc must be a BlockJacobi of this operator")
+            pch = pc.handle
+        _lib.check(_lib.lib().ddgx_pcg_solve(self._h, float(alpha), float(shift), pch,
+                                             _ptr(b), _ptr(x), C.byref(cfg), C.byref(info),
+                                             _stream()), "ddgx_pcg_solve")
+        return x, {"iterations": info.iterations, "converged": bool(info.converged),
+                   "residual2": info.residual2, "atol2": info.atol2}
+
+
+class BlockJacobi:
+    """Per-element inverse diagonal blocks on the device (sipg.py:707-805)."""
+
+    def __init__(self, op64: SipgOperator, alpha=0.0, dtype=None):
+        if op64.dtype != torch.float64:
+            raise ValueError("block-Jacobi blocks are built from a float64 operator")
+        self.op = op64
+        self.alpha = float(alpha)
+        self.dtype = op64.dtype if dtype is None else dtype
+        code = _lib.DDGX_F64 if self.dtype == torch.float64 else _lib.DDGX_F32
+        h = C.c_void_p()
+        _lib.check(_lib.lib().ddgx_block_jacobi_create(op64.handle, self.alpha, code, _stream(),
+                                                       C.byref(h)), "ddgx_block_jacobi_create")
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            _lib.lib().ddgx_pc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def blocks(self):
+        K, Np = self.op.K, self.op.Np
+        out = torch.empty((K, Np, Np), dtype=torch.float64, device=self.op.device)
+        _lib.check(_lib.lib().ddgx_block_jacobi_blocks(self._h, _ptr(out), _stream()),
+                   "ddgx_block_jacobi_blocks")
+        return out
+
+    def __call__(self, r, z=None):
+        if r.dtype != self.dtype:
+            raise ValueError("vector dtype differs from the preconditioner dtype")
+        z = torch.empty_like(r) if z is None else z
+        _lib.check(_lib.lib().ddgx_pc_apply(self._h, _ptr(r), _ptr(z), _stream()),
+                   "ddgx_pc_apply")
+        return z

@@ -1,0 +1,286 @@
+"""Drop-in for ``ddg.dg2d.sipg``: SIPG Poisson/Helmholtz apply, loads and
+matrix-free CG solves, executed by the gfx950 HIP kernels of libddgx.
+
+Same names, keyword arguments, array shapes and error behaviour as
+``src/ddg/dg2d/sipg.py`` (BC codes 54-56, ``IterativeConfig`` 59-71,
+``sipg_apply_2d`` 304-337, loads 398-453, solves 536-612 and 812-918,
+block-Jacobi builders 718-805), so a caller can switch by import path.
+
+Inputs may be numpy arrays (copied to the device, results copied back as
+numpy) or torch tensors on the HIP device (results stay on the device).
+Operator handles are cached per (mesh, bc_types, tau_scale, dtype), so
+repeated calls do not repeat the setup that the reference recomputes on
+every call (sipg.py:412-413, 849-850).
+
+Differences from the reference, by design:
+* ``cub`` (curved-element cubature) is not supported: NotImplementedError.
+* ``iterative=None`` (direct LU) assembles the dense matrix from device
+  applies and factors it with torch on the device; like the reference it
+  is only practical for small meshes.
+* Solves return the solution; ``return_info=True`` also returns the CG
+  iteration count and final residual (JAX's ``cg`` returns ``info=None``).
+"""
+
+from __future__ import annotations
+
+import weakref
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from .operator import BlockJacobi, SipgOperator
+from .reference import ref_element
+
+BC_INTERIOR = 0
+BC_DIRICHLET = 1
+BC_NEUMANN = 2
+
+
+@dataclass(frozen=True)
+class IterativeConfig:
+    """Iterative solver configuration (sipg.py:59-71)."""
+    method: str = "cg"
+    tol: float = 1e-9
+    atol: float = 0.0
+    maxiter: int = 1000
+
+
+# ---------------------------------------------------------------------------
+# operator cache
+# ---------------------------------------------------------------------------
+_CACHE: "weakref.WeakKeyDictionary" = weakref.WeakKeyDictionary()
+
+
+def _key_of(bc_types, tau_scale, kappa, dtype):
+    bc = np.ascontiguousarray(np.asarray(bc_types, dtype=np.int32))
+    tau = np.ascontiguousarray(np.asarray(tau_scale, dtype=np.float64))
+    kap = None if kappa is None else np.ascontiguousarray(np.asarray(kappa, dtype=np.float64))
+    return (bc.tobytes(), bc.shape, tau.tobytes(), tau.shape,
+            None if kap is None else kap.tobytes(), str(dtype))
+
+
+def get_operator(mesh, bc_types, tau_scale=200.0, kappa=None, dtype=torch.float64) -> SipgOperator:
+    """Cached device operator for this mesh / BC tagging / penalty."""
+    try:
+        per_mesh = _CACHE.setdefault(mesh, {})
+    except TypeError:  # mesh not weak-referenceable: no caching
+        return SipgOperator(mesh, bc_types, tau_scale, kappa, dtype)
+    key = _key_of(bc_types, tau_scale, kappa, dtype)
+    op = per_mesh.get(key)
+    if op is None:
+        op = SipgOperator(mesh, bc_types, tau_scale, kappa, dtype)
+        per_mesh[key] = op
+    return op
+
+
+def _check_cub(cub):
+    if cub is not None:
+        raise NotImplementedError("curved-element cubature (cub=...) is not supported by the "
+                                  "MI355X operator; only affine elements")
+
+
+def _check_ref_mats(mesh, M_ref, m1d_per_face):
+    ref = ref_element(int(mesh.N))
+    if M_ref is not None and not np.allclose(np.asarray(M_ref), ref.M, rtol=1e-12, atol=1e-12):
+        raise ValueError("M_ref differs from the reference-element mass matrix")
+    if m1d_per_face is not None and not np.allclose(np.asarray(m1d_per_face), ref.m1d,
+                                                    rtol=1e-12, atol=1e-12):
+        raise ValueError("m1d_per_face differs from the reference-element face mass")
+
+
+class _Field:
+    """(Np, K) field <-> device flat element-major vector."""
+
+    def __init__(self, x, dtype, device, shape):
+        self.was_torch = isinstance(x, torch.Tensor)
+        self.shape = shape
+        if self.was_torch:
+            t = x.to(device=device, dtype=dtype)
+        else:
+            t = torch.as_tensor(np.asarray(x, dtype=np.float64), device=device).to(dtype)
+        self.flat = t.reshape(shape).T.contiguous().reshape(-1) if t.dim() == 2 else t.contiguous()
+
+    def back(self, flat, like_torch=None):
+        Np, K = self.shape
+        out = flat.reshape(K, Np).T
+        keep = self.was_torch if like_torch is None else like_torch
+        return out.contiguous() if keep else out.double().cpu().numpy()
+
+
+def _field(x, op):
+    return _Field(x, op.dtype, op.device, (op.Np, op.K))
+
+
+def _face_field(q, op):
+    """(3*Nfp, K) face array -> device flat (K, 3, Nfp) = F-order flat."""
+    t = q if isinstance(q, torch.Tensor) else torch.as_tensor(np.asarray(q, dtype=np.float64))
+    t = t.to(device=op.device, dtype=op.dtype)
+    return t.reshape(3 * op.Nfp, op.K).T.contiguous().reshape(-1)
+
+
+# ---------------------------------------------------------------------------
+# apply and loads
+# ---------------------------------------------------------------------------
+def sipg_apply_2d(mesh, u, *, bc_types, u_dir=None, tau_scale=200.0, M_ref=None,
+                  m1d_per_face=None, cub=None, kappa=None):
+    """``M (-Delta_h) u`` plus the Dirichlet-load offset when ``u_dir`` is
+    given (sipg.py:304-337).  ``kappa`` (K,) is an extension for
+    ``-div(kappa grad u)``; ``None`` is the reference operator."""
+    _check_cub(cub)
+    _check_ref_mats(mesh, M_ref, m1d_per_face)
+    op = get_operator(mesh, bc_types, tau_scale, kappa)
+    fu = _field(u, op)
+    ud = None if u_dir is None else _field(u_dir, op).flat
+    return fu.back(op.apply(fu.flat, u_dir=ud))
+
+
+def sipg_dirichlet_load_2d(mesh, u_dir, *, bc_types, tau_scale=200.0):
+    """A_aff(0; u_dir) (sipg.py:398-417)."""
+    op = get_operator(mesh, bc_types, tau_scale)
+    fd = _field(u_dir, op)
+    return fd.back(op.apply(torch.zeros_like(fd.flat), u_dir=fd.flat))
+
+
+def sipg_neumann_load_2d(mesh, q_neu, *, bc_types):
+    """Neumann surface load (sipg.py:420-453); ``q_neu`` is (3*Nfp, K)."""
+    op = get_operator(mesh, bc_types)
+    out = op.rhs(None, q_neu=_face_field(q_neu, op))
+    Np, K = op.Np, op.K
+    res = out.reshape(K, Np).T
+    return res.contiguous() if isinstance(q_neu, torch.Tensor) else res.double().cpu().numpy()
+
+
+def _rhs(op, source, u_dir, q_neu):
+    f = _field(source, op)
+    ud = None if u_dir is None else _field(u_dir, op).flat
+    q = None if q_neu is None else _face_field(q_neu, op)
+    return f, op.rhs(f.flat, u_dir=ud, q_neu=q)
+
+
+# ---------------------------------------------------------------------------
+# dense path (small meshes), sipg.py:344-391
+# ---------------------------------------------------------------------------
+def _dense(op, alpha=0.0, shift=0.0):
+    n = op.K * op.Np
+    A = torch.empty((n, n), dtype=torch.float64, device=op.device)
+    e = torch.zeros(n, dtype=op.dtype, device=op.device)
+    col = op.new()
+    for j in range(n):
+        e.zero_()
+        e[j] = 1.0
+        op.apply(e, col, alpha=alpha, shift=shift)
+        A[:, j] = col
+    return A
+
+
+def sipg_matrix_2d(mesh, *, bc_types, tau_scale=200.0, cub=None):
+    """Dense (Np K)^2 SIPG matrix from device applies (sipg.py:344-364)."""
+    _check_cub(cub)
+    return _dense(get_operator(mesh, bc_types, tau_scale)).cpu().numpy()
+
+
+def factor_sipg_2d(mesh, *, bc_types, tau_scale=200.0, regularize=False, eps=1.0e-8, cub=None):
+    """(A, lu, piv) with the LU factors on the device (sipg.py:367-391)."""
+    _check_cub(cub)
+    A = _dense(get_operator(mesh, bc_types, tau_scale))
+    if regularize:
+        A = A + eps * A.diagonal().abs().max() * torch.eye(A.shape[0], dtype=A.dtype, device=A.device)
+    lu, piv = torch.linalg.lu_factor(A)
+    return A, lu, piv
+
+
+def factor_sipg_helmholtz_2d(mesh, *, alpha, nu, bc_types, tau_scale=200.0, cub=None):
+    """alpha M + A factored (sipg.py:518-533); ``nu`` is unused as there."""
+    _check_cub(cub)
+    A = _dense(get_operator(mesh, bc_types, tau_scale), alpha=alpha)
+    lu, piv = torch.linalg.lu_factor(A)
+    return A, lu, piv
+
+
+def _lu_solve(factored, b):
+    _, lu, piv = factored
+    return torch.linalg.lu_solve(lu, piv, b.double().reshape(-1, 1)).reshape(-1)
+
+
+# ---------------------------------------------------------------------------
+# preconditioners (sipg.py:636-805)
+# ---------------------------------------------------------------------------
+def make_sipg_block_jacobi_preconditioner(mesh, *, bc_types, tau_scale=200.0, A_dense=None,
+                                          matfree=False) -> BlockJacobi:
+    """Block-Jacobi ``M^{-1}`` for the SIPG matrix.  Blocks are always built
+    on the device in closed form (the reference's matfree result); the
+    returned object is callable on device vectors and is accepted by the
+    solves as ``preconditioner``."""
+    return get_operator(mesh, bc_types, tau_scale).block_jacobi(0.0)
+
+
+def make_helmholtz_block_jacobi_preconditioner(mesh, *, alpha, nu, bc_types, tau_scale=200.0,
+                                               A_dense=None, matfree=False) -> BlockJacobi:
+    return get_operator(mesh, bc_types, tau_scale).block_jacobi(float(alpha))
+
+
+# ---------------------------------------------------------------------------
+# solves (sipg.py:536-612, 812-918)
+# ---------------------------------------------------------------------------
+def _run_cg(op, b, x0, iterative, preconditioner, alpha=0.0, shift=0.0):
+    if iterative.method != "cg":
+        raise ValueError(f"iterative method {iterative.method!r} not supported for SPD")
+    pc = None
+    if preconditioner is not None:
+        if not isinstance(preconditioner, BlockJacobi):
+            raise TypeError("preconditioner must come from make_*_block_jacobi_preconditioner "
+                            "or ddg_amd.multigrid")
+        if preconditioner.op.K != op.K or preconditioner.op.N != op.N:
+            raise ValueError("preconditioner was built for a different mesh")
+        pc = preconditioner
+    return op.pcg(b, x0, alpha=alpha, shift=shift, pc=pc, tol=iterative.tol,
+                  atol=iterative.atol, maxiter=iterative.maxiter)
+
+
+def solve_sipg_poisson_2d(mesh, source, *, bc_types, u_dir=None, q_neu=None, tau_scale=200.0,
+                          A_factored=None, iterative=None, x0=None, preconditioner=None,
+                          cub=None, return_info=False):
+    """Solve ``-Delta u = source`` with mixed BCs (sipg.py:812-918)."""
+    _check_cub(cub)
+    op = get_operator(mesh, bc_types, tau_scale)
+    f, b = _rhs(op, source, u_dir, q_neu)
+    if iterative is None:
+        if A_factored is None:
+            A_factored = factor_sipg_2d(mesh, bc_types=bc_types, tau_scale=tau_scale)
+        x = _lu_solve(A_factored, b).to(op.dtype)
+        u = f.back(x)
+        return (u, None) if return_info else u
+    shift = 0.0
+    if not np.any(np.asarray(bc_types) == BC_DIRICHLET):
+        # singular pure-Neumann operator: eps-shift from one probe apply and a
+        # mean-zero right-hand side (sipg.py:885-906)
+        n = op.K * op.Np
+        probe = torch.sin(torch.arange(n, dtype=torch.float64, device=op.device)).to(op.dtype)
+        probe = probe - probe.mean()
+        Ap = op.apply(probe)
+        shift = 1.0e-6 * float(torch.linalg.norm(Ap) / torch.linalg.norm(probe))
+        b = b - b.mean()
+    xi = None if x0 is None else _field(x0, op).flat.clone()
+    x, info = _run_cg(op, b, xi, iterative, preconditioner, 0.0, shift)
+    u = f.back(x)
+    return (u, info) if return_info else u
+
+
+def solve_sipg_helmholtz_2d(mesh, rhs_field, *, alpha, nu, bc_types, u_dir=None, q_neu=None,
+                            tau_scale=200.0, A_factored=None, iterative=None, x0=None,
+                            preconditioner=None, cub=None, return_info=False):
+    """Solve ``(alpha M - Delta) u = rhs_field`` (sipg.py:536-612)."""
+    _check_cub(cub)
+    op = get_operator(mesh, bc_types, tau_scale)
+    f, b = _rhs(op, rhs_field, u_dir, q_neu)
+    if iterative is None:
+        if A_factored is None:
+            A_factored = factor_sipg_helmholtz_2d(mesh, alpha=alpha, nu=nu, bc_types=bc_types,
+                                                  tau_scale=tau_scale)
+        u = f.back(_lu_solve(A_factored, b).to(op.dtype))
+        return (u, None) if return_info else u
+    xi = None if x0 is None else _field(x0, op).flat.clone()
+    x, info = _run_cg(op, b, xi, iterative, preconditioner, float(alpha), 0.0)
+    u = f.back(x)
+    return (u, info) if return_info else u

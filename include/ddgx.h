@@ -1,0 +1,122 @@
+/*
+ * ddgx -- MI355X (gfx950) SIPG Poisson/Helmholtz operator and CG solver.
+ *
+ * Plain C ABI: opaque handles, plain pointers and sizes, int status codes.
+ * Device buffers are caller-owned (e.g. torch tensors on the HIP device);
+ * the library allocates device memory only inside handles, which are freed
+ * by the matching *_destroy call.  Every call that launches work takes a
+ * hipStream_t (passed as void*) and is ordered on it.
+ *
+ * Vector layout: element-major, ``x[k*Np + i]`` -- the reference's flat
+ * ``u.T.reshape(-1)`` of an (Np, K) field (src/ddg/dg2d/sipg.py:204, 864).
+ * Face data (Neumann values) are ``q[k*3*Nfp + f*Nfp + p]`` -- the
+ * reference's (3*Nfp, K) array flattened in F order (sipg.py:211-214).
+ *
+ * The reference is pure Python/JAX and has no FFI of its own; each entry
+ * point below names the reference function it stands in for.  The Python
+ * binding that calls them is ddg_amd/_lib.py (ctypes).
+ */
+#ifndef DDGX_H
+#define DDGX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  DDGX_OK = 0,
+  DDGX_ERR_INVALID = 1,      /* bad argument, shape or handle            */
+  DDGX_ERR_UNSUPPORTED = 2,  /* polynomial order / dtype not built       */
+  DDGX_ERR_HIP = 3,          /* a HIP runtime call failed                */
+  DDGX_ERR_NOMEM = 4         /* device allocation failed                 */
+};
+
+enum { DDGX_F64 = 0, DDGX_F32 = 1 };
+
+/* Number of words in one element's geometry record and table length for
+ * order N (float64 on the host side). */
+#define DDGX_REC_WORDS 16
+int ddgx_table_size(int N);
+
+typedef struct ddgx_op_s* ddgx_op;
+typedef struct ddgx_pc_s* ddgx_pc;
+
+typedef struct {
+  double tol;       /* relative tolerance on ||r|| / ||b||   (IterativeConfig.tol) */
+  double atol;      /* absolute tolerance                     (IterativeConfig.atol) */
+  int maxiter;      /* iteration cap                          (IterativeConfig.maxiter) */
+  int check_every;  /* iterations between host polls of the device stop flag (>=1) */
+} ddgx_cg_cfg;
+
+typedef struct {
+  int iterations;    /* CG iterations executed                              */
+  int converged;     /* 1 if the stopping test was met before maxiter        */
+  double residual2;  /* final value tested: <r,z> without PC, <r,r> with PC  */
+  double atol2;      /* threshold max(tol^2 <b,b>, atol^2)                   */
+} ddgx_cg_info;
+
+/* Last error message of the calling thread (static storage). */
+const char* ddgx_last_error(void);
+
+/* Operator handle: one SIPG operator (mesh geometry + BC tags + penalty +
+ * optional per-element coefficient) at order N in precision dtype.
+ * Replaces the per-call setup of sipg_apply_2d / _bc_face_arrays /
+ * _local_mass / _face_1d_mass_matrices (src/ddg/dg2d/sipg.py:78-139,
+ * 304-337).  Host inputs (float64 / int32):
+ *   tables  ddgx_table_size(N) reference-element constants
+ *   rec     K x 16 per-element geometry/penalty record
+ *   nbr     K x 4  face codes: (k'<<3)|(f'<<1)|reversed, -1 Dirichlet,
+ *                  -2 no coupling (Neumann)
+ *   fsj     K x 4  face Jacobian sJ on Neumann faces, 0 elsewhere (used by
+ *                  the Neumann load only)                              */
+int ddgx_op_create(int N, int64_t K, int dtype, const double* tables,
+                   const double* rec, const int32_t* nbr, const double* fsj,
+                   ddgx_op* out);
+int ddgx_op_destroy(ddgx_op op);
+int ddgx_op_info(ddgx_op op, int* N, int64_t* K, int* dtype);
+
+/* y = (A + alpha*M + shift*I) u  with Dirichlet ghost values u_dir on
+ * Dirichlet faces (u_dir may be NULL: homogeneous).  Stands in for
+ * sipg_apply_2d (sipg.py:304-337) and the Helmholtz/multigrid applies
+ * (sipg.py:589-601, multigrid.py:53-74).  u, y, u_dir: device, op dtype. */
+int ddgx_sipg_apply(ddgx_op op, const void* u, void* y, const void* u_dir,
+                    double alpha, double shift, void* stream);
+
+/* rhs = J M f - L_D(u_dir) + L_N(q_neu)   (sipg.py:849-864, 398-453).
+ * f, u_dir (NULL = none), q_neu (NULL = none), rhs: device, op dtype. */
+int ddgx_sipg_rhs(ddgx_op op, const void* f, const void* u_dir,
+                  const void* q_neu, void* rhs, void* stream);
+
+/* Block-Jacobi preconditioner of A + alpha*M: per-element diagonal blocks
+ * built on the device from a float64 operator, inverted in float64, and
+ * stored in ``dtype`` (the precision of the vectors it is applied to).
+ * Replaces make_sipg_block_jacobi_preconditioner(matfree=True) /
+ * make_helmholtz_block_jacobi_preconditioner (sipg.py:671-805). */
+int ddgx_block_jacobi_create(ddgx_op op64, double alpha, int dtype, void* stream,
+                             ddgx_pc* out);
+/* Copy the (K, Np, Np) diagonal blocks (before inversion) to a device buffer. */
+int ddgx_block_jacobi_blocks(ddgx_pc pc, void* blocks, void* stream);
+/* z = M^{-1} r  (the preconditioner callable, sipg.py:711-713). */
+int ddgx_pc_apply(ddgx_pc pc, const void* r, void* z, void* stream);
+int ddgx_pc_destroy(ddgx_pc pc);
+
+/* Preconditioned CG on (A + alpha*M + shift*I) x = b with the stopping rule
+ * of jax.scipy.sparse.linalg.cg as called at sipg.py:603-609, 909-915.
+ * x holds x0 on entry and the solution on exit.  pc may be NULL.
+ * Iterations run without host synchronisation; the host polls a device
+ * stop flag every cfg->check_every iterations. */
+int ddgx_pcg_solve(ddgx_op op, double alpha, double shift, ddgx_pc pc,
+                   const void* b, void* x, const ddgx_cg_cfg* cfg,
+                   ddgx_cg_info* info, void* stream);
+
+/* Deterministic dot product in float64 of two device vectors of the op's
+ * dtype and length K*Np; result written to *out (host). */
+int ddgx_dot(ddgx_op op, const void* a, const void* b, double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DDGX_H */

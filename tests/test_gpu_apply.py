@@ -1,0 +1,119 @@
+"""HIP apply / RHS parity against the oracle (float64 bar: 1e-12 relative in
+max norm; float32 bar: 2e-5), plus size-independent properties at the
+2x256x256, N=4 workload size."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import unit_square, rel
+from kernel_model import model_apply
+from oracle import sipg as S
+from ddg_amd import mesh as pmesh
+from ddg_amd.operator import SipgOperator, pack
+from ddg_amd.reference import ref_element
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _flat(u, dev, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(u.T).reshape(-1), device=dev).to(dtype)
+
+
+def _field(t, Np, K):
+    return t.double().cpu().numpy().reshape(K, Np).T
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("bcname", ["dir", "mixed"])
+def test_apply_parity_f64(dev, N, bcname):
+    pm, om = unit_square(5, 3, N, jitter=0.1, seed=N)
+    bc = pmesh.all_dirichlet(pm) if bcname == "dir" else pmesh.dirichlet_x_neumann_y(pm)
+    rng = np.random.default_rng(10 * N)
+    for kw in ({}, {"kappa": np.exp(0.5 * rng.standard_normal(pm.K)),
+                    "tau_scale": 200 * (1 + rng.random(pm.K))}):
+        _say(f"apply f64 N={N} bc={bcname} K={pm.K} kw={sorted(kw)}")
+        op = SipgOperator(pm, bc, **kw)
+        u = rng.standard_normal((pm.Np, pm.K))
+        ud = rng.standard_normal((pm.Np, pm.K))
+        y = _field(op.apply(_flat(u, dev)), pm.Np, pm.K)
+        assert rel(y, S.sipg_apply_2d(om, u, bc_types=bc, **kw)) < 1e-12
+        y = _field(op.apply(_flat(u, dev), u_dir=_flat(ud, dev)), pm.Np, pm.K)
+        assert rel(y, S.sipg_apply_2d(om, u, bc_types=bc, u_dir=ud, **kw)) < 1e-12
+        y = _field(op.apply(_flat(u, dev), alpha=3.5, shift=0.25), pm.Np, pm.K)
+        ref = S.sipg_apply_2d(om, u, bc_types=bc, **kw) + 3.5 * S.mass_apply(om, u) + 0.25 * u
+        assert rel(y, ref) < 1e-12
+
+
+@pytest.mark.parametrize("N", [3, 4, 6])
+def test_apply_parity_f32(dev, N):
+    pm, om = unit_square(6, 6, N, jitter=0.1, seed=3)
+    bc = pmesh.dirichlet_x_neumann_y(pm)
+    _say(f"apply f32 N={N} K={pm.K}")
+    op = SipgOperator(pm, bc, dtype=torch.float32)
+    u = np.random.default_rng(0).standard_normal((pm.Np, pm.K))
+    y = _field(op.apply(_flat(u, dev, torch.float32)), pm.Np, pm.K)
+    assert rel(y, S.sipg_apply_2d(om, u, bc_types=bc)) < 2e-5
+
+
+def test_apply_config0_mesh(dev):
+    # 2x16x16, N=3, all Dirichlet, tau_scale=200: the CPU-runnable workload
+    pm, om = unit_square(16, 16, 3)
+    bc = pmesh.all_dirichlet(pm)
+    _say(f"apply config0 K={pm.K}")
+    op = SipgOperator(pm, bc)
+    u = np.sin(np.pi * om.x) * np.sin(np.pi * om.y)
+    y = _field(op.apply(_flat(u, dev)), pm.Np, pm.K)
+    assert rel(y, S.sipg_apply_2d(om, u, bc_types=bc)) < 1e-12
+
+
+def test_rhs_parity(dev):
+    pm, om = unit_square(6, 5, 4, jitter=0.1, seed=5)
+    bc = pmesh.dirichlet_x_neumann_y(pm)
+    rng = np.random.default_rng(2)
+    f = rng.standard_normal((pm.Np, pm.K))
+    ud = rng.standard_normal((pm.Np, pm.K))
+    q = rng.standard_normal((3 * pm.Nfp, pm.K))
+    _say(f"rhs K={pm.K}")
+    op = SipgOperator(pm, bc)
+    qd = torch.as_tensor(np.ascontiguousarray(q.T).reshape(-1), device=dev)
+    got = _field(op.rhs(_flat(f, dev), u_dir=_flat(ud, dev), q_neu=qd), pm.Np, pm.K)
+    ref = (S.mass_apply(om, f) - S.sipg_dirichlet_load_2d(om, ud, bc_types=bc)
+           + S.sipg_neumann_load_2d(om, q, bc_types=bc))
+    assert rel(got, ref) < 1e-12
+
+
+def test_large_mesh_properties(dev):
+    """2x256x256, N=4, jittered, Dirichlet x / Neumann y: exact agreement
+    with the numpy model of the same closed form, symmetry u.Av == v.Au,
+    and linearity."""
+    VX, VY, EToV = pmesh.uniform_rectangle_mesh_2d(0, 1, 0, 1, 256, 256)
+    VX, VY = pmesh.jitter_vertices(VX, VY, 1 / 256, 0.1, seed=0)
+    m = pmesh.build_mesh_2d(VX, VY, EToV, 4)
+    bc = pmesh.dirichlet_x_neumann_y(m)
+    _say(f"apply large K={m.K}")
+    op = SipgOperator(m, bc)
+    rng = np.random.default_rng(1)
+    u = rng.standard_normal((m.K, m.Np))
+    v = rng.standard_normal((m.K, m.Np))
+    ut = torch.as_tensor(u.reshape(-1), device=dev)
+    vt = torch.as_tensor(v.reshape(-1), device=dev)
+    Au, Av = op.apply(ut), op.apply(vt)
+    rec, nbr, _ = pack(m, bc)
+    ref = model_apply(4, ref_element(4).kernel_tables(), rec, nbr, u)
+    assert rel(Au.cpu().numpy().reshape(m.K, m.Np), ref) < 1e-12
+    a, b = float(vt @ Au), float(ut @ Av)
+    assert abs(a - b) <= 1e-11 * abs(a)
+    w = op.apply(2.0 * ut - 3.0 * vt)
+    assert float((w - (2.0 * Au - 3.0 * Av)).abs().max()) <= 1e-11 * float(Au.abs().max())
+
+
+def test_repeatable(dev):
+    pm, _ = unit_square(32, 32, 4)
+    op = SipgOperator(pm, pmesh.all_dirichlet(pm))
+    u = torch.randn(pm.K * pm.Np, dtype=torch.float64, device=dev)
+    _say("apply repeatable")
+    assert torch.equal(op.apply(u), op.apply(u))

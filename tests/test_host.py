@@ -1,0 +1,96 @@
+"""Host-side logic without a GPU: record packing + the kernel's closed form
+(numpy model) against the oracle, validation errors, and the C ABI exports."""
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from conftest import unit_square, rel
+from kernel_model import model_apply
+from oracle import sipg as S
+from ddg_amd import mesh as pmesh
+from ddg_amd.operator import pack
+from ddg_amd.reference import ref_element
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6])
+def test_packed_closed_form_matches_oracle(N):
+    pm, om = unit_square(4, 3, N, jitter=0.1, seed=N)
+    rng = np.random.default_rng(N)
+    tab = ref_element(N).kernel_tables()
+    for bc in (pmesh.all_dirichlet(pm), pmesh.dirichlet_x_neumann_y(pm)):
+        for kw in ({}, {"kappa": np.exp(0.5 * rng.standard_normal(pm.K)),
+                        "tau_scale": 200 * (1 + rng.random(pm.K))}):
+            rec, nbr, _ = pack(pm, bc, **kw)
+            u = rng.standard_normal((pm.Np, pm.K))
+            ud = rng.standard_normal((pm.Np, pm.K))
+            y = model_apply(N, tab, rec, nbr, u.T.copy(), udir=ud.T.copy(), alpha=2.5).T
+            yo = S.sipg_apply_2d(om, u, bc_types=bc, u_dir=ud, **kw) + 2.5 * S.mass_apply(om, u)
+            assert rel(y, yo) < 1e-12
+
+
+def test_probe_mode_gives_diagonal_blocks():
+    pm, om = unit_square(3, 3, 3)
+    bc = pmesh.dirichlet_x_neumann_y(pm)
+    rec, nbr, _ = pack(pm, bc)
+    tab = ref_element(3).kernel_tables()
+    blocks = np.stack([model_apply(3, tab, rec, nbr, np.tile(np.eye(pm.Np)[j], (pm.K, 1)),
+                                   probe=True) for j in range(pm.Np)], axis=2)
+    ref = S.extract_block_diag_matfree(S.flat_apply(om, bc), om)
+    assert rel(blocks, ref) < 1e-12
+
+
+def test_asymmetric_interior_tag_rejected():
+    pm, _ = unit_square(2, 2, 2)
+    bc = pmesh.all_dirichlet(pm)
+    k = int(np.argmax((pm.EToE != np.arange(pm.K)[:, None]).any(1)))
+    f = int(np.argmax(pm.EToE[k] != k))
+    bc[k, f] = 1
+    with pytest.raises(ValueError):
+        pack(pm, bc)
+
+
+def test_bad_bc_shape_rejected():
+    pm, _ = unit_square(2, 2, 2)
+    with pytest.raises(ValueError):
+        pack(pm, np.zeros((pm.K, 2), dtype=np.int32))
+
+
+def test_boundary_face_tagged_interior_is_no_coupling():
+    # the reference gives such a face zero jump and cancelling averages
+    pm, om = unit_square(3, 3, 2)
+    bc = pmesh.all_dirichlet(pm)
+    bc[pmesh.boundary_faces(pm) & (pmesh.face_normals(pm)[1] > 0.5)] = 0
+    rec, nbr, _ = pack(pm, bc)
+    u = np.random.default_rng(0).standard_normal((pm.Np, pm.K))
+    y = model_apply(2, ref_element(2).kernel_tables(), rec, nbr, u.T.copy()).T
+    assert rel(y, S.sipg_apply_2d(om, u, bc_types=bc)) < 1e-12
+
+
+def _header_functions():
+    txt = (ROOT / "include" / "ddgx.h").read_text()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(ddgx_\w+)\s*\(", txt, re.M)))
+
+
+def test_library_exports_every_header_symbol():
+    lib = ROOT / "ddg_amd" / "libddgx.so"
+    if not lib.exists():
+        pytest.fail("libddgx.so not built: run `python -m ddg_amd.build`")
+    L = ctypes.CDLL(str(lib))
+    names = _header_functions()
+    assert len(names) >= 12
+    for n in names:
+        assert hasattr(L, n), n
+    from ddg_amd import _lib
+    assert set(names) == set(_lib.EXPORTS)
+
+
+def test_table_size_matches_python():
+    from ddg_amd import _lib
+    L = _lib.lib()
+    for N in range(1, 7):
+        assert L.ddgx_table_size(N) == ref_element(N).kernel_tables().size

@@ -66,7 +66,7 @@ constexpr int CODE_DIRICHLET = -1;
 constexpr int CODE_NONE = -2;
 
 // Kernel mode bits for the apply
-enum : int { M_HELM = 1, M_UDIR = 2, M_PROBE = 4, M_DOT = 8 };
+enum : int { M_HELM = 1, M_UDIR = 2, M_PROBE = 4, M_DOT = 8, M_RESID = 16 };
 
 // Device scalars of one CG solve
 struct CGScalars {
@@ -95,11 +95,40 @@ struct Op {
   CGScalars* scal_host = nullptr;  // pinned
 };
 
+struct MG;
+
 struct PC {
-  Op* op = nullptr;          // the float64 operator the blocks came from
+  int type = 0;              // 0 block-Jacobi, 1 multigrid V-cycle
+  Op* op = nullptr;          // BJ: the float64 operator the blocks came from
   int dtype = 0;             // precision of binvT and of the vectors
-  void* binvT = nullptr;   // K x Np x Np, transposed inverse blocks (dtype)
+  void* binvT = nullptr;     // K x Np x Np, transposed inverse blocks (dtype)
   double* blocks = nullptr;  // K x Np x Np float64 blocks (kept for inspection)
+  MG* mg = nullptr;          // type 1
+  int64_t K = 0;             // vector shape the PC applies to
+  int N = 0;
+};
+
+// One level of a multigrid hierarchy (fine -> coarse); the last level is
+// solved directly with a dense float64 inverse.
+struct MGLevel {
+  Op* op = nullptr;        // operator at this level (MG dtype)
+  void* binvT = nullptr;   // block-Jacobi inverse blocks of this level (MG dtype)
+  double alpha = 0.0;      // Helmholtz coefficient of this level's operator
+  int tkind = 0;           // transfer to the next-coarser level: 0 = p, 1 = h
+  int np_lo = 0;           // Np of the next-coarser level (p transfer)
+  void* P = nullptr;       // p: Np x np_lo matrix; h: 4 x Np x Np child matrices (MG dtype)
+  void* b = nullptr;       // level right-hand side / solution / residual (MG dtype)
+  void* x = nullptr;
+  void* r = nullptr;
+};
+
+struct MG {
+  int dtype = 0, nu = 2;
+  double omega = 0.7;
+  MGLevel* lv = nullptr;
+  int nlev = 0;
+  double* Ainv = nullptr;  // coarsest level dense inverse (float64, symmetric)
+  int nc = 0;
 };
 
 void set_error(const std::string& msg);
@@ -114,7 +143,9 @@ int hip_fail(hipError_t e, const char* what);
 // launchers (defined per translation unit)
 int launch_apply(const Op& op, const void* u, void* y, const void* udir, double alpha,
                  double shift, int mode, int probe_j, double* partial, const int* skip,
-                 hipStream_t s);
+                 hipStream_t s, const void* bvec = nullptr);
+int pc_apply(PC& pc, const void* r, void* z, hipStream_t s);
+int mg_vcycle(MG& mg, const void* b, void* x, hipStream_t s);
 int launch_rhs(const Op& op, const void* f, const void* udir, const void* q, void* rhs,
                hipStream_t s);
 int launch_invert_blocks(int N, int64_t K, int dtype, const double* blocks, void* binvT,

@@ -31,11 +31,11 @@ sipg_apply_kernel(const T* __restrict__ tab, const T* __restrict__ rec,
                   const int* __restrict__ nbr, const T* __restrict__ u,
                   T* __restrict__ y, const T* __restrict__ udir, T alpha, T shift,
                   int K, int probe_j, double* __restrict__ partial,
-                  const int* __restrict__ skip) {
+                  const int* __restrict__ skip, const T* __restrict__ bvec) {
   using S = Shape<N>;
   constexpr int Np = S::Np, Nfp = S::Nfp, NL = S::NL, EPB = S::EPB;
   constexpr bool HELM = MODE & M_HELM, UDIR = MODE & M_UDIR;
-  constexpr bool PROBE = MODE & M_PROBE, DOT = MODE & M_DOT;
+  constexpr bool PROBE = MODE & M_PROBE, DOT = MODE & M_DOT, RESID = MODE & M_RESID;
 
   if (skip != nullptr && *skip) return;  // CG already converged
 
@@ -184,6 +184,8 @@ sipg_apply_kernel(const T* __restrict__ tab, const T* __restrict__ rec,
       }
       if constexpr (PROBE)
         y[((size_t)(k0 + e) * Np + li) * Np + probe_j] = acc;
+      else if constexpr (RESID)
+        y[(size_t)(k0 + e) * Np + li] = bvec[(size_t)(k0 + e) * Np + li] - acc;
       else
         y[(size_t)(k0 + e) * Np + li] = acc;
       if constexpr (DOT) dot_acc += (double)s_u[e * Np + li] * (double)acc;
@@ -202,14 +204,15 @@ sipg_apply_kernel(const T* __restrict__ tab, const T* __restrict__ rec,
 template <typename T, int N, int MODE>
 static int launch_t(const Op& op, const void* u, void* y, const void* udir, double alpha,
                     double shift, int probe_j, double* partial, const int* skip,
-                    hipStream_t s) {
+                    hipStream_t s, const void* bvec) {
   const int K = (int)op.K;
   const int ntiles = (K + Shape<N>::EPB - 1) / Shape<N>::EPB;
   int grid = op.grid_apply > 0 ? op.grid_apply : 1;
   if (!(MODE & M_DOT)) grid = ntiles < grid ? ntiles : grid;
   hipLaunchKernelGGL((sipg_apply_kernel<T, N, MODE>), dim3(grid), dim3(256), 0, s,
                      (const T*)op.tab, (const T*)op.rec, op.nbr, (const T*)u, (T*)y,
-                     (const T*)udir, (T)alpha, (T)shift, K, probe_j, partial, skip);
+                     (const T*)udir, (T)alpha, (T)shift, K, probe_j, partial, skip,
+                     (const T*)bvec);
   DDGX_HIP(hipGetLastError());
   return DDGX_OK;
 }
@@ -217,11 +220,11 @@ static int launch_t(const Op& op, const void* u, void* y, const void* udir, doub
 template <typename T, int N>
 static int launch_mode(const Op& op, const void* u, void* y, const void* udir, double alpha,
                        double shift, int mode, int probe_j, double* partial, const int* skip,
-                       hipStream_t s) {
+                       hipStream_t s, const void* bvec) {
   switch (mode) {
 #define CASE(m) \
-  case m: return launch_t<T, N, m>(op, u, y, udir, alpha, shift, probe_j, partial, skip, s);
-    CASE(0) CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(8) CASE(9)
+  case m: return launch_t<T, N, m>(op, u, y, udir, alpha, shift, probe_j, partial, skip, s, bvec);
+    CASE(0) CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(8) CASE(9) CASE(16) CASE(17)
 #undef CASE
     default:
       set_error("unsupported apply mode " + std::to_string(mode));
@@ -231,17 +234,21 @@ static int launch_mode(const Op& op, const void* u, void* y, const void* udir, d
 
 int launch_apply(const Op& op, const void* u, void* y, const void* udir, double alpha,
                  double shift, int mode, int probe_j, double* partial, const int* skip,
-                 hipStream_t s) {
+                 hipStream_t s, const void* bvec) {
+  if ((mode & M_RESID) && bvec == nullptr) {
+    set_error("residual mode needs the right-hand side");
+    return DDGX_ERR_INVALID;
+  }
   if (alpha != 0.0 || shift != 0.0) mode |= M_HELM;
   if (udir != nullptr) mode |= M_UDIR;
 #define DISPATCH_N(T)                                                                      \
   switch (op.N) {                                                                          \
-    case 1: return launch_mode<T, 1>(op, u, y, udir, alpha, shift, mode, probe_j, partial, skip, s); \
-    case 2: return launch_mode<T, 2>(op, u, y, udir, alpha, shift, mode, probe_j, partial, skip, s); \
-    case 3: return launch_mode<T, 3>(op, u, y, udir, alpha, shift, mode, probe_j, partial, skip, s); \
-    case 4: return launch_mode<T, 4>(op, u, y, udir, alpha, shift, mode, probe_j, partial, skip, s); \
-    case 5: return launch_mode<T, 5>(op, u, y, udir, alpha, shift, mode, probe_j, partial, skip, s); \
-    case 6: return launch_mode<T, 6>(op, u, y, udir, alpha, shift, mode, probe_j, partial, skip, s); \
+    case 1: return launch_mode<T, 1>(op, u, y, udir, alpha, shift, mode, probe_j, partial, skip, s, bvec); \
+    case 2: return launch_mode<T, 2>(op, u, y, udir, alpha, shift, mode, probe_j, partial, skip, s, bvec); \
+    case 3: return launch_mode<T, 3>(op, u, y, udir, alpha, shift, mode, probe_j, partial, skip, s, bvec); \
+    case 4: return launch_mode<T, 4>(op, u, y, udir, alpha, shift, mode, probe_j, partial, skip, s, bvec); \
+    case 5: return launch_mode<T, 5>(op, u, y, udir, alpha, shift, mode, probe_j, partial, skip, s, bvec); \
+    case 6: return launch_mode<T, 6>(op, u, y, udir, alpha, shift, mode, probe_j, partial, skip, s, bvec); \
     default: break;                                                                        \
   }
   if (op.dtype == DDGX_F64) {

@@ -209,3 +209,46 @@ def all_dirichlet(mesh) -> np.ndarray:
 def dirichlet_x_neumann_y(mesh) -> np.ndarray:
     """Dirichlet on x = const walls, Neumann on y = const walls."""
     return bc_by_normal(mesh, lambda nx, ny: np.abs(nx) > 0.5, lambda nx, ny: np.abs(ny) > 0.5)
+
+
+def refine_uniform(VX, VY, EToV):
+    """One uniform red refinement with the child numbering of the reference's
+    refinement forest (refinement.py:359-382, 389-455): fine element 4k+c is
+    child c of element k, children (v0,m01,m20), (m01,v1,m12), (m20,m12,v2),
+    (m01,m12,m20).  Midpoint vertices are numbered in order of first use
+    (element order, edges 01, 12, 20).  Returns (VX, VY, EToV)."""
+    VX = np.asarray(VX, dtype=np.float64)
+    VY = np.asarray(VY, dtype=np.float64)
+    EToV = np.asarray(EToV, dtype=np.int64)
+    K, nv = EToV.shape[0], VX.size
+    e = np.stack([EToV[:, [0, 1]], EToV[:, [1, 2]], EToV[:, [2, 0]]], axis=1).reshape(-1, 2)
+    key = np.sort(e, axis=1)
+    key = key[:, 0] * nv + key[:, 1]
+    uniq, first, inv = np.unique(key, return_index=True, return_inverse=True)
+    order = np.argsort(first, kind="stable")           # unique edges by first use
+    rank = np.empty_like(order)
+    rank[order] = np.arange(order.size)
+    mid = nv + rank[inv].reshape(K, 3)                 # (K,3): m01, m12, m20
+    ea = e.reshape(K, 3, 2)
+    first_edges = ea.reshape(-1, 2)[first[order]]
+    VX2 = np.concatenate([VX, 0.5 * (VX[first_edges[:, 0]] + VX[first_edges[:, 1]])])
+    VY2 = np.concatenate([VY, 0.5 * (VY[first_edges[:, 0]] + VY[first_edges[:, 1]])])
+    v0, v1, v2 = EToV[:, 0], EToV[:, 1], EToV[:, 2]
+    m01, m12, m20 = mid[:, 0], mid[:, 1], mid[:, 2]
+    kids = np.stack([np.stack([v0, m01, m20], 1), np.stack([m01, v1, m12], 1),
+                     np.stack([m20, m12, v2], 1), np.stack([m01, m12, m20], 1)], axis=1)
+    return VX2, VY2, kids.reshape(4 * K, 3)
+
+
+def forest_rectangle_hierarchy(n_base: int, levels: int, bounds=(0.0, 1.0, 0.0, 1.0)):
+    """Vertex/element arrays of a structured ``2*n_base^2`` rectangle mesh
+    refined uniformly ``levels`` times: a list ``[(VX, VY, EToV), ...]``
+    from coarsest to finest, element 4k+c of level l+1 being child c of
+    element k of level l.  Vertex ids of a coarse level are kept on every
+    finer level, so vertex moves made on the finest level can be carried to
+    the coarse ones by truncation."""
+    xmin, xmax, ymin, ymax = bounds
+    out = [uniform_rectangle_mesh_2d(xmin, xmax, ymin, ymax, n_base, n_base)]
+    for _ in range(levels):
+        out.append(refine_uniform(*out[-1]))
+    return out

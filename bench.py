@@ -32,9 +32,19 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 
 
-def workload_mesh(n=256, N=4, seed=0):
+def workload_mesh(n=256, N=4, seed=0, order="forest"):
+    """2 n^2 triangles of [0,1]^2, interior vertices jittered by U(-0.1h, 0.1h).
+    order="forest": the structured mesh generated by uniform red refinement of
+    a 2x2 cell mesh (quadtree element order, element 4k+c = child c of k);
+    order="rowmajor": the reference's structured builder order."""
     from ddg_amd import mesh as pm
-    VX, VY, EToV = pm.uniform_rectangle_mesh_2d(0.0, 1.0, 0.0, 1.0, n, n)
+    if order == "forest":
+        lv = int(round(np.log2(n))) - 1
+        if 2 ** (lv + 1) != n:
+            raise ValueError("forest order needs n a power of two")
+        VX, VY, EToV = pm.forest_rectangle_hierarchy(2, lv)[-1]
+    else:
+        VX, VY, EToV = pm.uniform_rectangle_mesh_2d(0.0, 1.0, 0.0, 1.0, n, n)
     VX, VY = pm.jitter_vertices(VX, VY, 1.0 / n, 0.1, seed)
     m = pm.build_mesh_2d(VX, VY, EToV, N)
     return m, pm.dirichlet_x_neumann_y(m)
@@ -57,20 +67,35 @@ def apply_bytes(K, Np, s):
 
 
 def time_apply(op, u, y, steps, warmup, **kw):
+    """Average device time of one apply.  The applies are captured once into
+    a HIP graph and replayed, so host launch overhead is not in the timed
+    region; HIP events on the launch stream bracket the replays."""
     import torch
     for _ in range(warmup):
         op.apply(u, y, **kw)
     torch.cuda.synchronize()
-    s = torch.cuda.current_stream()
+    reps = 10
+    graph = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(graph, stream=s):
+            for _ in range(reps):
+                op.apply(u, y, **kw)
+    torch.cuda.synchronize()
+    graph.replay()
+    torch.cuda.synchronize()
+    n_rep = max(1, steps // reps)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
-    e0.record(s)
-    for _ in range(steps):
-        op.apply(u, y, **kw)
-    e1.record(s)
+    with torch.cuda.stream(s):
+        e0.record(s)
+        for _ in range(n_rep):
+            graph.replay()
+        e1.record(s)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    return e0.elapsed_time(e1) / 1e3 / steps, wall
+    return e0.elapsed_time(e1) / 1e3 / (n_rep * reps), wall
 
 
 def cpu_baseline(m, bc, budget_s=12.0):
@@ -98,6 +123,12 @@ def main():
     ap.add_argument("--solves", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--order", default="forest", choices=["forest", "rowmajor"])
+    ap.add_argument("--sweep", action="store_true",
+                    help="time the apply for N=1..6 in float64 and float32 and exit")
+    ap.add_argument("--sweep-n", type=int, nargs="*", default=[1, 2, 3, 4, 5, 6])
+    ap.add_argument("--copy", action="store_true",
+                    help="time a device-to-device copy at several sizes and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the SIPG path is "
@@ -109,7 +140,31 @@ def main():
     from ddg_amd.operator import SipgOperator
 
     torch.cuda.set_device(0)
-    m, bc = workload_mesh(args.n)
+    if args.copy:
+        for mb in (8, 16, 24, 48, 96, 256, 1024):
+            n = mb * (1 << 20) // 8
+            a = torch.randn(n, dtype=torch.float64, device="cuda")
+            b = torch.empty_like(a)
+
+            class _C:
+                def apply(self, x, y):
+                    y.copy_(x)
+            t, _ = time_apply(_C(), a, b, args.steps, args.warmup)
+            print(f"[copy] {mb:5d} MB read + write: {t * 1e6:9.2f} us  {2 * n * 8 / t / 1e9:8.1f} GB/s",
+                  flush=True)
+        return
+    if args.sweep:
+        for N in args.sweep_n:
+            m, bc = workload_mesh(args.n, N=N, order=args.order)
+            for dt, s in ((torch.float64, 8), (torch.float32, 4)):
+                op = SipgOperator(m, bc, dtype=dt)
+                u = torch.randn(m.K * m.Np, dtype=torch.float64, device="cuda").to(dt)
+                t, _ = time_apply(op, u, torch.empty_like(u), args.steps, args.warmup)
+                frac = apply_bytes(m.K, m.Np, s) / t / 1e9 / HBM_PEAK_GBS
+                print(f"[sweep] N={N} {str(dt)[6:]} K={m.K} {t * 1e6:8.2f} us "
+                      f"{m.K * m.Np / t / 1e9:7.1f} GDOF/s  {frac * 100:5.1f}% HBM", flush=True)
+        return
+    m, bc = workload_mesh(args.n, order=args.order)
     K, Np = m.K, m.Np
     op = SipgOperator(m, bc)
     print(f"[bench] mesh K={K} N={m.N} Np={Np} dofs={K * Np}", file=sys.stderr, flush=True)
@@ -136,6 +191,7 @@ def main():
     cfg = D.IterativeConfig(tol=1e-10, maxiter=20000)
     lat, iters = [], []
     ue = ud
+    us = np.zeros_like(ue)
     for i in range(args.solves):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -157,7 +213,7 @@ def main():
                          f"the same 2x{args.n}x{args.n} N=4 mesh; numpy/BLAS threads = host cores"}
         print(f"[bench] cpu oracle apply: {t_cpu:.3f} s", file=sys.stderr, flush=True)
 
-    lat_s = np.array(lat)
+    lat_s = np.array(lat) if lat else np.array([np.nan])
     out = {
         "metric": "SIPG matvec GDOF/s (% HBM roofline); CG solves/s & p99 at 1e-10 rel err",
         "value": round(gdof, 3),
@@ -166,8 +222,8 @@ def main():
         "ms_per_step": round(t_apply * 1e3, 5),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"2x{args.n}x{args.n} jittered triangles, N=4, Dirichlet x / "
-                               "Neumann y, tau_scale=200, float64 SIPG apply",
+        "config": {"workload": f"2x{args.n}x{args.n} jittered triangles ({args.order} order), N=4, "
+                               "Dirichlet x / Neumann y, tau_scale=200, float64 SIPG apply",
                    "K": K, "Np": Np, "dofs": K * Np},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
@@ -175,7 +231,7 @@ def main():
         "apply_f32": {"us": round(t32 * 1e6, 3), "GDOF/s": round(K * Np / t32 / 1e9, 2),
                       "frac": round(apply_bytes(K, Np, 4) / t32 / 1e9 / HBM_PEAK_GBS, 4)},
         "cg": {"preconditioner": "block-Jacobi", "tol": 1e-10, "solves": len(lat),
-               "solves_per_s": round(len(lat) / float(lat_s.sum()), 4),
+               "solves_per_s": round(len(lat) / float(lat_s.sum()), 4) if lat else None,
                "p50_s": round(float(np.percentile(lat_s, 50)), 4),
                "p99_s": round(float(np.percentile(lat_s, 99)), 4),
                "iterations": iters, "rel_L2_err_vs_exact": err},

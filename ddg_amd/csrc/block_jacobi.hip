@@ -2,17 +2,109 @@
 // (or Helmholtz) operator and their inverses (src/ddg/dg2d/sipg.py:671-805).
 //
 // The reference extracts A_kk with ncolours*Np operator applies on coloured
-// probe vectors (sipg.py:671-704).  Here the apply kernel runs in its probe
-// mode: every element carries the unit vector e_j and neighbour traces are
-// taken as zero -- exactly what a colour probe sees, since face neighbours
-// never share a colour -- so Np applies give every block's column j.  The
-// blocks are inverted in float64 by Gauss-Jordan elimination (SPD blocks, no
-// pivoting), one wavefront per element with the augmented block in LDS, and
-// stored transposed in the precision of the vectors they will multiply.
+// probe vectors (sipg.py:671-704).  Here a probe kernel applies the operator
+// to the unit vector e_j in every element with all neighbour traces taken as
+// zero -- exactly what a colour probe sees, since face neighbours never share
+// a colour -- so Np launches give every block's column j.  The blocks are
+// inverted in float64 by Gauss-Jordan elimination (SPD blocks, no pivoting),
+// one wavefront per element with the augmented block in LDS, and stored
+// transposed in the precision of the vectors they will multiply.
 
 #include "ddgx_internal.h"
 
 namespace ddgx {
+
+// Column j of every diagonal block: y = (A_kk + alpha J_k M) e_j, one thread
+// per element (tile position k; the output is indexed by the element id).
+template <int N>
+__global__ void __launch_bounds__(256)
+block_probe_kernel(const double* __restrict__ tab, const double* __restrict__ rec,
+                   const int* __restrict__ nbr, const int* __restrict__ perm,
+                   double* __restrict__ blocks, double alpha, int K, int j) {
+  using S = Shape<N>;
+  constexpr int Np = S::Np, Nfp = S::Nfp;
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= K) return;
+  const double* Dr = tab + S::oDr;
+  const double* Ds = tab + S::oDs;
+  const double* Ar = tab + S::oAr;
+  const double* As = tab + S::oAs;
+  const double* Mm = tab + S::oM;
+  const double* L1 = tab + S::oL1;
+  const double* LR = tab + S::oLR;
+  const double* LS = tab + S::oLS;
+  double rc[DDGX_REC_WORDS];
+  for (int w = 0; w < DDGX_REC_WORDS; ++w) rc[w] = rec[(size_t)w * K + k];
+  // u = e_j: ur = Dr[:, j], us = Ds[:, j] (tables are stored by columns)
+  double wr[Np], ws[Np], y[Np];
+  for (int i = 0; i < Np; ++i) {
+    const double a = Dr[j * S::RS + i], b = Ds[j * S::RS + i];
+    wr[i] = rc[R_VRR] * a + rc[R_VRS] * b;
+    ws[i] = rc[R_VRS] * a + rc[R_VSS] * b;
+  }
+  for (int i = 0; i < Np; ++i) {
+    double acc = 0.0;
+    for (int c = 0; c < Np; ++c) acc += Ar[c * S::RS + i] * wr[c] + As[c * S::RS + i] * ws[c];
+    y[i] = acc + alpha * rc[R_J] * Mm[j * S::RS + i];
+  }
+  for (int f = 0; f < 3; ++f) {
+    const int code = nbr[(size_t)f * K + k];
+    if (code == CODE_NONE) continue;  // no coupling
+    // neighbours carry no probe (u+ = 0, dn+ = 0); Dirichlet ghost is 0 too
+    double P[Nfp], Xa[Nfp], Xb[Nfp];
+    for (int p = 0; p < Nfp; ++p) {
+      const int n = fmask<N>(f, p);
+      const double uM = (n == j) ? 1.0 : 0.0;
+      const double dnM = rc[R_A + f] * Dr[j * S::RS + n] + rc[R_B + f] * Ds[j * S::RS + n];
+      P[p] = rc[R_PEN + f] * uM - rc[R_SJW + f] * dnM;
+      const double X = -rc[R_SJW + f] * uM;
+      Xa[p] = rc[R_A + f] * X;
+      Xb[p] = rc[R_B + f] * X;
+    }
+    for (int q = 0; q < Nfp; ++q) {
+      double acc = 0.0;
+      for (int p = 0; p < Nfp; ++p) acc += L1[(f * Nfp + p) * S::RSF + q] * P[p];
+      y[fmask<N>(f, q)] += acc;
+    }
+    for (int i = 0; i < Np; ++i) {
+      double acc = 0.0;
+      for (int p = 0; p < Nfp; ++p)
+        acc += LR[(f * Nfp + p) * S::RS + i] * Xa[p] + LS[(f * Nfp + p) * S::RS + i] * Xb[p];
+      y[i] += acc;
+    }
+  }
+  double* out = blocks + (size_t)perm[k] * Np * Np;
+  for (int i = 0; i < Np; ++i) out[i * Np + j] = y[i];
+}
+
+template <int N>
+static int probe_t(const Op& op, double* blocks, double alpha, hipStream_t s) {
+  const int K = (int)op.K;
+  for (int j = 0; j < Shape<N>::Np; ++j) {
+    hipLaunchKernelGGL((block_probe_kernel<N>), dim3((K + 255) / 256), dim3(256), 0, s,
+                       (const double*)op.tab, (const double*)op.rec, op.nbr, op.perm, blocks,
+                       alpha, K, j);
+    DDGX_HIP(hipGetLastError());
+  }
+  return DDGX_OK;
+}
+
+int launch_block_probe(const Op& op, double* blocks, double alpha, hipStream_t s) {
+  if (op.dtype != DDGX_F64) {
+    set_error("block probes need a float64 operator");
+    return DDGX_ERR_INVALID;
+  }
+  switch (op.N) {
+    case 1: return probe_t<1>(op, blocks, alpha, s);
+    case 2: return probe_t<2>(op, blocks, alpha, s);
+    case 3: return probe_t<3>(op, blocks, alpha, s);
+    case 4: return probe_t<4>(op, blocks, alpha, s);
+    case 5: return probe_t<5>(op, blocks, alpha, s);
+    case 6: return probe_t<6>(op, blocks, alpha, s);
+  }
+  set_error("polynomial order not built");
+  return DDGX_ERR_UNSUPPORTED;
+}
 
 template <typename T, int N>
 __global__ void __launch_bounds__(64)

@@ -2,11 +2,13 @@
 // include/ddgx.h).  Handles own their device memory; every compute entry is
 // stream-ordered on the caller's stream.
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "ddgx_internal.h"
+#include "sipg_apply_mfma.h"
 
 struct ddgx_op_s {
   ddgx::Op op;
@@ -35,12 +37,20 @@ int device_cu_count() {
   return cus;
 }
 
-int apply_grid(int N, int dtype);
-
-static int table_size(int N) {
+static int valu_table_size(int N) {
   const int Np = (N + 1) * (N + 2) / 2, Nfp = N + 1;
-  return 5 * Np * Np + 3 * Nfp * Nfp + 6 * Np * Nfp;
+  const int RS = (Np + 7) / 8 * 8, RSF = (Nfp + 7) / 8 * 8;
+  return 5 * Np * RS + 3 * Nfp * RSF + 6 * Nfp * RS;
 }
+
+static int frag_size(int N) {
+  const int Np = (N + 1) * (N + 2) / 2, Nfp = N + 1;
+  const int NB = (Np + 15) / 16, KB = 4 * NB, FK = (9 * Nfp + 3) / 4;
+  return 64 * (2 * NB * KB + NB * 2 * KB + NB * FK + NB * KB);
+}
+
+// host table: column tables followed by one copy of the 64-lane MFMA fragments
+static int table_size(int N) { return valu_table_size(N) + frag_size(N); }
 
 template <typename T>
 static int upload(const double* src, size_t n, void** dst) {
@@ -56,7 +66,8 @@ static int upload_as(int dtype, const double* src, size_t n, void** dst) {
 }
 
 static void free_op(Op& o) {
-  void* ptrs[] = {o.tab, o.rec, o.nbr, o.fsj, o.r, o.p, o.z, o.Ap, o.partials, o.scal};
+  void* ptrs[] = {o.tab, o.rec, o.nbr, o.fcode, o.perm, o.hid, o.hab, o.fsj,
+                  o.r, o.p, o.z, o.Ap, o.partials, o.scal};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (o.scal_host) (void)hipHostFree(o.scal_host);
@@ -74,7 +85,7 @@ const char* ddgx_last_error(void) { return g_err.c_str(); }
 int ddgx_table_size(int N) { return table_size(N); }
 
 int ddgx_op_create(int N, int64_t K, int dtype, const double* tables, const double* rec,
-                   const int32_t* nbr, const double* fsj, ddgx_op* out) {
+                   const int32_t* nbr, const double* fsj, const int32_t* order, ddgx_op* out) {
   if (out == nullptr || tables == nullptr || rec == nullptr || nbr == nullptr || fsj == nullptr) {
     set_error("ddgx_op_create: null argument");
     return DDGX_ERR_INVALID;
@@ -105,6 +116,68 @@ int ddgx_op_create(int N, int64_t K, int dtype, const double* tables, const doub
         return DDGX_ERR_INVALID;
       }
     }
+  // tile order: position i holds element perm[i]
+  std::vector<int> perm(K), inv(K, -1);
+  for (int64_t i = 0; i < K; ++i) {
+    const int64_t k = order ? order[i] : i;
+    if (k < 0 || k >= K || inv[k] >= 0) {
+      set_error("ddgx_op_create: order is not a permutation of 0..K-1");
+      return DDGX_ERR_INVALID;
+    }
+    perm[i] = (int)k;
+    inv[k] = (int)i;
+  }
+  // per-tile halo lists (ids of out-of-tile neighbour elements) and the packed
+  // apply code of every face (neighbour trace slot: 0..63 in the tile, 64+h
+  // in the halo)
+  const int64_t ntiles = (K + TILE - 1) / TILE;
+  std::vector<int> hid(ntiles * HALO_MAX, -1), hpos(ntiles * HALO_MAX, -1), fcode((size_t)K * 3);
+  int max_halo = 0;
+  for (int64_t t = 0; t < ntiles; ++t) {
+    const int64_t i0 = t * TILE, i1 = std::min<int64_t>(K, i0 + TILE);
+    int nh = 0;
+    for (int64_t i = i0; i < i1; ++i) {
+      for (int f = 0; f < 3; ++f) {
+        const int c = nbr[(int64_t)perm[i] * 4 + f];
+        int code = fc_pack(0, 0, 0, c == CODE_DIRICHLET ? FT_DIRICHLET : FT_NONE);
+        if (c >= 0) {
+          const int in = inv[nbr_elem(c)];
+          int sl;
+          if (in >= i0 && in < i1) {
+            sl = (int)(in - i0);
+          } else {
+            int h = 0;
+            while (h < nh && hpos[t * HALO_MAX + h] != in) ++h;
+            if (h == nh) {
+              if (nh == HALO_MAX) {
+                set_error("ddgx_op_create: a 64-element tile has more than 64 out-of-tile "
+                          "neighbours; pass an element order that keeps tiles compact");
+                return DDGX_ERR_INVALID;
+              }
+              hpos[t * HALO_MAX + nh] = in;
+              hid[t * HALO_MAX + nh] = nbr_elem(c);
+              ++nh;
+            }
+            sl = TILE + h;
+          }
+          code = fc_pack(sl, nbr_face(c), nbr_rev(c), FT_INTERIOR);
+        }
+        fcode[(size_t)f * K + i] = code;
+      }
+    }
+    max_halo = std::max(max_halo, nh);
+  }
+  // halo face coefficients a_f, b_f gathered per tile: hab[t][c][h]
+  std::vector<double> hab((size_t)ntiles * 6 * HALO_MAX, 0.0);
+  for (int64_t t = 0; t < ntiles; ++t)
+    for (int hh = 0; hh < HALO_MAX; ++hh) {
+      const int kid = hid[t * HALO_MAX + hh];
+      if (kid < 0) continue;
+      for (int f = 0; f < 3; ++f) {
+        hab[((size_t)t * 6 + f) * HALO_MAX + hh] = rec[(int64_t)kid * DDGX_REC_WORDS + R_A + f];
+        hab[((size_t)t * 6 + 3 + f) * HALO_MAX + hh] = rec[(int64_t)kid * DDGX_REC_WORDS + R_B + f];
+      }
+    }
   auto* h = new ddgx_op_s();
   Op& o = h->op;
   o.N = N;
@@ -112,17 +185,48 @@ int ddgx_op_create(int N, int64_t K, int dtype, const double* tables, const doub
   o.dtype = dtype;
   o.Np = (N + 1) * (N + 2) / 2;
   o.Nfp = N + 1;
+  o.max_halo = max_halo;
   int rc = DDGX_OK;
+  auto put = [&](int** dst, const std::vector<int>& v, const char* what) -> bool {
+    hipError_t e = hipMalloc((void**)dst, v.size() * sizeof(int));
+    if (e == hipSuccess) e = hipMemcpy(*dst, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      rc = hip_fail(e, what);
+      return false;
+    }
+    return true;
+  };
   do {
     if (hipGetDevice(&o.device) != hipSuccess) { rc = hip_fail(hipGetLastError(), "hipGetDevice"); break; }
-    if ((rc = upload_as(dtype, tables, table_size(N), &o.tab)) != DDGX_OK) break;
-    if ((rc = upload_as(dtype, rec, (size_t)K * DDGX_REC_WORDS, &o.rec)) != DDGX_OK) break;
-    if ((rc = upload_as(dtype, fsj, (size_t)K * 4, &o.fsj)) != DDGX_OK) break;
-    hipError_t e = hipMalloc((void**)&o.nbr, (size_t)K * 4 * sizeof(int));
-    if (e != hipSuccess) { rc = hip_fail(e, "hipMalloc(nbr)"); break; }
-    e = hipMemcpy(o.nbr, nbr, (size_t)K * 4 * sizeof(int), hipMemcpyHostToDevice);
-    if (e != hipSuccess) { rc = hip_fail(e, "hipMemcpy(nbr)"); break; }
-    o.grid_apply = apply_grid(N, dtype);
+    {
+      // device table: column tables, then FRAG_COPIES replicas of the fragments
+      const int nv = valu_table_size(N), nf = frag_size(N);
+      std::vector<double> dev((size_t)nv + (size_t)nf * FRAG_COPIES);
+      std::copy(tables, tables + nv, dev.begin());
+      for (int c = 0; c < FRAG_COPIES; ++c)
+        std::copy(tables + nv, tables + nv + nf, dev.begin() + nv + (size_t)c * nf);
+      if ((rc = upload_as(dtype, dev.data(), dev.size(), &o.tab)) != DDGX_OK) break;
+      o.frag = (char*)o.tab + (size_t)nv * (dtype == DDGX_F64 ? 8 : 4);
+    }
+    // records, face codes and Neumann Jacobians in tile order (SoA records)
+    std::vector<double> recT((size_t)K * DDGX_REC_WORDS), fsjT((size_t)K * 4);
+    std::vector<int> nbrT((size_t)K * 3);
+    for (int64_t i = 0; i < K; ++i) {
+      const int64_t k = perm[i];
+      for (int w = 0; w < DDGX_REC_WORDS; ++w) recT[(size_t)w * K + i] = rec[k * DDGX_REC_WORDS + w];
+      for (int f = 0; f < 3; ++f) nbrT[(size_t)f * K + i] = nbr[k * 4 + f];
+      for (int f = 0; f < 4; ++f) fsjT[(size_t)i * 4 + f] = fsj[k * 4 + f];
+    }
+    if ((rc = upload_as(dtype, recT.data(), recT.size(), &o.rec)) != DDGX_OK) break;
+    if ((rc = upload_as(dtype, fsjT.data(), fsjT.size(), &o.fsj)) != DDGX_OK) break;
+    if ((rc = upload_as(dtype, hab.data(), hab.size(), &o.hab)) != DDGX_OK) break;
+    if (!put(&o.nbr, nbrT, "nbr") || !put(&o.fcode, fcode, "fcode") || !put(&o.perm, perm, "perm") ||
+        !put(&o.hid, hid, "hid"))
+      break;
+    {
+      const int cap = apply_grid(N, dtype);
+      o.grid_apply = (int)(ntiles < cap ? ntiles : cap);
+    }
     const int64_t nvec = K * o.Np;
     int64_t gv = (nvec + 255) / 256;
     const int64_t cap = 4 * (int64_t)device_cu_count();
@@ -165,7 +269,7 @@ int ddgx_sipg_apply(ddgx_op op, const void* u, void* y, const void* u_dir, doubl
     set_error("ddgx_sipg_apply: u and y must not alias");
     return DDGX_ERR_INVALID;
   }
-  return launch_apply(op->op, u, y, u_dir, alpha, shift, 0, 0, nullptr, nullptr,
+  return launch_apply(op->op, u, y, u_dir, alpha, shift, 0, nullptr, nullptr,
                       (hipStream_t)stream);
 }
 
@@ -204,8 +308,7 @@ int ddgx_block_jacobi_create(ddgx_op op64, double alpha, int dtype, void* stream
     if (e != hipSuccess) { rc = hip_fail(e, "hipMalloc(blocks)"); break; }
     e = hipMalloc(&pc.binvT, nb * (dtype == DDGX_F64 ? 8 : 4));
     if (e != hipSuccess) { rc = hip_fail(e, "hipMalloc(binvT)"); break; }
-    for (int j = 0; j < o.Np && rc == DDGX_OK; ++j)
-      rc = launch_apply(o, nullptr, pc.blocks, nullptr, alpha, 0.0, M_PROBE, j, nullptr, nullptr, s);
+    rc = launch_block_probe(o, pc.blocks, alpha, s);
     if (rc != DDGX_OK) break;
     rc = launch_invert_blocks(o.N, o.K, dtype, pc.blocks, pc.binvT, s);
     if (rc != DDGX_OK) break;

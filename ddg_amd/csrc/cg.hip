@@ -19,8 +19,6 @@
 
 namespace ddgx {
 
-int apply_grid(int N, int dtype);
-
 enum { F_INIT_BB = 0, F_INIT_GAMMA = 1, F_ALPHA = 2, F_BETA = 3, F_SUM = 4 };
 
 __device__ inline double block_sum_256(double v, double* s_red) {
@@ -339,7 +337,7 @@ static int pcg_t(Op& op, double alpha, double shift, PC* pc, const void* b, void
   CHECK(dot2<T>(op, b, b, nullptr, nullptr, nullptr, s));
   CHECK(finalize(op, op.grid_vec, F_INIT_BB, s));
   // r0 = b - A x0
-  CHECK(launch_apply(op, x, op.Ap, nullptr, alpha, shift, 0, 0, nullptr, nullptr, s));
+  CHECK(launch_apply(op, x, op.Ap, nullptr, alpha, shift, 0, nullptr, nullptr, s));
   hipLaunchKernelGGL((residual_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (const T*)b,
                      (const T*)op.Ap, (T*)op.r, n);
   DDGX_HIP(hipGetLastError());
@@ -358,7 +356,7 @@ static int pcg_t(Op& op, double alpha, double shift, PC* pc, const void* b, void
     if (op.scal_host->done || launched >= cfg.maxiter) break;
     for (int c = 0; c < every && launched < cfg.maxiter; ++c, ++launched) {
       // Ap = A p, partial <p, Ap>
-      CHECK(launch_apply(op, op.p, op.Ap, nullptr, alpha, shift, M_DOT, 0, op.partials, done, s));
+      CHECK(launch_apply(op, op.p, op.Ap, nullptr, alpha, shift, M_DOT, op.partials, done, s));
       CHECK(finalize(op, op.grid_apply, F_ALPHA, s));
       if (bj) {
         CHECK(update_bj_dispatch<T>(op, x, pc->binvT, s));

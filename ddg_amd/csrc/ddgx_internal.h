@@ -20,16 +20,24 @@ struct Shape {
   // lanes per element in the thread-per-node kernels (power of two >= Np)
   static constexpr int NL = Np <= 4 ? 4 : Np <= 8 ? 8 : Np <= 16 ? 16 : 32;
   static constexpr int EPB = 256 / NL;  // elements per 256-thread tile
-  // table offsets (see RefElement.kernel_tables in ddg_amd/reference.py)
+  // table layout (see RefElement.kernel_tables in ddg_amd/reference.py):
+  // every matrix is stored by columns, each column zero-padded to RS (RSF
+  // for face-length columns) entries, a multiple of 8, so columns start on
+  // 64-byte boundaries for the scalar loads.
+  //   Dr, Ds, ArT, AsT, M : Np columns of length Np
+  //   L1[f]               : Nfp columns (index p) of length Nfp (index q)
+  //   LR[f], LS[f]        : Nfp columns (index p) of length Np (index i)
+  static constexpr int RS = (Np + 7) / 8 * 8;
+  static constexpr int RSF = (Nfp + 7) / 8 * 8;
   static constexpr int oDr = 0;
-  static constexpr int oDs = Np * Np;
-  static constexpr int oAr = 2 * Np * Np;
-  static constexpr int oAs = 3 * Np * Np;
-  static constexpr int oM = 4 * Np * Np;
-  static constexpr int oL1 = 5 * Np * Np;
-  static constexpr int oLR = oL1 + 3 * Nfp * Nfp;
-  static constexpr int oLS = oLR + 3 * Np * Nfp;
-  static constexpr int size = oLS + 3 * Np * Nfp;
+  static constexpr int oDs = Np * RS;
+  static constexpr int oAr = 2 * Np * RS;
+  static constexpr int oAs = 3 * Np * RS;
+  static constexpr int oM = 4 * Np * RS;
+  static constexpr int oL1 = 5 * Np * RS;
+  static constexpr int oLR = oL1 + 3 * Nfp * RSF;
+  static constexpr int oLS = oLR + 3 * Nfp * RS;
+  static constexpr int size = oLS + 3 * Nfp * RS;
 };
 
 // Node index of face node p on face f.  Nodes are ordered in rows of
@@ -66,7 +74,7 @@ constexpr int CODE_DIRICHLET = -1;
 constexpr int CODE_NONE = -2;
 
 // Kernel mode bits for the apply
-enum : int { M_HELM = 1, M_UDIR = 2, M_PROBE = 4, M_DOT = 8, M_RESID = 16 };
+enum : int { M_HELM = 1, M_UDIR = 2, M_DOT = 8, M_RESID = 16 };
 
 // Device scalars of one CG solve
 struct CGScalars {
@@ -75,15 +83,38 @@ struct CGScalars {
   double tol, atol;
 };
 
+// Per-element data of an operator is stored in tile order: tile position
+// k (0..K-1, 64 per tile) holds element perm[k].  Vectors (u, y, ...) keep
+// the caller's element order and are addressed through perm.
+constexpr int TILE = 64;      // elements per matrix-core workgroup
+constexpr int HALO_MAX = 64;  // out-of-tile neighbour elements per tile
+// Packed per-face code of the apply: bits 0-7 LDS slot of the neighbour
+// trace (0..63 in the tile, 64+h halo), bits 8-9 neighbour face, bit 10
+// reversed node order, bits 11-12 face type.
+enum : int { FT_INTERIOR = 0, FT_DIRICHLET = 1, FT_NONE = 2 };
+__host__ __device__ inline int fc_slot(int c) { return c & 0xff; }
+__host__ __device__ inline int fc_face(int c) { return (c >> 8) & 3; }
+__host__ __device__ inline int fc_rev(int c) { return (c >> 10) & 1; }
+__host__ __device__ inline int fc_type(int c) { return (c >> 11) & 3; }
+__host__ __device__ inline int fc_pack(int slot, int face, int rev, int type) {
+  return slot | (face << 8) | (rev << 10) | (type << 11);
+}
+
 struct Op {
   int N = 0, dtype = 0, device = 0;
   int64_t K = 0;
   int Np = 0, Nfp = 0;
-  void* tab = nullptr;   // table (dtype)
-  void* rec = nullptr;   // K x 16 (dtype)
-  int* nbr = nullptr;    // K x 4
-  void* fsj = nullptr;   // K x 4 (dtype)
-  int grid_apply = 0;    // persistent grid for the apply
+  void* tab = nullptr;   // table (dtype): column tables, then MFMA fragments
+  void* frag = nullptr;  // = tab + Shape<N>::size (MFMA A fragments)
+  void* rec = nullptr;   // 16 x K records, tile order (dtype)
+  int* nbr = nullptr;    // 3 x K face codes (see above), tile order
+  int* fcode = nullptr;  // 3 x K packed apply codes (fc_* below), tile order
+  int* perm = nullptr;   // K: element id at each tile position
+  int* hid = nullptr;    // ntiles x HALO_MAX element ids of halo elements (-1: none)
+  void* hab = nullptr;   // ntiles x 6 x HALO_MAX halo face coefficients a_f, b_f (dtype)
+  void* fsj = nullptr;   // K x 4 Neumann face Jacobians, tile order (dtype)
+  int grid_apply = 0;    // workgroups of one apply (64-element tiles)
+  int max_halo = 0;      // largest halo of a tile
   int grid_vec = 0;      // grid for vector kernels / reductions
   // CG workspace
   void* r = nullptr;
@@ -142,12 +173,14 @@ int hip_fail(hipError_t e, const char* what);
 
 // launchers (defined per translation unit)
 int launch_apply(const Op& op, const void* u, void* y, const void* udir, double alpha,
-                 double shift, int mode, int probe_j, double* partial, const int* skip,
-                 hipStream_t s, const void* bvec = nullptr);
+                 double shift, int mode, double* partial, const int* skip, hipStream_t s,
+                 const void* bvec = nullptr);
 int pc_apply(PC& pc, const void* r, void* z, hipStream_t s);
 int mg_vcycle(MG& mg, const void* b, void* x, hipStream_t s);
 int launch_rhs(const Op& op, const void* f, const void* udir, const void* q, void* rhs,
                hipStream_t s);
+int apply_grid(int N, int dtype);
+int launch_block_probe(const Op& op, double* blocks, double alpha, hipStream_t s);
 int launch_invert_blocks(int N, int64_t K, int dtype, const double* blocks, void* binvT,
                          hipStream_t s);
 int launch_bj_apply(int N, int64_t K, int dtype, const void* binvT, const void* r, void* z,

@@ -1,0 +1,352 @@
+// SIPG apply on the matrix cores (v_mfma_*_16x16x4) for gfx950:
+//   y = (A_sipg + alpha*J*M + shift*I) u          (or y = b - that, residual mode)
+// with A_sipg the mass-weighted SIPG Laplacian of src/ddg/dg2d/sipg.py:142-337
+// in the closed form of its reverse-mode gradient:
+//
+//   volume : y_k = ArT (vrr ur + vrs us) + AsT (vrs ur + vss us),
+//            ur = Dr u_k, us = Ds u_k, ArT = Dr^T M^T, AsT = Ds^T M^T
+//   face f : P = pen_f [u] - sJw_f (dn^- + dn^+)    -> y[Fmask_f] += m1d_f^T P
+//            X = -sJw_f [u]                          -> y += (a_f LR_f + b_f LS_f) X
+//
+// [u] = u^- - u^+, dn^- = a_f ur + b_f us at the own face nodes, dn^+ the
+// neighbour's normal derivative taken with the own normal.
+//
+// The element-local contractions are 16x16x4 MFMA products with 16 elements
+// of a wave as the N dimension:
+//
+//   [ur; us] = [Dr; Ds] . U                      (GEMM 1)
+//   y = ArT . Wr + AsT . Ws + G . F  (+ M . aJu)  (GEMM 2)
+//
+// U, Wr, Ws and y are held in the accumulator layout of the MFMA: lane
+// (g = lane/16, e = lane%16), register r holds node 16b + crow(g, r) of
+// element e, with crow(g, r) = g + 4r for float64 and 4g + r for float32 (the
+// output layouts of the two instructions).  Because the k index of a product
+// may be permuted freely, the constant A fragments are laid out on the host
+// (reference.mfma_fragments) with the matching permutation, so an output
+// feeds the next product as its B operand without any lane shuffle.  F holds
+// the per-face-node values P, a*X, b*X (9*Nfp rows); G = [E_f m1d_f^T | LR_f |
+// LS_f] lifts them.
+//
+// A workgroup is 4 waves; it processes tiles of 64 elements in a loop
+// (persistent workgroups, so the A fragments are loaded once per workgroup).
+// Face traces (u, dn) of the tile's elements and of its halo -- the out-of-tile neighbours, listed
+// per tile on the host -- are computed with the same GEMM 1 and exchanged
+// through LDS, so every face reads its neighbour trace from LDS through a
+// host-computed slot index.  All global loads of a tile are issued up front;
+// every element owns its output row (no atomics).
+
+#pragma once
+
+#include "ddgx_internal.h"
+
+namespace ddgx {
+
+template <typename T>
+struct MfmaT;
+template <>
+struct MfmaT<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  __device__ static v4 mma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __host__ __device__ static constexpr int crow(int g, int r) { return g + 4 * r; }
+};
+template <>
+struct MfmaT<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  __device__ static v4 mma(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  __host__ __device__ static constexpr int crow(int g, int r) { return 4 * g + r; }
+};
+
+// Replicas of the fragment table in device memory (workgroup b reads replica
+// b % FRAG_COPIES), so the burst of identical loads at kernel start is spread
+// over many L2 channels instead of hammering the few that hold one copy.
+constexpr int FRAG_COPIES = 32;
+
+// fragment counts / offsets (in 64-lane fragments) of the MFMA operand table
+template <int N>
+struct MS {
+  static constexpr int Np = Shape<N>::Np, Nfp = Shape<N>::Nfp;
+  static constexpr int NB = (Np + 15) / 16;     // 16-row blocks of a nodal vector
+  static constexpr int KB = 4 * NB;             // k-steps over one nodal vector
+  static constexpr int NF = 9 * Nfp;            // face rows: P, aX, bX
+  static constexpr int FK = (NF + 3) / 4;       // k-steps over the face rows
+  static constexpr int FS = 4 * FK + 1;         // LDS row stride of the face rows (odd)
+  static constexpr int TRW = 3 * Nfp * 2;       // trace words per element (u, dn per face node)
+  static constexpr int f1 = 0;                  // GEMM1: 2*NB out blocks x KB
+  static constexpr int f2 = f1 + 2 * NB * KB;   // GEMM2 volume: NB x 2*KB
+  static constexpr int fF = f2 + NB * 2 * KB;   // GEMM2 face: NB x FK
+  static constexpr int fM = fF + NB * FK;       // mass: NB x KB
+  static constexpr int nfrag = fM + NB * KB;
+};
+
+// normal-derivative coefficients a_f, b_f of one element's three faces
+template <typename T>
+struct FaceAB {
+  T a0, a1, a2, b0, b1, b2;
+  __device__ T a(int f) const { return f == 0 ? a0 : (f == 1 ? a1 : a2); }
+  __device__ T b(int f) const { return f == 0 ? b0 : (f == 1 ? b1 : b2); }
+};
+
+// GEMM 1 for 16 elements of a wave and their face traces into LDS slots.
+template <typename T, int N>
+__device__ __forceinline__ void grad_and_traces(const T (&A1)[2 * MS<N>::NB][MS<N>::KB],
+                                                const T (&U)[MS<N>::NB][4],
+                                                T (&UR)[MS<N>::NB][4], T (&US)[MS<N>::NB][4],
+                                                const FaceAB<T>& ab, T* tr, int g) {
+  using Q = MS<N>;
+  using MF = MfmaT<T>;
+  using v4 = typename MF::v4;
+  constexpr int NB = Q::NB, KB = Q::KB, Nfp = Q::Nfp, Np = Q::Np;
+#pragma unroll
+  for (int b = 0; b < 2 * NB; ++b) {
+    v4 acc = {0, 0, 0, 0};
+#pragma unroll
+    for (int t = 0; t < KB; ++t) acc = MF::mma(A1[b][t], U[t / 4][t % 4], acc);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (b < NB)
+        UR[b][r] = acc[r];
+      else
+        US[b - NB][r] = acc[r];
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < NB; ++b)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int n = 16 * b + MF::crow(g, r);
+      if (n >= Np) continue;
+#pragma unroll
+      for (int f = 0; f < 3; ++f) {
+        int p = -1;
+#pragma unroll
+        for (int q = 0; q < Nfp; ++q)
+          if (fmask<N>(f, q) == n) p = q;
+        if (p >= 0) {
+          tr[(f * Nfp + p) * 2] = U[b][r];
+          tr[(f * Nfp + p) * 2 + 1] = ab.a(f) * UR[b][r] + ab.b(f) * US[b][r];
+        }
+      }
+    }
+}
+
+template <typename T, int N, int MODE>
+__global__ void __launch_bounds__(256)
+sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
+                       const int* __restrict__ fcode, const int* __restrict__ perm,
+                       const int* __restrict__ hid, const T* __restrict__ hab,
+                       const T* __restrict__ u,
+                       T* __restrict__ y, const T* __restrict__ udir, T alpha, T shift, int K,
+                       double* __restrict__ partial, const int* __restrict__ skip,
+                       const T* __restrict__ bvec) {
+  using Q = MS<N>;
+  using MF = MfmaT<T>;
+  using v4 = typename MF::v4;
+  constexpr int Np = Q::Np, Nfp = Q::Nfp, NB = Q::NB, KB = Q::KB, FK = Q::FK;
+  constexpr int TRW = Q::TRW;
+  constexpr bool HELM = MODE & M_HELM, UDIR = MODE & M_UDIR, DOT = MODE & M_DOT;
+  constexpr bool RESID = MODE & M_RESID;
+  constexpr int XT = (TILE + HALO_MAX) * TRW;  // trace slots: tile then halo
+  constexpr int XF = TILE * Q::FS;             // face rows
+  constexpr int XW = XT > XF ? XT : XF;
+  // s_x holds the face traces during the face phase and, after a barrier,
+  // the face rows F that feed GEMM 2.
+  __shared__ T s_x[XW];
+  __shared__ T s_rec[TILE * DDGX_REC_WORDS];
+  __shared__ int s_fc[3 * TILE];
+  __shared__ double s_red[4];
+
+  if (skip != nullptr && *skip) return;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6;
+  const int g = lane >> 4, e = lane & 15;
+  const int el = 16 * wv + e;  // this lane's element within a tile
+  const int hi = 16 * wv + e;  // this lane's halo element within a tile
+  // A fragments: loaded once per workgroup from one of the table's replicas
+  // (all workgroups start at once; replicas spread those loads over L2)
+  const T* fr = frag + (size_t)(blockIdx.x % FRAG_COPIES) * Q::nfrag * 64;
+  T A1[2 * NB][KB];
+#pragma unroll
+  for (int b = 0; b < 2 * NB; ++b)
+#pragma unroll
+    for (int t = 0; t < KB; ++t) A1[b][t] = fr[(size_t)(Q::f1 + b * KB + t) * 64 + lane];
+  double dacc = 0.0;
+  const int ntiles = (K + TILE - 1) / TILE;
+
+  // persistent workgroups: each loops over tiles
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  const int k0 = tile * TILE;
+  const int ne = min(TILE, K - k0);
+  const int k = k0 + el;       // this lane's tile position
+  const bool valid = el < ne;
+  const int kg = valid ? perm[k] : 0;  // its element id
+  __syncthreads();  // the previous tile is done with LDS
+
+  // ---- global loads of the tile: indices first, then nodal values --------
+  // halo: the tile's out-of-tile neighbours, 16 per wave, listed from slot 0
+  const bool hwave = hid[tile * HALO_MAX + 16 * wv] >= 0;  // uniform per wave
+  const int hg = hwave ? hid[tile * HALO_MAX + hi] : -1;
+  FaceAB<T> abh{};
+  if (hwave && hg >= 0) {
+    const T* hb = hab + (size_t)tile * 6 * HALO_MAX + hi;
+    abh = FaceAB<T>{hb[0], hb[HALO_MAX], hb[2 * HALO_MAX], hb[3 * HALO_MAX], hb[4 * HALO_MAX],
+                    hb[5 * HALO_MAX]};
+  }
+  T U[NB][4], UH[NB][4];
+#pragma unroll
+  for (int b = 0; b < NB; ++b)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int n = 16 * b + MF::crow(g, r);
+      U[b][r] = (valid && n < Np) ? u[(size_t)kg * Np + n] : T(0);
+      UH[b][r] = (hg >= 0 && n < Np) ? u[(size_t)hg * Np + n] : T(0);
+    }
+  for (int i = tid; i < DDGX_REC_WORDS * TILE; i += 256) {
+    const int w = i / TILE, ee = i % TILE;
+    s_rec[ee * DDGX_REC_WORDS + w] = ee < ne ? rec[(size_t)w * K + k0 + ee] : T(0);
+  }
+  for (int i = tid; i < 3 * TILE; i += 256) {
+    const int f = i / TILE, ee = i % TILE;
+    s_fc[i] = ee < ne ? fcode[(size_t)f * K + k0 + ee] : (FT_NONE << 11);
+  }
+  // GEMM 2 fragments (issued with the tile's other loads; L2 resident).
+  // The empty asm makes the pointer loop-variant so the compiler keeps these
+  // loads here instead of hoisting them (and 40 more registers) out of the loop.
+  const T* fr2 = fr;
+  asm volatile("" : "+s"(fr2));
+  T A2[NB][2 * KB + FK];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+#pragma unroll
+    for (int t = 0; t < 2 * KB; ++t) A2[b][t] = fr2[(size_t)(Q::f2 + b * 2 * KB + t) * 64 + lane];
+#pragma unroll
+    for (int t = 0; t < FK; ++t) A2[b][2 * KB + t] = fr2[(size_t)(Q::fF + b * FK + t) * 64 + lane];
+  }
+  __syncthreads();
+
+  // ---- GEMM 1 and face traces: own elements, then the halo --------------
+  const T* re = s_rec + el * DDGX_REC_WORDS;
+  T UR[NB][4], US[NB][4];
+  {
+    const FaceAB<T> ab{re[R_A], re[R_A + 1], re[R_A + 2], re[R_B], re[R_B + 1], re[R_B + 2]};
+    grad_and_traces<T, N>(A1, U, UR, US, ab, s_x + el * TRW, g);
+  }
+  if (hwave) {
+    T HR[NB][4], HS[NB][4];
+    grad_and_traces<T, N>(A1, UH, HR, HS, abh, s_x + (TILE + hi) * TRW, g);
+  }
+  // metric products (in place): Wr, Ws
+#pragma unroll
+  for (int b = 0; b < NB; ++b)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const T a = UR[b][r], c = US[b][r];
+      UR[b][r] = re[R_VRR] * a + re[R_VRS] * c;
+      US[b][r] = re[R_VRS] * a + re[R_VSS] * c;
+    }
+  __syncthreads();
+
+  // ---- face phase: P, aX, bX per (element, face, node) ---------------------
+  constexpr int NIT = (TILE * 3 * Nfp + 255) / 256;
+  T fv[NIT][3];
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int item = tid + 256 * it;
+    fv[it][0] = fv[it][1] = fv[it][2] = T(0);
+    if (item < TILE * 3 * Nfp) {
+      const int e2 = item / (3 * Nfp), f = (item / Nfp) % 3, p = item % Nfp;
+      if (e2 < ne) {
+        const T* rr = s_rec + e2 * DDGX_REC_WORDS;
+        const T uM = s_x[e2 * TRW + (f * Nfp + p) * 2];
+        const T dnM = s_x[e2 * TRW + (f * Nfp + p) * 2 + 1];
+        const int c = s_fc[f * TILE + e2];
+        const int ty = fc_type(c);
+        T uP, dnP;
+        if (ty == FT_INTERIOR) {
+          const int sl = fc_slot(c), fn = fc_face(c);
+          const int pn = fc_rev(c) ? (Nfp - 1 - p) : p;
+          uP = s_x[sl * TRW + (fn * Nfp + pn) * 2];
+          dnP = -s_x[sl * TRW + (fn * Nfp + pn) * 2 + 1];
+        } else if (ty == FT_DIRICHLET) {
+          if constexpr (UDIR)
+            uP = udir[(size_t)perm[k0 + e2] * Np + fmask<N>(f, p)];
+          else
+            uP = T(0);
+          dnP = T(0);
+        } else {
+          uP = uM;
+          dnP = T(0);
+        }
+        const T ju = uM - uP;
+        const T X = -rr[R_SJW + f] * ju;
+        fv[it][0] = rr[R_PEN + f] * ju - rr[R_SJW + f] * (dnM + dnP);
+        fv[it][1] = rr[R_A + f] * X;
+        fv[it][2] = rr[R_B + f] * X;
+      }
+    }
+  }
+  __syncthreads();  // traces consumed: s_x becomes the face rows
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int item = tid + 256 * it;
+    if (item < TILE * 3 * Nfp) {
+      const int e2 = item / (3 * Nfp), fp = item % (3 * Nfp);
+      s_x[e2 * Q::FS + fp] = fv[it][0];
+      s_x[e2 * Q::FS + 3 * Nfp + fp] = fv[it][1];
+      s_x[e2 * Q::FS + 6 * Nfp + fp] = fv[it][2];
+    }
+  }
+  constexpr int PADC = 4 * FK - Q::NF;
+  if constexpr (PADC > 0) {
+    for (int i = tid; i < TILE * PADC; i += 256) s_x[(i / PADC) * Q::FS + Q::NF + i % PADC] = T(0);
+  }
+  __syncthreads();
+
+  // ---- GEMM 2: volume + face lift (+ mass) -------------------------------
+  T Jm = T(0);
+  if constexpr (HELM) Jm = alpha * re[R_J];
+#pragma unroll
+  for (int bo = 0; bo < NB; ++bo) {
+    v4 acc = {0, 0, 0, 0};
+#pragma unroll
+    for (int t = 0; t < KB; ++t) acc = MF::mma(A2[bo][t], UR[t / 4][t % 4], acc);
+#pragma unroll
+    for (int t = 0; t < KB; ++t) acc = MF::mma(A2[bo][KB + t], US[t / 4][t % 4], acc);
+#pragma unroll
+    for (int t = 0; t < FK; ++t) acc = MF::mma(A2[bo][2 * KB + t], s_x[el * Q::FS + 4 * t + g], acc);
+    if constexpr (HELM) {
+#pragma unroll
+      for (int t = 0; t < KB; ++t)
+        acc = MF::mma(fr[(size_t)(Q::fM + bo * KB + t) * 64 + lane], Jm * U[t / 4][t % 4], acc);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int n = 16 * bo + MF::crow(g, r);
+      T v = acc[r];
+      if constexpr (HELM) v += shift * U[bo][r];
+      if (valid && n < Np) {
+        const size_t gi = (size_t)kg * Np + n;
+        if constexpr (RESID)
+          y[gi] = bvec[gi] - v;
+        else
+          y[gi] = v;
+        if constexpr (DOT) dacc += (double)U[bo][r] * (double)v;
+      }
+    }
+  }
+
+  }  // tile loop
+
+  if constexpr (DOT) {
+    for (int off = 32; off > 0; off >>= 1) dacc += __shfl_down(dacc, off, 64);
+    if (lane == 0) s_red[wv] = dacc;
+    __syncthreads();
+    if (tid == 0) partial[blockIdx.x] = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
+  }
+}
+
+}  // namespace ddgx

@@ -26,7 +26,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .reference import ref_element
+from .reference import mfma_fragments, ref_element
 
 BC_INTERIOR, BC_DIRICHLET, BC_NEUMANN = 0, 1, 2
 
@@ -98,6 +98,60 @@ def pack(mesh, bc_types, tau_scale=200.0, kappa=None):
     return rec, nbr, fsj
 
 
+TILE = 64          # elements per matrix-core workgroup
+HALO_MAX = 64      # out-of-tile neighbour elements a tile may have
+
+
+def max_tile_halo(EToE, perm=None, tile=TILE):
+    """Largest number of distinct out-of-tile face neighbours of a tile when
+    the elements are processed in the order ``perm``."""
+    E = np.asarray(EToE, dtype=np.int64)
+    K = E.shape[0]
+    perm = np.arange(K) if perm is None else np.asarray(perm, dtype=np.int64)
+    inv = np.empty(K, dtype=np.int64)
+    inv[perm] = np.arange(K)
+    En = E[perm]                                   # neighbours of each position
+    pn = inv[En]
+    t = np.arange(K) // tile
+    out = (pn // tile != t[:, None]) & (En != perm[:, None])
+    keys = np.unique(t[:, None].repeat(3, 1)[out] * K + pn[out])
+    return int(np.bincount(keys // K, minlength=1).max()) if keys.size else 0
+
+
+def morton_order(mesh):
+    """Element order along a Z-order curve of the element centroids."""
+    cx = np.asarray(mesh.x).mean(axis=0)
+    cy = np.asarray(mesh.y).mean(axis=0)
+    span = max(np.ptp(cx), np.ptp(cy), 1e-300)
+    qx = ((cx - cx.min()) / span * 65535).astype(np.uint64)
+    qy = ((cy - cy.min()) / span * 65535).astype(np.uint64)
+
+    def spread(v):
+        v = (v | (v << np.uint64(8))) & np.uint64(0x00FF00FF)
+        v = (v | (v << np.uint64(4))) & np.uint64(0x0F0F0F0F)
+        v = (v | (v << np.uint64(2))) & np.uint64(0x33333333)
+        v = (v | (v << np.uint64(1))) & np.uint64(0x55555555)
+        return v
+    return np.argsort(spread(qx) | (spread(qy) << np.uint64(1)), kind="stable")
+
+
+def tile_order(mesh):
+    """Processing order of the elements: None (the mesh's own order) when it
+    keeps every 64-element tile's halo at 32 elements or fewer (e.g.
+    refinement-forest meshes), else a Morton order of the centroids when that
+    is more compact.  Each 16 halo elements cost one extra gradient product."""
+    natural = max_tile_halo(mesh.EToE)
+    if natural <= HALO_MAX // 2:
+        return None
+    perm = morton_order(mesh)
+    m = max_tile_halo(mesh.EToE, perm)
+    if m < natural:
+        return perm.astype(np.int32)
+    if natural > HALO_MAX:
+        raise ValueError("no element order keeps the 64-element tiles compact enough")
+    return None
+
+
 def _stream():
     return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
@@ -122,9 +176,13 @@ class SipgOperator:
         if dtype not in (torch.float64, torch.float32):
             raise ValueError("dtype must be torch.float64 or torch.float32")
         rec, nbr, fsj = pack(mesh, bc_types, tau_scale, kappa)
-        tab = ref_element(self.N).kernel_tables()
+        ref = ref_element(self.N)
+        tab = np.concatenate([ref.kernel_tables(),
+                              mfma_fragments(ref, "f64" if dtype == torch.float64 else "f32")])
         L = _lib.lib()
         assert tab.size == L.ddgx_table_size(self.N)
+        order = tile_order(mesh)
+        self.order = order
         h = C.c_void_p()
         torch.cuda.current_device()  # make sure the HIP context exists
         _lib.check(L.ddgx_op_create(self.N, self.K, self.code,
@@ -132,6 +190,7 @@ class SipgOperator:
                                     np.ascontiguousarray(rec).ctypes.data_as(C.c_void_p),
                                     np.ascontiguousarray(nbr).ctypes.data_as(C.c_void_p),
                                     np.ascontiguousarray(fsj).ctypes.data_as(C.c_void_p),
+                                    None if order is None else order.ctypes.data_as(C.c_void_p),
                                     C.byref(h)), "ddgx_op_create")
         self._h = h
         self.device = torch.device("cuda", torch.cuda.current_device())

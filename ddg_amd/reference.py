@@ -179,12 +179,13 @@ class RefElement:
                              for p in params])
 
     def kernel_tables(self) -> np.ndarray:
-        """Flat float64 table in the order the HIP apply kernel expects.
+        """Flat float64 table in the layout the HIP kernels read.
 
-        [Dr | Ds | ArT | AsT | M | L1 | LR | LS] with
-          ArT = Dr^T M^T, AsT = Ds^T M^T        (Np x Np, row-major)
-          L1[f][q][p] = m1d_f[p][q]              (3 x Nfp x Nfp)
-          LR[f][i][p] = sum_q Dr[Fm_f[q], i] m1d_f[p][q]   (3 x Np x Nfp)
+        Matrices, each stored by columns (column j contiguous), every column
+        zero-padded to a multiple of 8 entries (64-byte aligned):
+          Dr, Ds, ArT = Dr^T M^T, AsT = Ds^T M^T, M      (Np x Np)
+          L1[f][q][p] = m1d_f[p][q]                       (3 x Nfp x Nfp)
+          LR[f][i][p] = sum_q Dr[Fm_f[q], i] m1d_f[p][q]  (3 x Np x Nfp)
           LS[f][i][p] = sum_q Ds[Fm_f[q], i] m1d_f[p][q]
         """
         Fm = self.Fmask
@@ -193,8 +194,74 @@ class RefElement:
         L1 = np.transpose(self.m1d, (0, 2, 1))
         LR = np.stack([self.Dr[Fm[:, f], :].T @ self.m1d[f].T for f in range(3)])
         LS = np.stack([self.Ds[Fm[:, f], :].T @ self.m1d[f].T for f in range(3)])
+
+        def by_columns(a):
+            c = np.swapaxes(a, -1, -2)                    # (..., columns, rows)
+            w = (c.shape[-1] + 7) // 8 * 8
+            out = np.zeros(c.shape[:-1] + (w,))
+            out[..., :c.shape[-1]] = c
+            return out.reshape(-1)
         parts = [self.Dr, self.Ds, ArT, AsT, self.M, L1, LR, LS]
-        return np.concatenate([np.ascontiguousarray(p).reshape(-1) for p in parts])
+        return np.concatenate([by_columns(np.ascontiguousarray(p)) for p in parts])
+
+
+def mfma_crow(dtype: str, g, r):
+    """Node row held by lane group g, accumulator register r, in the output
+    layout of v_mfma_f64_16x16x4f64 (rows g + 4r) / v_mfma_f32_16x16x4f32
+    (rows 4g + r)."""
+    return g + 4 * r if dtype == "f64" else 4 * g + r
+
+
+def mfma_fragments(ref: "RefElement", dtype: str) -> np.ndarray:
+    """A-operand fragments of the matrix-core apply, one 64-lane fragment per
+    (output block, k-step), in the order GEMM1 | GEMM2 volume | face | mass.
+
+    Lane l = 16 g + m supplies A[16 b + m][sigma(t, g)].  For products whose
+    B operand is a nodal vector held in accumulator layout, the k index is
+    sigma(t, g) = 16 (t // 4) + crow(g, t % 4); for the face rows it is
+    4 t + g.  Entries outside the matrix are zero.
+    """
+    Np, Nfp, Fm = ref.Np, ref.Nfp, ref.Fmask
+    NB = (Np + 15) // 16
+    KB = 4 * NB
+    NF = 9 * Nfp
+    FK = (NF + 3) // 4
+    lanes = np.arange(64)
+    m, g = lanes % 16, lanes // 16
+
+    def frag_nodal(Mat, b, t):
+        row = 16 * b + m
+        col = 16 * (t // 4) + mfma_crow(dtype, g, t % 4)
+        ok = (row < Mat.shape[0]) & (col < Mat.shape[1])
+        return np.where(ok, Mat[np.minimum(row, Mat.shape[0] - 1), np.minimum(col, Mat.shape[1] - 1)], 0.0)
+
+    ArT = ref.Dr.T @ ref.M.T
+    AsT = ref.Ds.T @ ref.M.T
+    G = np.zeros((Np, NF))
+    for f in range(3):
+        for q in range(Nfp):
+            G[Fm[q, f], f * Nfp:(f + 1) * Nfp] = ref.m1d[f][:, q]
+        LR = ref.Dr[Fm[:, f], :].T @ ref.m1d[f].T
+        LS = ref.Ds[Fm[:, f], :].T @ ref.m1d[f].T
+        G[:, 3 * Nfp + f * Nfp:3 * Nfp + (f + 1) * Nfp] = LR
+        G[:, 6 * Nfp + f * Nfp:6 * Nfp + (f + 1) * Nfp] = LS
+    out = []
+    for b in range(2 * NB):                              # GEMM1: Dr blocks then Ds blocks
+        R = ref.Dr if b < NB else ref.Ds
+        for t in range(KB):
+            out.append(frag_nodal(R, b % NB, t))
+    for b in range(NB):                                  # GEMM2 volume: ArT then AsT
+        for t in range(2 * KB):
+            out.append(frag_nodal(ArT if t < KB else AsT, b, t % KB))
+    for b in range(NB):                                  # face lift
+        for t in range(FK):
+            row, col = 16 * b + m, 4 * t + g
+            ok = (row < Np) & (col < NF)
+            out.append(np.where(ok, G[np.minimum(row, Np - 1), np.minimum(col, NF - 1)], 0.0))
+    for b in range(NB):                                  # mass
+        for t in range(KB):
+            out.append(frag_nodal(ref.M, b, t))
+    return np.concatenate(out)
 
 
 @functools.lru_cache(maxsize=None)

@@ -70,10 +70,17 @@ const char* ddgx_last_error(void);
  *   nbr     K x 4  face codes: (k'<<3)|(f'<<1)|reversed, -1 Dirichlet,
  *                  -2 no coupling (Neumann)
  *   fsj     K x 4  face Jacobian sJ on Neumann faces, 0 elsewhere (used by
- *                  the Neumann load only)                              */
+ *                  the Neumann load only)
+ *   order   K      optional (NULL = identity) processing order of the
+ *                  elements; the apply works on tiles of 64 consecutive
+ *                  entries, each with at most 64 out-of-tile neighbours, so
+ *                  a spatially compact order is required for large meshes
+ *                  whose own numbering is not (ddg_amd.operator computes a
+ *                  Morton order when needed).  Vectors keep the caller's
+ *                  element numbering.                                   */
 int ddgx_op_create(int N, int64_t K, int dtype, const double* tables,
                    const double* rec, const int32_t* nbr, const double* fsj,
-                   ddgx_op* out);
+                   const int32_t* order, ddgx_op* out);
 int ddgx_op_destroy(ddgx_op op);
 int ddgx_op_info(ddgx_op op, int* N, int64_t* K, int* dtype);
 

@@ -92,5 +92,24 @@ def test_library_exports_every_header_symbol():
 def test_table_size_matches_python():
     from ddg_amd import _lib
     L = _lib.lib()
+    from ddg_amd.reference import mfma_fragments
     for N in range(1, 7):
-        assert L.ddgx_table_size(N) == ref_element(N).kernel_tables().size
+        ref = ref_element(N)
+        assert L.ddgx_table_size(N) == ref.kernel_tables().size + mfma_fragments(ref, "f64").size
+
+
+@pytest.mark.parametrize("N", [1, 3, 4, 6])
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_mfma_fragments_reproduce_apply(N, dtype):
+    """Lane-level emulation of the matrix-core data flow (fragment layout,
+    k permutation, output layout) against the closed-form model."""
+    from kernel_model import model_apply_mfma
+    from ddg_amd.reference import mfma_fragments
+    pm, _ = unit_square(3, 3, N, jitter=0.1, seed=4)
+    bc = pmesh.dirichlet_x_neumann_y(pm)
+    rec, nbr, _ = pack(pm, bc)
+    ref = ref_element(N)
+    tab = ref.kernel_tables()
+    u = np.random.default_rng(N).standard_normal((pm.K, pm.Np))
+    y = model_apply_mfma(N, tab, mfma_fragments(ref, dtype), rec, nbr, u, dtype)
+    assert rel(y, model_apply(N, tab, rec, nbr, u)) < 1e-12

@@ -330,6 +330,10 @@ int ddgx_block_jacobi_blocks(ddgx_pc pc, void* blocks, void* stream) {
     set_error("ddgx_block_jacobi_blocks: null argument");
     return DDGX_ERR_INVALID;
   }
+  if (pc->pc.type != 0) {
+    set_error("ddgx_block_jacobi_blocks: not a block-Jacobi preconditioner");
+    return DDGX_ERR_INVALID;
+  }
   const Op& o = *pc->pc.op;
   DDGX_HIP(hipMemcpyAsync(blocks, pc->pc.blocks, (size_t)o.K * o.Np * o.Np * sizeof(double),
                           hipMemcpyDeviceToDevice, (hipStream_t)stream));
@@ -341,7 +345,12 @@ int ddgx_pc_apply(ddgx_pc pc, const void* r, void* z, void* stream) {
     set_error("ddgx_pc_apply: null argument");
     return DDGX_ERR_INVALID;
   }
+  if (r == z) {
+    set_error("ddgx_pc_apply: r and z must not alias");
+    return DDGX_ERR_INVALID;
+  }
   const Op& o = *pc->pc.op;
+  if (pc->pc.type == 1) return mg_vcycle(*pc->pc.mg, r, z, nullptr, (hipStream_t)stream);
   return launch_bj_apply(o.N, o.K, pc->pc.dtype, pc->pc.binvT, r, z, (hipStream_t)stream);
 }
 
@@ -349,7 +358,44 @@ int ddgx_pc_destroy(ddgx_pc pc) {
   if (pc == nullptr) return DDGX_OK;
   if (pc->pc.blocks) (void)hipFree(pc->pc.blocks);
   if (pc->pc.binvT) (void)hipFree(pc->pc.binvT);
+  if (pc->pc.mg) mg_free(pc->pc.mg);
   delete pc;
+  return DDGX_OK;
+}
+
+int ddgx_mg_create(int n_levels, const ddgx_op* ops, const ddgx_op* ops64,
+                   const int32_t* transfer_kind, const double* const* transfer,
+                   const ddgx_mg_cfg* cfg, void* stream, ddgx_pc* out) {
+  if (ops == nullptr || ops64 == nullptr || cfg == nullptr || out == nullptr ||
+      (n_levels > 1 && (transfer_kind == nullptr || transfer == nullptr))) {
+    set_error("ddgx_mg_create: null argument");
+    return DDGX_ERR_INVALID;
+  }
+  if (n_levels < 1 || n_levels > 16) {
+    set_error("ddgx_mg_create: n_levels must be 1..16");
+    return DDGX_ERR_INVALID;
+  }
+  std::vector<Op*> lv(n_levels), lv64(n_levels);
+  for (int l = 0; l < n_levels; ++l) {
+    if (ops[l] == nullptr || ops64[l] == nullptr) {
+      set_error("ddgx_mg_create: null level operator");
+      return DDGX_ERR_INVALID;
+    }
+    lv[l] = &ops[l]->op;
+    lv64[l] = &ops64[l]->op;
+  }
+  MG* mg = nullptr;
+  const int rc = mg_create(n_levels, lv.data(), lv64.data(), transfer_kind, transfer, *cfg,
+                           (hipStream_t)stream, &mg);
+  if (rc != DDGX_OK) return rc;
+  auto* h = new ddgx_pc_s();
+  h->pc.type = 1;
+  h->pc.op = lv[0];
+  h->pc.dtype = lv[0]->dtype;
+  h->pc.mg = mg;
+  h->pc.K = lv[0]->K;
+  h->pc.N = lv[0]->N;
+  *out = h;
   return DDGX_OK;
 }
 

@@ -322,16 +322,17 @@ template <typename T>
 static int pcg_t(Op& op, double alpha, double shift, PC* pc, const void* b, void* x,
                  const ddgx_cg_cfg& cfg, ddgx_cg_info* info, hipStream_t s) {
   const size_t n = (size_t)op.K * op.Np;
-  const bool bj = pc != nullptr;
+  const bool has_pc = pc != nullptr;
+  const bool bj = has_pc && pc->type == 0;  // block-Jacobi: fused into the update
   CGScalars& init = *op.scal_host;  // pinned staging buffer
   init = CGScalars{};
   init.tol = cfg.tol;
   init.atol = cfg.atol;
   init.maxiter = cfg.maxiter;
-  init.use_pc = bj ? 1 : 0;
+  init.use_pc = has_pc ? 1 : 0;
   DDGX_HIP(hipMemcpyAsync(op.scal, &init, sizeof(CGScalars), hipMemcpyHostToDevice, s));
   const int* done = &op.scal->done;
-  void* z = bj ? op.z : op.r;  // without a preconditioner z aliases r
+  void* z = has_pc ? op.z : op.r;  // without a preconditioner z aliases r
 
   // atol2 from <b,b>
   CHECK(dot2<T>(op, b, b, nullptr, nullptr, nullptr, s));
@@ -342,6 +343,7 @@ static int pcg_t(Op& op, double alpha, double shift, PC* pc, const void* b, void
                      (const T*)op.Ap, (T*)op.r, n);
   DDGX_HIP(hipGetLastError());
   if (bj) CHECK(launch_bj_apply(op.N, op.K, op.dtype, pc->binvT, op.r, op.z, s));
+  else if (has_pc) CHECK(mg_vcycle(*pc->mg, op.r, op.z, nullptr, s));
   hipLaunchKernelGGL((copy_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (const T*)z,
                      (T*)op.p, n);
   DDGX_HIP(hipGetLastError());
@@ -363,8 +365,12 @@ static int pcg_t(Op& op, double alpha, double shift, PC* pc, const void* b, void
       } else {
         hipLaunchKernelGGL((update_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (T*)x,
                            (T*)op.r, (const T*)op.p, (const T*)op.Ap,
-                           (const CGScalars*)op.scal, n, op.partials);
+                           (const CGScalars*)op.scal, n, has_pc ? nullptr : op.partials);
         DDGX_HIP(hipGetLastError());
+        if (has_pc) {
+          CHECK(mg_vcycle(*pc->mg, op.r, op.z, done, s));
+          CHECK(dot2<T>(op, op.r, op.z, op.r, op.r, done, s));
+        }
       }
       CHECK(finalize(op, op.grid_vec, F_BETA, s));
       hipLaunchKernelGGL((pupdate_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (T*)op.p,

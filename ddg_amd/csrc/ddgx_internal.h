@@ -156,10 +156,12 @@ struct MGLevel {
 struct MG {
   int dtype = 0, nu = 2;
   double omega = 0.7;
+  double alpha = 0.0;
   MGLevel* lv = nullptr;
   int nlev = 0;
-  double* Ainv = nullptr;  // coarsest level dense inverse (float64, symmetric)
-  int nc = 0;
+  double* Ainv = nullptr;       // coarsest level dense inverse (float64), n x n
+  double* gemv_part = nullptr;  // coarse solve partial sums
+  int nc = 0;                   // unknowns of the coarsest level
 };
 
 void set_error(const std::string& msg);
@@ -175,8 +177,10 @@ int hip_fail(hipError_t e, const char* what);
 int launch_apply(const Op& op, const void* u, void* y, const void* udir, double alpha,
                  double shift, int mode, double* partial, const int* skip, hipStream_t s,
                  const void* bvec = nullptr);
-int pc_apply(PC& pc, const void* r, void* z, hipStream_t s);
-int mg_vcycle(MG& mg, const void* b, void* x, hipStream_t s);
+int mg_vcycle(MG& mg, const void* b, void* x, const int* skip, hipStream_t s);
+int mg_create(int nlev, Op* const* ops, Op* const* ops64, const int* tkind,
+              const double* const* P, const ddgx_mg_cfg& cfg, hipStream_t s, MG** out);
+void mg_free(MG* mg);
 int launch_rhs(const Op& op, const void* f, const void* udir, const void* q, void* rhs,
                hipStream_t s);
 int apply_grid(int N, int dtype);

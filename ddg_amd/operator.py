@@ -268,8 +268,12 @@ class SipgOperator:
         info = _lib.CGInfo()
         pch = None
         if pc is not None:
-            if not isinstance(pc, BlockJacobi):
-                raise TypeError("pc must be a BlockJacobi of this operator")
+            if not isinstance(pc, DevicePreconditioner):
+                raise TypeError("pc must be a device preconditioner (block-Jacobi or multigrid)")
+            if pc.K != self.K or pc.N != self.N:
+                raise ValueError("preconditioner was built for a different mesh/order")
+            if pc.dtype != self.dtype:
+                raise ValueError("preconditioner dtype differs from the operator dtype")
             pch = pc.handle
         _lib.check(_lib.lib().ddgx_pcg_solve(self._h, float(alpha), float(shift), pch,
                                              _ptr(b), _ptr(x), C.byref(cfg), C.byref(info),
@@ -278,20 +282,14 @@ class SipgOperator:
                    "residual2": info.residual2, "atol2": info.atol2}
 
 
-class BlockJacobi:
-    """Per-element inverse diagonal blocks on the device (sipg.py:707-805)."""
+class DevicePreconditioner:
+    """A preconditioner handle of libddgx: callable ``z = M^{-1} r`` on device
+    vectors (the reference's preconditioner protocol, sipg.py:726-729,
+    multigrid.py:246) and accepted by ``SipgOperator.pcg``."""
 
-    def __init__(self, op64: SipgOperator, alpha=0.0, dtype=None):
-        if op64.dtype != torch.float64:
-            raise ValueError("block-Jacobi blocks are built from a float64 operator")
-        self.op = op64
-        self.alpha = float(alpha)
-        self.dtype = op64.dtype if dtype is None else dtype
-        code = _lib.DDGX_F64 if self.dtype == torch.float64 else _lib.DDGX_F32
-        h = C.c_void_p()
-        _lib.check(_lib.lib().ddgx_block_jacobi_create(op64.handle, self.alpha, code, _stream(),
-                                                       C.byref(h)), "ddgx_block_jacobi_create")
-        self._h = h
+    _h = None
+    K = N = 0
+    dtype = torch.float64
 
     @property
     def handle(self):
@@ -308,17 +306,37 @@ class BlockJacobi:
         except Exception:
             pass
 
+    def __call__(self, r, z=None):
+        if not (isinstance(r, torch.Tensor) and r.is_cuda):
+            raise TypeError("r must be a torch tensor on the HIP device")
+        if r.dtype != self.dtype or r.numel() != self.K * ((self.N + 1) * (self.N + 2) // 2) \
+                or not r.is_contiguous():
+            raise ValueError("vector dtype/size differs from the preconditioner's")
+        z = torch.empty_like(r) if z is None else z
+        _lib.check(_lib.lib().ddgx_pc_apply(self._h, _ptr(r), _ptr(z), _stream()),
+                   "ddgx_pc_apply")
+        return z
+
+
+class BlockJacobi(DevicePreconditioner):
+    """Per-element inverse diagonal blocks on the device (sipg.py:707-805)."""
+
+    def __init__(self, op64: SipgOperator, alpha=0.0, dtype=None):
+        if op64.dtype != torch.float64:
+            raise ValueError("block-Jacobi blocks are built from a float64 operator")
+        self.op = op64
+        self.K, self.N = op64.K, op64.N
+        self.alpha = float(alpha)
+        self.dtype = op64.dtype if dtype is None else dtype
+        code = _lib.DDGX_F64 if self.dtype == torch.float64 else _lib.DDGX_F32
+        h = C.c_void_p()
+        _lib.check(_lib.lib().ddgx_block_jacobi_create(op64.handle, self.alpha, code, _stream(),
+                                                       C.byref(h)), "ddgx_block_jacobi_create")
+        self._h = h
+
     def blocks(self):
         K, Np = self.op.K, self.op.Np
         out = torch.empty((K, Np, Np), dtype=torch.float64, device=self.op.device)
         _lib.check(_lib.lib().ddgx_block_jacobi_blocks(self._h, _ptr(out), _stream()),
                    "ddgx_block_jacobi_blocks")
         return out
-
-    def __call__(self, r, z=None):
-        if r.dtype != self.dtype:
-            raise ValueError("vector dtype differs from the preconditioner dtype")
-        z = torch.empty_like(r) if z is None else z
-        _lib.check(_lib.lib().ddgx_pc_apply(self._h, _ptr(r), _ptr(z), _stream()),
-                   "ddgx_pc_apply")
-        return z

@@ -274,3 +274,23 @@ def p_interp_matrix(n_lo: int, n_hi: int) -> np.ndarray:
     (multigrid.py:106-114)."""
     lo, hi = ref_element(n_lo), ref_element(n_hi)
     return vandermonde_2d(n_lo, hi.r, hi.s) @ lo.invV
+
+
+def h_child_interp_matrices(n: int) -> np.ndarray:
+    """(4, Np, Np): a parent element's nodal values -> the nodal values of its
+    four red children (multigrid.py:137-163).  Child c has vertices
+    (A, mAB, mCA), (mAB, B, mBC), (mCA, mBC, C), (mAB, mBC, mCA) of the
+    reference triangle A=(-1,-1), B=(1,-1), C=(-1,1) -- the child order of the
+    refinement forest (refinement.py:359-382)."""
+    ref = ref_element(n)
+    A, B, Cv = np.array([-1.0, -1.0]), np.array([1.0, -1.0]), np.array([-1.0, 1.0])
+    mAB, mBC, mCA = (A + B) / 2, (B + Cv) / 2, (Cv + A) / 2
+    kids = ((A, mAB, mCA), (mAB, B, mBC), (mCA, mBC, Cv), (mAB, mBC, mCA))
+    # barycentric weights of the child's own nodes
+    l0, l1, l2 = -(ref.r + ref.s) / 2, (1 + ref.r) / 2, (1 + ref.s) / 2
+    out = np.empty((4, ref.Np, ref.Np))
+    for c, (P0, P1, P2) in enumerate(kids):
+        pr = l0 * P0[0] + l1 * P1[0] + l2 * P2[0]
+        ps = l0 * P0[1] + l1 * P1[1] + l2 * P2[1]
+        out[c] = vandermonde_2d(n, pr, ps) @ ref.invV
+    return out

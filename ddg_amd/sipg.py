@@ -29,7 +29,7 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
-from .operator import BlockJacobi, SipgOperator
+from .operator import BlockJacobi, DevicePreconditioner, SipgOperator
 from .reference import ref_element
 
 BC_INTERIOR = 0
@@ -228,10 +228,10 @@ def _run_cg(op, b, x0, iterative, preconditioner, alpha=0.0, shift=0.0):
         raise ValueError(f"iterative method {iterative.method!r} not supported for SPD")
     pc = None
     if preconditioner is not None:
-        if not isinstance(preconditioner, BlockJacobi):
+        if not isinstance(preconditioner, DevicePreconditioner):
             raise TypeError("preconditioner must come from make_*_block_jacobi_preconditioner "
                             "or ddg_amd.multigrid")
-        if preconditioner.op.K != op.K or preconditioner.op.N != op.N:
+        if preconditioner.K != op.K or preconditioner.N != op.N:
             raise ValueError("preconditioner was built for a different mesh")
         pc = preconditioner
     return op.pcg(b, x0, alpha=alpha, shift=shift, pc=pc, tol=iterative.tol,

@@ -103,11 +103,41 @@ int ddgx_sipg_rhs(ddgx_op op, const void* f, const void* u_dir,
  * make_helmholtz_block_jacobi_preconditioner (sipg.py:671-805). */
 int ddgx_block_jacobi_create(ddgx_op op64, double alpha, int dtype, void* stream,
                              ddgx_pc* out);
-/* Copy the (K, Np, Np) diagonal blocks (before inversion) to a device buffer. */
+/* Copy the (K, Np, Np) diagonal blocks (before inversion) of a block-Jacobi
+ * preconditioner to a device buffer. */
 int ddgx_block_jacobi_blocks(ddgx_pc pc, void* blocks, void* stream);
-/* z = M^{-1} r  (the preconditioner callable, sipg.py:711-713). */
+/* z = M^{-1} r  (the preconditioner callable: sipg.py:711-713 for
+ * block-Jacobi, multigrid.py:246 for a V-cycle). */
 int ddgx_pc_apply(ddgx_pc pc, const void* r, void* z, void* stream);
 int ddgx_pc_destroy(ddgx_pc pc);
+
+/* Multigrid V-cycle settings (make_sipg_mg_preconditioner keywords,
+ * src/ddg/dg2d/multigrid.py:253-264). */
+typedef struct {
+  int nu_smooth;   /* block-Jacobi sweeps before and after the coarse correction */
+  double omega;    /* smoother damping                                           */
+  double alpha;    /* every level applies A + alpha*M                            */
+  int regularize;  /* add eps*max|diag| to the coarsest matrix                   */
+  double eps;
+} ddgx_mg_cfg;
+
+/* Multigrid V-cycle preconditioner over n_levels operators, fine to coarse
+ * (_make_level / _assemble_vcycle / make_sipg_mg_preconditioner /
+ * make_sipg_hpmg_preconditioner, multigrid.py:87-99, 211-367).
+ *   ops[l]     level operator in the precision of the cycle (one dtype for
+ *              all levels; the preconditioner applies to vectors of it)
+ *   ops64[l]   float64 operator of the same level (the smoother blocks and
+ *              the coarsest matrix are built from it; may equal ops[l])
+ *   transfer_kind[l], l < n_levels-1: 0 = p (same mesh, order N_{l+1} <=
+ *              N_l), 1 = h (same order, element 4k+c of level l is child c
+ *              of element k of level l+1)
+ *   transfer[l] host float64, row-major: p: (Np_l x Np_{l+1}) interpolation
+ *              matrix; h: the (4, Np, Np) child interpolation matrices
+ * The coarsest level is solved with a dense float64 inverse (at most 8192
+ * unknowns).  The level operators must outlive the preconditioner. */
+int ddgx_mg_create(int n_levels, const ddgx_op* ops, const ddgx_op* ops64,
+                   const int32_t* transfer_kind, const double* const* transfer,
+                   const ddgx_mg_cfg* cfg, void* stream, ddgx_pc* out);
 
 /* Preconditioned CG on (A + alpha*M + shift*I) x = b with the stopping rule
  * of jax.scipy.sparse.linalg.cg as called at sipg.py:603-609, 909-915.

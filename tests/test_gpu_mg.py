@@ -1,0 +1,161 @@
+"""Device multigrid V-cycle and MG-preconditioned CG against the oracle.
+
+Bars: one V-cycle matches the oracle's to 1e-10 relative in float64 (the
+coarsest solve is a dense inverse here and an LU solve in the oracle, so
+agreement is to round-off times the coarse condition number), 1e-3 in
+float32; MG-PCG solutions match a direct solve to 1e-7
+(test/test_multigrid_2d.py:77-96) and the oracle's MG-PCG iteration count
+to within 2."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import unit_square, rel
+from oracle import cg as ocg
+from oracle import mesh as om
+from oracle import multigrid as omg
+from oracle import sipg as S
+from ddg_amd import mesh as pmesh
+from ddg_amd import multigrid as MG
+from ddg_amd import sipg as D
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _all_dir(m):
+    return pmesh.all_dirichlet(m)
+
+
+@pytest.mark.parametrize("N,alpha,bcname", [(4, 0.0, "dir"), (3, 50.0, "dir"),
+                                            (4, 0.0, "mixed"), (2, 7.5, "mixed")])
+def test_pmg_vcycle_matches_oracle(dev, N, alpha, bcname):
+    pm, omesh = unit_square(8, 8, N, jitter=0.1, seed=4)
+    bc = _all_dir(pm) if bcname == "dir" else pmesh.dirichlet_x_neumann_y(pm)
+    _say(f"pmg vcycle N={N} alpha={alpha} bc={bcname}")
+    mg = MG.make_sipg_mg_preconditioner(pm, bc_types=bc, alpha=alpha)
+    ref = omg.make_sipg_mg_preconditioner(omesh, bc_types=bc, alpha=alpha)
+    r = np.random.default_rng(5).standard_normal(pm.K * pm.Np)
+    z = mg(torch.as_tensor(r, device=dev)).cpu().numpy()
+    assert rel(z, ref(r)) < 1e-10
+
+
+def test_pmg_custom_degrees_and_smoother(dev):
+    pm, omesh = unit_square(6, 5, 4, jitter=0.1, seed=6)
+    bc = _all_dir(pm)
+    _say("pmg degrees 4,2,1 nu=1 omega=0.5")
+    mg = MG.make_sipg_mg_preconditioner(pm, bc_types=bc, degrees=[4, 2, 1], nu_smooth=1,
+                                        omega=0.5)
+    ref = omg.make_sipg_mg_preconditioner(omesh, bc_types=bc, degrees=[4, 2, 1], nu_smooth=1,
+                                          omega=0.5)
+    r = np.random.default_rng(7).standard_normal(pm.K * pm.Np)
+    assert rel(mg(torch.as_tensor(r, device=dev)).cpu().numpy(), ref(r)) < 1e-10
+    _say("pmg nu=0")
+    mg0 = MG.make_sipg_mg_preconditioner(pm, bc_types=bc, degrees=[4, 1], nu_smooth=0)
+    ref0 = omg.make_sipg_mg_preconditioner(omesh, bc_types=bc, degrees=[4, 1], nu_smooth=0)
+    assert rel(mg0(torch.as_tensor(r, device=dev)).cpu().numpy(), ref0(r)) < 1e-10
+
+
+def test_single_level_is_direct_solve(dev):
+    pm, omesh = unit_square(3, 3, 2)
+    bc = _all_dir(pm)
+    _say("single-level mg")
+    mg = MG.make_sipg_mg_preconditioner(pm, bc_types=bc, p_coarse=2)
+    A = S.sipg_matrix_2d(omesh, bc_types=bc)
+    b = np.random.default_rng(8).standard_normal(pm.K * pm.Np)
+    x = mg(torch.as_tensor(b, device=dev)).cpu().numpy()
+    assert rel(A @ x, b) < 1e-10
+
+
+@pytest.mark.parametrize("alpha", [0.0, 20.0])
+def test_hpmg_vcycle_matches_oracle(dev, alpha):
+    VX, VY, EToV = pmesh.uniform_rectangle_mesh_2d(0.0, 1.0, 0.0, 1.0, 2, 2)
+    base = pmesh.build_mesh_2d(VX, VY, EToV, 3)
+    # jitter the finest level's interior vertices; coarse levels share them
+    hier = omg.forest_hierarchy(VX, VY, EToV, 2)
+    FX, FY = pmesh.jitter_vertices(hier[-1][0], hier[-1][1], 1.0 / 8, 0.1, 9)
+    _say(f"hpmg vcycle alpha={alpha}")
+    mg, finest = MG.make_sipg_hpmg_preconditioner(base, 2, bc_fn=pmesh.dirichlet_x_neumann_y,
+                                                  alpha=alpha, fine_vertices=(FX, FY))
+    ref, ofinest = omg.make_sipg_hpmg_preconditioner(
+        VX, VY, EToV, 3, 2, bc_fn=lambda m: pmesh.dirichlet_x_neumann_y(m), alpha=alpha,
+        fine_vertices=(FX, FY))
+    assert finest.K == ofinest.K == 128
+    np.testing.assert_array_equal(np.asarray(finest.EToV), ofinest.EToV)
+    assert mg.sizes == [128, 128, 128, 32, 8] and mg.degrees == [3, 2, 1, 1, 1]
+    r = np.random.default_rng(10).standard_normal(finest.K * finest.Np)
+    assert rel(mg(torch.as_tensor(r, device=dev)).cpu().numpy(), ref(r)) < 1e-10
+
+
+def test_regularized_pure_neumann(dev):
+    pm, omesh = unit_square(6, 6, 3, jitter=0.1, seed=11)
+    bc = np.zeros((pm.K, 3), dtype=np.int32)
+    bc[pmesh.boundary_faces(pm)] = 2
+    _say("pmg regularized neumann")
+    mg = MG.make_sipg_mg_preconditioner(pm, bc_types=bc, regularize=True, eps=1e-6)
+    ref = omg.make_sipg_mg_preconditioner(omesh, bc_types=bc, regularize=True, eps=1e-6)
+    r = np.random.default_rng(12).standard_normal(pm.K * pm.Np)
+    # the regularised coarse matrix has condition ~1/eps: compare at 1e-7
+    assert rel(mg(torch.as_tensor(r, device=dev)).cpu().numpy(), ref(r)) < 1e-7
+
+
+def test_float32_vcycle(dev):
+    pm, omesh = unit_square(8, 8, 4, jitter=0.1, seed=13)
+    bc = pmesh.dirichlet_x_neumann_y(pm)
+    _say("pmg float32 vcycle")
+    mg = MG.make_sipg_mg_preconditioner(pm, bc_types=bc, dtype=torch.float32)
+    ref = omg.make_sipg_mg_preconditioner(omesh, bc_types=bc)
+    r = np.random.default_rng(14).standard_normal(pm.K * pm.Np)
+    z = mg(torch.as_tensor(r, dtype=torch.float32, device=dev)).double().cpu().numpy()
+    assert rel(z, ref(r.astype(np.float32).astype(np.float64))) < 1e-3
+
+
+@pytest.mark.parametrize("alpha", [0.0, 50.0])
+def test_mg_pcg_matches_direct_and_oracle(dev, alpha):
+    """test_multigrid_2d.py:77-96, 130-150 on the device, plus the iteration
+    count of the oracle's JAX-semantics CG with the oracle V-cycle."""
+    pm, omesh = unit_square(8, 8, 4)
+    bc = _all_dir(pm)
+    n = pm.K * pm.Np
+    b = np.random.default_rng(0).standard_normal(n)
+    x_direct = np.linalg.solve(S.sipg_matrix_2d(omesh, bc_types=bc, alpha=alpha), b)
+    ref = omg.make_sipg_mg_preconditioner(omesh, bc_types=bc, alpha=alpha)
+    _, info_o = ocg.cg(S.flat_apply(omesh, bc, alpha=alpha), b, tol=1e-10, M=ref)
+    _say(f"mg pcg alpha={alpha}")
+    mg = MG.make_sipg_mg_preconditioner(pm, bc_types=bc, alpha=alpha)
+    op = D.get_operator(pm, bc)
+    x, info = op.pcg(torch.as_tensor(b, device=dev), alpha=alpha, pc=mg, tol=1e-10)
+    assert info["converged"]
+    assert abs(info["iterations"] - info_o["iterations"]) <= 2
+    x = x.cpu().numpy()
+    assert np.linalg.norm(x - x_direct) / np.linalg.norm(x_direct) < 1e-7
+
+
+def test_solve_with_hpmg_preconditioner(dev):
+    """Drop-in solve with an hp-MG preconditioner: Poisson with Dirichlet and
+    Neumann data on a forest mesh, against the oracle LU solve."""
+    VX, VY, EToV = pmesh.uniform_rectangle_mesh_2d(0.0, 1.0, 0.0, 1.0, 2, 2)
+    base = pmesh.build_mesh_2d(VX, VY, EToV, 4)
+    _say("hpmg drop-in solve")
+    mg, finest = MG.make_sipg_hpmg_preconditioner(base, 2, bc_fn=pmesh.dirichlet_x_neumann_y,
+                                                  degrees=[4, 2, 1])
+    bc = pmesh.dirichlet_x_neumann_y(finest)
+    ue = np.exp(finest.x) * np.cos(3 * finest.y)
+    f = 8.0 * ue
+    xf = finest.x.reshape(-1, order="F")[finest.vmapM]
+    yf = finest.y.reshape(-1, order="F")[finest.vmapM]
+    q = (finest.nx.reshape(-1, order="F") * np.exp(xf) * np.cos(3 * yf)
+         - finest.ny.reshape(-1, order="F") * 3 * np.exp(xf) * np.sin(3 * yf))
+    q = q.reshape((3 * finest.Nfp, finest.K), order="F")
+    u, info = D.solve_sipg_poisson_2d(finest, f, bc_types=bc, u_dir=ue, q_neu=q,
+                                      iterative=D.IterativeConfig(tol=1e-10, maxiter=500),
+                                      preconditioner=mg, return_info=True)
+    assert info["converged"] and info["iterations"] < 200
+    ofin = om.build_mesh_2d(finest.VX, finest.VY, finest.EToV, 4)
+    uo = S.solve_sipg_poisson_2d(ofin, f, bc_types=bc, u_dir=ue, q_neu=q)
+    assert rel(u, uo) < 1e-7
+    # discretisation error against the manufactured solution
+    assert np.sqrt(np.sum(finest.J * (u - ue) ** 2) / np.sum(finest.J * ue ** 2)) < 1e-3

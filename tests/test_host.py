@@ -113,3 +113,35 @@ def test_mfma_fragments_reproduce_apply(N, dtype):
     u = np.random.default_rng(N).standard_normal((pm.K, pm.Np))
     y = model_apply_mfma(N, tab, mfma_fragments(ref, dtype), rec, nbr, u, dtype)
     assert rel(y, model_apply(N, tab, rec, nbr, u)) < 1e-12
+
+
+def test_forest_hierarchy_matches_oracle():
+    """The vectorised refinement (ddg_amd.mesh.refine_uniform) numbers
+    vertices and elements exactly as the loop restatement of the reference's
+    refinement forest does."""
+    from ddg_amd import mesh as pmesh
+    from ddg_amd.multigrid import build_h_meshes
+    from oracle import multigrid as omg
+    VX, VY, EToV = pmesh.uniform_rectangle_mesh_2d(0.0, 1.0, 0.0, 2.0, 3, 2)
+    base = pmesh.build_mesh_2d(VX, VY, EToV, 1)
+    ours = build_h_meshes(base, 3)
+    ref = omg.forest_hierarchy(VX, VY, EToV, 3)
+    assert len(ours) == len(ref) == 4
+    for (a, b, c), (x, y, z) in zip(ours, ref):
+        np.testing.assert_array_equal(a, x)
+        np.testing.assert_array_equal(b, y)
+        np.testing.assert_array_equal(c, z)
+
+
+def test_transfer_matrices_match_oracle():
+    from ddg_amd.reference import h_child_interp_matrices, p_interp_matrix
+    from oracle import mesh as om
+    from oracle import multigrid as omg
+    VX, VY, EToV = om.uniform_rectangle_mesh_2d(0.0, 1.0, 0.0, 1.0, 1, 1)
+    ms = {n: om.build_mesh_2d(VX, VY, EToV, n) for n in range(1, 7)}
+    for lo, hi in [(1, 2), (2, 4), (1, 4), (3, 6), (5, 6)]:
+        np.testing.assert_allclose(p_interp_matrix(lo, hi), omg.p_interp_matrix(ms[lo], ms[hi]),
+                                   atol=1e-13)
+    for n in range(1, 7):
+        np.testing.assert_allclose(h_child_interp_matrices(n),
+                                   omg.h_child_interp_matrices(ms[n]), atol=1e-13)

@@ -20,7 +20,7 @@ EXPORTS = (
     "ddgx_last_error", "ddgx_table_size", "ddgx_op_create", "ddgx_op_destroy",
     "ddgx_op_info", "ddgx_sipg_apply", "ddgx_sipg_rhs", "ddgx_block_jacobi_create",
     "ddgx_block_jacobi_blocks", "ddgx_pc_apply", "ddgx_pc_destroy", "ddgx_pcg_solve",
-    "ddgx_dot", "ddgx_mg_create",
+    "ddgx_dot", "ddgx_mg_create", "ddgx_ir_solve",
 )
 
 
@@ -37,6 +37,16 @@ class CGInfo(C.Structure):
 class MGConfig(C.Structure):
     _fields_ = [("nu_smooth", C.c_int), ("omega", C.c_double), ("alpha", C.c_double),
                 ("regularize", C.c_int), ("eps", C.c_double)]
+
+
+class IRConfig(C.Structure):
+    _fields_ = [("tol", C.c_double), ("atol", C.c_double), ("max_outer", C.c_int),
+                ("inner_tol", C.c_double), ("inner_maxiter", C.c_int), ("check_every", C.c_int)]
+
+
+class IRInfo(C.Structure):
+    _fields_ = [("outer", C.c_int), ("inner_total", C.c_int), ("converged", C.c_int),
+                ("residual2", C.c_double), ("atol2", C.c_double)]
 
 
 class DdgxError(RuntimeError):
@@ -68,6 +78,8 @@ def lib():
         "ddgx_pcg_solve": (i32, [vp, dbl, dbl, vp, vp, vp, C.POINTER(CGConfig),
                                  C.POINTER(CGInfo), vp]),
         "ddgx_dot": (i32, [vp, vp, vp, C.POINTER(dbl), vp]),
+        "ddgx_ir_solve": (i32, [vp, vp, dbl, dbl, vp, vp, vp, C.POINTER(IRConfig),
+                                C.POINTER(IRInfo), vp]),
         "ddgx_mg_create": (i32, [i32, C.POINTER(vp), C.POINTER(vp), C.POINTER(C.c_int32),
                                  C.POINTER(C.POINTER(dbl)), C.POINTER(MGConfig), vp,
                                  C.POINTER(vp)]),

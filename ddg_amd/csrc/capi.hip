@@ -67,7 +67,7 @@ static int upload_as(int dtype, const double* src, size_t n, void** dst) {
 
 static void free_op(Op& o) {
   void* ptrs[] = {o.tab, o.rec, o.nbr, o.fcode, o.perm, o.hid, o.hab, o.fsj,
-                  o.r, o.p, o.z, o.Ap, o.partials, o.scal};
+                  o.r, o.p, o.z, o.Ap, o.partials, o.ir_b, o.ir_x, o.scal};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (o.scal_host) (void)hipHostFree(o.scal_host);
@@ -415,6 +415,31 @@ int ddgx_pcg_solve(ddgx_op op, double alpha, double shift, ddgx_pc pc, const voi
   }
   return pcg_solve(op->op, alpha, shift, pc ? &pc->pc : nullptr, b, x, *cfg, info,
                    (hipStream_t)stream);
+}
+
+int ddgx_ir_solve(ddgx_op op64, ddgx_op op32, double alpha, double shift, ddgx_pc pc32,
+                  const void* b, void* x, const ddgx_ir_cfg* cfg, ddgx_ir_info* info,
+                  void* stream) {
+  if (op64 == nullptr || op32 == nullptr || b == nullptr || x == nullptr || cfg == nullptr) {
+    set_error("ddgx_ir_solve: null argument");
+    return DDGX_ERR_INVALID;
+  }
+  const Op &a = op64->op, &c = op32->op;
+  if (a.dtype != DDGX_F64 || c.dtype != DDGX_F32 || a.K != c.K || a.N != c.N) {
+    set_error("ddgx_ir_solve: needs a float64 and a float32 operator of one mesh and order");
+    return DDGX_ERR_INVALID;
+  }
+  if (pc32 != nullptr && (pc32->pc.dtype != DDGX_F32 || pc32->pc.op->K != c.K ||
+                          pc32->pc.op->N != c.N)) {
+    set_error("ddgx_ir_solve: the preconditioner must be float32 and of the same mesh/order");
+    return DDGX_ERR_INVALID;
+  }
+  if (cfg->max_outer < 0 || cfg->inner_maxiter < 0 || !(cfg->inner_tol > 0.0)) {
+    set_error("ddgx_ir_solve: bad refinement settings");
+    return DDGX_ERR_INVALID;
+  }
+  return ir_solve(op64->op, op32->op, alpha, shift, pc32 ? &pc32->pc : nullptr, b, x, *cfg,
+                  info, (hipStream_t)stream);
 }
 
 int ddgx_dot(ddgx_op op, const void* a, const void* b, double* out, void* stream) {

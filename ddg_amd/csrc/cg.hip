@@ -15,6 +15,8 @@
 // Reductions are two-stage and fixed-order (per-workgroup partials in
 // float64, then one workgroup sums them), so results are bitwise repeatable.
 
+#include <cmath>
+
 #include "ddgx_internal.h"
 
 namespace ddgx {
@@ -397,6 +399,66 @@ int pcg_solve(Op& op, double alpha, double shift, PC* pc, const void* b, void* x
   }
   if (op.dtype == DDGX_F64) return pcg_t<double>(op, alpha, shift, pc, b, x, cfg, info, s);
   return pcg_t<float>(op, alpha, shift, pc, b, x, cfg, info, s);
+}
+
+// rs = (float) (scale * r)
+__global__ void __launch_bounds__(256)
+scale_down_kernel(const double* __restrict__ r, float* __restrict__ rs, double scale, size_t n) {
+  const size_t stride = (size_t)gridDim.x * 256;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
+    rs[i] = (float)(scale * r[i]);
+}
+
+// x += scale * (double) d
+__global__ void __launch_bounds__(256)
+refine_update_kernel(double* __restrict__ x, const float* __restrict__ d, double scale, size_t n) {
+  const size_t stride = (size_t)gridDim.x * 256;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
+    x[i] += scale * (double)d[i];
+}
+
+int ir_solve(Op& op64, Op& op32, double alpha, double shift, PC* pc32, const void* b, void* x,
+             const ddgx_ir_cfg& cfg, ddgx_ir_info* info, hipStream_t s) {
+  CHECK(ensure_workspace(op64));
+  CHECK(ensure_workspace(op32));
+  const size_t n = (size_t)op64.K * op64.Np;
+  if (op32.ir_b == nullptr) {
+    DDGX_HIP(hipMalloc(&op32.ir_b, n * sizeof(float)));
+    DDGX_HIP(hipMalloc(&op32.ir_x, n * sizeof(float)));
+  }
+  double bb = 0.0;
+  CHECK(dot(op64, b, b, &bb, s));
+  const double t = cfg.tol * cfg.tol * bb, a = cfg.atol * cfg.atol;
+  const double atol2 = t > a ? t : a;
+  ddgx_cg_cfg icfg{cfg.inner_tol, 0.0, cfg.inner_maxiter, cfg.check_every};
+  int outer = 0, inner = 0;
+  double rr = 0.0;
+  double* r = (double*)op64.r;
+  while (true) {
+    CHECK(launch_apply(op64, x, r, nullptr, alpha, shift, M_RESID, nullptr, nullptr, s, b));
+    CHECK(dot(op64, r, r, &rr, s));
+    if (!(rr > atol2) || outer >= cfg.max_outer) break;
+    const double nr = std::sqrt(rr);
+    hipLaunchKernelGGL(scale_down_kernel, dim3(op64.grid_vec), dim3(256), 0, s, r,
+                       (float*)op32.ir_b, 1.0 / nr, n);
+    DDGX_HIP(hipGetLastError());
+    DDGX_HIP(hipMemsetAsync(op32.ir_x, 0, n * sizeof(float), s));
+    ddgx_cg_info ii{};
+    CHECK(pcg_solve(op32, alpha, shift, pc32, op32.ir_b, op32.ir_x, icfg, &ii, s));
+    inner += ii.iterations;
+    hipLaunchKernelGGL(refine_update_kernel, dim3(op64.grid_vec), dim3(256), 0, s, (double*)x,
+                       (const float*)op32.ir_x, nr, n);
+    DDGX_HIP(hipGetLastError());
+    ++outer;
+  }
+  if (info != nullptr) {
+    info->outer = outer;
+    info->inner_total = inner;
+    info->residual2 = rr;
+    info->atol2 = atol2;
+    info->converged = rr <= atol2 ? 1 : 0;
+  }
+  return DDGX_OK;
 }
 
 int dot(Op& op, const void* a, const void* b, double* out, hipStream_t s) {

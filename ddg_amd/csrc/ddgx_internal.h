@@ -122,6 +122,8 @@ struct Op {
   void* z = nullptr;
   void* Ap = nullptr;
   double* partials = nullptr;  // 2 * max(grid) doubles
+  void* ir_b = nullptr;        // refinement: scaled residual / correction (op dtype)
+  void* ir_x = nullptr;
   CGScalars* scal = nullptr;
   CGScalars* scal_host = nullptr;  // pinned
 };
@@ -193,6 +195,8 @@ int launch_bj_apply(int N, int64_t K, int dtype, const void* binvT, const void* 
 int pcg_solve(Op& op, double alpha, double shift, PC* pc, const void* b, void* x,
               const ddgx_cg_cfg& cfg, ddgx_cg_info* info, hipStream_t s);
 int dot(Op& op, const void* a, const void* b, double* out, hipStream_t s);
+int ir_solve(Op& op64, Op& op32, double alpha, double shift, PC* pc32, const void* b, void* x,
+             const ddgx_ir_cfg& cfg, ddgx_ir_info* info, hipStream_t s);
 
 int device_cu_count();
 

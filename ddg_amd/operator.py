@@ -282,6 +282,35 @@ class SipgOperator:
                    "residual2": info.residual2, "atol2": info.atol2}
 
 
+def refine_solve(op64: SipgOperator, op32: SipgOperator, b, x=None, *, alpha=0.0, shift=0.0,
+                 pc32=None, tol=1e-10, atol=0.0, max_outer=20, inner_tol=1e-4,
+                 inner_maxiter=1000, check_every=16):
+    """Mixed-precision iterative refinement: float64 residual and update
+    around float32 PCG solves (``ddgx_ir_solve``).  Returns ``(x, info)``."""
+    if op64.dtype != torch.float64 or op32.dtype != torch.float32:
+        raise ValueError("refine_solve needs a float64 and a float32 operator")
+    if op64.K != op32.K or op64.N != op32.N:
+        raise ValueError("operators differ in mesh or order")
+    b = op64._vec(b, "b")
+    x = torch.zeros_like(b) if x is None else op64._vec(x, "x")
+    pch = None
+    if pc32 is not None:
+        if not isinstance(pc32, DevicePreconditioner) or pc32.dtype != torch.float32:
+            raise ValueError("pc32 must be a float32 device preconditioner")
+        if pc32.K != op32.K or pc32.N != op32.N:
+            raise ValueError("preconditioner was built for a different mesh/order")
+        pch = pc32.handle
+    cfg = _lib.IRConfig(float(tol), float(atol), int(max_outer), float(inner_tol),
+                        int(inner_maxiter), int(check_every))
+    info = _lib.IRInfo()
+    _lib.check(_lib.lib().ddgx_ir_solve(op64.handle, op32.handle, float(alpha), float(shift),
+                                        pch, _ptr(b), _ptr(x), C.byref(cfg), C.byref(info),
+                                        _stream()), "ddgx_ir_solve")
+    return x, {"outer": info.outer, "iterations": info.inner_total,
+               "converged": bool(info.converged), "residual2": info.residual2,
+               "atol2": info.atol2}
+
+
 class DevicePreconditioner:
     """A preconditioner handle of libddgx: callable ``z = M^{-1} r`` on device
     vectors (the reference's preconditioner protocol, sipg.py:726-729,

@@ -29,12 +29,23 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
-from .operator import BlockJacobi, DevicePreconditioner, SipgOperator
+from .operator import BlockJacobi, DevicePreconditioner, SipgOperator, refine_solve
 from .reference import ref_element
 
 BC_INTERIOR = 0
 BC_DIRICHLET = 1
 BC_NEUMANN = 2
+
+
+@dataclass(frozen=True)
+class RefineConfig:
+    """Mixed-precision iterative refinement (an extension: the reference
+    solves in float64 only).  The outer loop keeps the float64 residual and
+    stops by ``IterativeConfig.tol``/``atol``; each correction is a float32
+    PCG solve to ``inner_tol`` with a float32 preconditioner."""
+    inner_tol: float = 1e-4
+    max_outer: int = 20
+    inner_maxiter: int = 1000
 
 
 @dataclass(frozen=True)
@@ -223,9 +234,18 @@ def make_helmholtz_block_jacobi_preconditioner(mesh, *, alpha, nu, bc_types, tau
 # ---------------------------------------------------------------------------
 # solves (sipg.py:536-612, 812-918)
 # ---------------------------------------------------------------------------
-def _run_cg(op, b, x0, iterative, preconditioner, alpha=0.0, shift=0.0):
+def _run_cg(op, b, x0, iterative, preconditioner, alpha=0.0, shift=0.0, refine=None, key=None):
     if iterative.method != "cg":
         raise ValueError(f"iterative method {iterative.method!r} not supported for SPD")
+    if refine is not None:
+        mesh, bc_types, tau_scale = key
+        op32 = get_operator(mesh, bc_types, tau_scale, dtype=torch.float32)
+        if preconditioner is not None and (not isinstance(preconditioner, DevicePreconditioner)
+                                           or preconditioner.dtype != torch.float32):
+            raise TypeError("refine needs a float32 device preconditioner (dtype=torch.float32)")
+        return refine_solve(op, op32, b, x0, alpha=alpha, shift=shift, pc32=preconditioner,
+                            tol=iterative.tol, atol=iterative.atol, max_outer=refine.max_outer,
+                            inner_tol=refine.inner_tol, inner_maxiter=refine.inner_maxiter)
     pc = None
     if preconditioner is not None:
         if not isinstance(preconditioner, DevicePreconditioner):
@@ -240,8 +260,9 @@ def _run_cg(op, b, x0, iterative, preconditioner, alpha=0.0, shift=0.0):
 
 def solve_sipg_poisson_2d(mesh, source, *, bc_types, u_dir=None, q_neu=None, tau_scale=200.0,
                           A_factored=None, iterative=None, x0=None, preconditioner=None,
-                          cub=None, return_info=False):
-    """Solve ``-Delta u = source`` with mixed BCs (sipg.py:812-918)."""
+                          cub=None, return_info=False, refine=None):
+    """Solve ``-Delta u = source`` with mixed BCs (sipg.py:812-918).
+    ``refine=RefineConfig(...)`` runs the mixed-precision refinement."""
     _check_cub(cub)
     op = get_operator(mesh, bc_types, tau_scale)
     f, b = _rhs(op, source, u_dir, q_neu)
@@ -262,15 +283,17 @@ def solve_sipg_poisson_2d(mesh, source, *, bc_types, u_dir=None, q_neu=None, tau
         shift = 1.0e-6 * float(torch.linalg.norm(Ap) / torch.linalg.norm(probe))
         b = b - b.mean()
     xi = None if x0 is None else _field(x0, op).flat.clone()
-    x, info = _run_cg(op, b, xi, iterative, preconditioner, 0.0, shift)
+    x, info = _run_cg(op, b, xi, iterative, preconditioner, 0.0, shift, refine,
+                      (mesh, bc_types, tau_scale))
     u = f.back(x)
     return (u, info) if return_info else u
 
 
 def solve_sipg_helmholtz_2d(mesh, rhs_field, *, alpha, nu, bc_types, u_dir=None, q_neu=None,
                             tau_scale=200.0, A_factored=None, iterative=None, x0=None,
-                            preconditioner=None, cub=None, return_info=False):
-    """Solve ``(alpha M - Delta) u = rhs_field`` (sipg.py:536-612)."""
+                            preconditioner=None, cub=None, return_info=False, refine=None):
+    """Solve ``(alpha M - Delta) u = rhs_field`` (sipg.py:536-612).
+    ``refine=RefineConfig(...)`` runs the mixed-precision refinement."""
     _check_cub(cub)
     op = get_operator(mesh, bc_types, tau_scale)
     f, b = _rhs(op, rhs_field, u_dir, q_neu)
@@ -281,6 +304,7 @@ def solve_sipg_helmholtz_2d(mesh, rhs_field, *, alpha, nu, bc_types, u_dir=None,
         u = f.back(_lu_solve(A_factored, b).to(op.dtype))
         return (u, None) if return_info else u
     xi = None if x0 is None else _field(x0, op).flat.clone()
-    x, info = _run_cg(op, b, xi, iterative, preconditioner, float(alpha), 0.0)
+    x, info = _run_cg(op, b, xi, iterative, preconditioner, float(alpha), 0.0, refine,
+                      (mesh, bc_types, tau_scale))
     u = f.back(x)
     return (u, info) if return_info else u

@@ -148,6 +148,33 @@ int ddgx_pcg_solve(ddgx_op op, double alpha, double shift, ddgx_pc pc,
                    const void* b, void* x, const ddgx_cg_cfg* cfg,
                    ddgx_cg_info* info, void* stream);
 
+/* Mixed-precision iterative refinement (no counterpart in the reference,
+ * whose solves are float64 only; it solves the same systems as
+ * ddgx_pcg_solve):
+ *   r = b - A x (float64); stop when <r,r> <= max(tol^2 <b,b>, atol^2)
+ *   d = PCG_float32(A, r/|r|, pc32) to inner_tol;  x += |r| d (float64)
+ * op64/op32: the same operator in float64 and float32; pc32: float32
+ * preconditioner (block-Jacobi or multigrid) or NULL.  b, x: float64. */
+typedef struct {
+  double tol, atol;   /* outer stopping rule on the float64 residual          */
+  int max_outer;      /* refinement steps cap                                 */
+  double inner_tol;   /* relative tolerance of each float32 PCG solve         */
+  int inner_maxiter;  /* iteration cap of each float32 PCG solve              */
+  int check_every;    /* host polls of the inner solves (as ddgx_cg_cfg)      */
+} ddgx_ir_cfg;
+
+typedef struct {
+  int outer;          /* refinement steps taken                               */
+  int inner_total;    /* float32 PCG iterations over all steps                */
+  int converged;      /* outer stopping rule met                              */
+  double residual2;   /* final <r,r> of the float64 residual                  */
+  double atol2;
+} ddgx_ir_info;
+
+int ddgx_ir_solve(ddgx_op op64, ddgx_op op32, double alpha, double shift, ddgx_pc pc32,
+                  const void* b, void* x, const ddgx_ir_cfg* cfg, ddgx_ir_info* info,
+                  void* stream);
+
 /* Deterministic dot product in float64 of two device vectors of the op's
  * dtype and length K*Np; result written to *out (host). */
 int ddgx_dot(ddgx_op op, const void* a, const void* b, double* out, void* stream);

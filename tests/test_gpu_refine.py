@@ -1,0 +1,75 @@
+"""Mixed-precision iterative refinement (float32 PCG inside a float64
+residual loop) against float64 solves.
+
+Bar: the refined solution matches the oracle's dense LU solution to 1e-9
+relative (L2) at tol 1e-12 -- the float32 inner solves must not limit the
+final accuracy -- and takes more than one refinement step (float32 alone
+cannot reach it)."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import unit_square
+from oracle import sipg as S
+from ddg_amd import mesh as pmesh
+from ddg_amd import multigrid as MG
+from ddg_amd import sipg as D
+from ddg_amd.operator import refine_solve
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _l2(a, b):
+    a, b = np.asarray(a).ravel(), np.asarray(b).ravel()
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("pc_kind,alpha", [("mg", 0.0), ("bj", 0.0), ("mg", 30.0), (None, 5.0)])
+def test_refine_reaches_fp64_accuracy(dev, pc_kind, alpha):
+    pm, om = unit_square(8, 8, 4, jitter=0.1, seed=21)
+    bc = pmesh.dirichlet_x_neumann_y(pm)
+    n = pm.K * pm.Np
+    b = np.random.default_rng(22).standard_normal(n)
+    x_lu = np.linalg.solve(S.sipg_matrix_2d(om, bc_types=bc, alpha=alpha), b)
+    _say(f"refine pc={pc_kind} alpha={alpha}")
+    op64 = D.get_operator(pm, bc)
+    op32 = D.get_operator(pm, bc, dtype=torch.float32)
+    if pc_kind == "mg":
+        pc = MG.make_sipg_mg_preconditioner(pm, bc_types=bc, alpha=alpha, dtype=torch.float32)
+    elif pc_kind == "bj":
+        pc = op64.block_jacobi(alpha, dtype=torch.float32)
+    else:
+        pc = None
+    x, info = refine_solve(op64, op32, torch.as_tensor(b, device=dev), alpha=alpha, pc32=pc,
+                           tol=1e-12, inner_tol=1e-4, inner_maxiter=5000)
+    assert info["converged"], info
+    assert info["outer"] >= 2
+    assert info["residual2"] <= info["atol2"]
+    assert _l2(x.cpu().numpy(), x_lu) < 1e-9
+
+
+def test_refine_drop_in_solve(dev):
+    """solve_sipg_poisson_2d(..., refine=RefineConfig()) with an f32 MG
+    preconditioner equals the float64 MG-PCG solve."""
+    pm, om = unit_square(8, 8, 3, jitter=0.1, seed=23)
+    bc = pmesh.dirichlet_x_neumann_y(pm)
+    ue = np.exp(pm.x) * np.cos(3 * pm.y)
+    f = 8.0 * ue
+    _say("refine drop-in")
+    cfg = D.IterativeConfig(tol=1e-12, maxiter=2000)
+    mg64 = MG.make_sipg_mg_preconditioner(pm, bc_types=bc)
+    mg32 = MG.make_sipg_mg_preconditioner(pm, bc_types=bc, dtype=torch.float32)
+    u64 = D.solve_sipg_poisson_2d(pm, f, bc_types=bc, u_dir=ue, iterative=cfg,
+                                  preconditioner=mg64)
+    u32, info = D.solve_sipg_poisson_2d(pm, f, bc_types=bc, u_dir=ue, iterative=cfg,
+                                        preconditioner=mg32, refine=D.RefineConfig(),
+                                        return_info=True)
+    assert info["converged"] and info["outer"] >= 2
+    assert _l2(u32, u64) < 1e-9
+    with pytest.raises(TypeError):
+        D.solve_sipg_poisson_2d(pm, f, bc_types=bc, u_dir=ue, iterative=cfg,
+                                preconditioner=mg64, refine=D.RefineConfig())

@@ -7,12 +7,20 @@ vertices jittered by a seeded U(-0.1h, 0.1h), polynomial order N=4
 y = 0, 1, penalty tau_scale = 200, float64.  A "step" is one SIPG apply over
 the whole mesh with its input already resident in HBM.
 
-Also reported: block-Jacobi preconditioned CG solves of the same problem
-(f = 8 e^x cos 3y, u = e^x cos 3y on the Dirichlet walls, dn u on the
-Neumann walls) to tol 1e-10, and the CPU baseline: the float64 numpy
-restatement of the reference algorithm (oracle/) timed on the host cores.
+Also reported, on the same mesh:
+* float64 CG solves with the hp-multigrid preconditioner (p-levels 4, 2, 1
+  on the mesh, then h-levels at degree 1 down to 2x16x16) of
+  -Delta u = 8 e^x cos 3y, u = e^x cos 3y on the Dirichlet walls, dn u on
+  the Neumann walls, to tol 1e-10: solves/s, p50/p99 latency;
+* the same solves by mixed-precision iterative refinement (float32
+  operator, float32 V-cycle, float64 residual and update);
+* both solvers on seeded random Fourier right-hand sides (64 modes,
+  amplitude |k|^-2) on this mesh and on 2x128x128 at N=6, with the error
+  against a float64 solve to tol 1e-13;
+* the CPU baseline: the float64 numpy restatement of the reference
+  algorithm (oracle/) timed on the host cores.
 
-    python bench.py [--gpus 1] [--steps 200] [--warmup 20] [--solves 3]
+    python bench.py [--gpus 1] [--steps 200] [--warmup 20] [--solves 20]
 """
 
 from __future__ import annotations
@@ -58,6 +66,40 @@ def problem_data(m):
     gx, gy = np.exp(xf) * np.cos(3 * yf), -3 * np.exp(xf) * np.sin(3 * yf)
     q = (m.nx.reshape(-1, order="F") * gx + m.ny.reshape(-1, order="F") * gy)
     return 8.0 * ue, ue, q.reshape((3 * m.Nfp, m.K), order="F")
+
+
+def fourier_rhs(m, seed=0, modes=64):
+    """Random Fourier series at the nodes: sum_m a_m |k_m|^-2 cos(2 pi k_m.x + phi_m),
+    a ~ N(0,1), phi ~ U(0, 2 pi), integer wave vectors k_m != 0 with |k_i| <= 8."""
+    rng = np.random.default_rng(seed)
+    f = np.zeros_like(m.x)
+    for _ in range(modes):
+        k = np.zeros(2)
+        while not k.any():
+            k = rng.integers(-8, 9, size=2).astype(np.float64)
+        a, phi = rng.standard_normal(), rng.uniform(0, 2 * np.pi)
+        f += a / (k @ k) * np.cos(2 * np.pi * (k[0] * m.x + k[1] * m.y) + phi)
+    return f
+
+
+def hpmg_for(m, N, dtype, n_base=16, degrees=None):
+    """hp-multigrid of the forest mesh m (2 n^2 elements): p-levels on m,
+    then h-levels at degree 1 down to the 2 n_base^2 base mesh."""
+    import torch
+    from ddg_amd import mesh as pm
+    from ddg_amd import multigrid as MG
+    n = int(round(np.sqrt(m.K / 2)))
+    lb = int(round(np.log2(n_base))) - 1
+    lv = int(round(np.log2(n))) - 1
+    VX, VY, EToV = pm.forest_rectangle_hierarchy(2, lb)[-1]
+    base = pm.build_mesh_2d(VX, VY, EToV, N)
+    if degrees is None:
+        degrees = [N, N // 2, 1] if N >= 4 else list(range(N, 0, -1))
+    mg, fin = MG.make_sipg_hpmg_preconditioner(base, lv - lb, bc_fn=pm.dirichlet_x_neumann_y,
+                                               degrees=degrees, fine_vertices=(m.VX, m.VY),
+                                               dtype=dtype)
+    assert np.array_equal(np.asarray(fin.EToV), np.asarray(m.EToV))
+    return mg, fin
 
 
 def apply_bytes(K, Np, s):
@@ -115,13 +157,85 @@ def cpu_baseline(m, bc, budget_s=12.0):
     return t / n, n
 
 
+def _flat(a, dev):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(a.T).reshape(-1), device=dev)
+
+
+def timed_solves(fn, n):
+    import torch
+    lat, out = [], None
+    for _ in range(n):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t0)
+    return np.array(lat), out
+
+
+def lat_stats(lat):
+    return {"solves": int(lat.size), "solves_per_s": round(lat.size / float(lat.sum()), 3),
+            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3)}
+
+
+def solver_block(m, bc, b, mg64, mg32, n_solves, tol, inner_tol, ref_x=None, ue=None, log=""):
+    """FP64 MG-PCG and mixed-precision refinement on one right-hand side."""
+    import torch
+    from ddg_amd import sipg as D
+    from ddg_amd.operator import refine_solve
+    op64 = D.get_operator(m, bc)
+    op32 = D.get_operator(m, bc, dtype=torch.float32)
+    if ref_x is None:
+        ref_x, _ = op64.pcg(b, pc=mg64, tol=1e-13, maxiter=5000)
+    nref = float(torch.linalg.norm(ref_x))
+    res = {}
+    for name, fn in (
+            ("fp64", lambda: op64.pcg(b, pc=mg64, tol=tol, maxiter=5000)),
+            ("mixed", lambda: refine_solve(op64, op32, b, pc32=mg32, tol=tol,
+                                           inner_tol=inner_tol, inner_maxiter=2000))):
+        fn()  # warm-up
+        lat, (x, info) = timed_solves(fn, n_solves)
+        d = lat_stats(lat)
+        d["iterations"] = info["iterations"]
+        if "outer" in info:
+            d["outer"] = info["outer"]
+        d["converged"] = info["converged"]
+        d["rel_err_vs_fp64_tight"] = float(torch.linalg.norm(x - ref_x)) / nref
+        if ue is not None:
+            xs = x.cpu().numpy().reshape(m.K, m.Np).T
+            d["rel_L2_err_vs_exact"] = float(np.sqrt(np.sum(m.J * (xs - ue) ** 2) /
+                                                     np.sum(m.J * ue ** 2)))
+        res[name] = d
+        print(f"[bench] {log} {name}: {d}", file=sys.stderr, flush=True)
+    res["speedup_mixed_vs_fp64"] = round(res["fp64"]["p50_ms"] / res["mixed"]["p50_ms"], 3)
+    return res
+
+
+def read_traffic(name):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE, scaled by the same counters on a copy of known size), kept in
+    profiles/<name>; None when absent."""
+    p = ROOT / "profiles" / name
+    if not p.exists():
+        return None
+    for line in p.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--solves", type=int, default=3)
+    ap.add_argument("--solves", type=int, default=20)
+    ap.add_argument("--tol", type=float, default=1e-12)
+    ap.add_argument("--inner-tol", type=float, default=1e-4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-config2", action="store_true")
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--order", default="forest", choices=["forest", "rowmajor"])
     ap.add_argument("--sweep", action="store_true",
@@ -129,6 +243,11 @@ def main():
     ap.add_argument("--sweep-n", type=int, nargs="*", default=[1, 2, 3, 4, 5, 6])
     ap.add_argument("--copy", action="store_true",
                     help="time a device-to-device copy at several sizes and exit")
+    ap.add_argument("--profile-apply", choices=["f64", "f32"], default=None,
+                    help="only run --steps eager applies (for rocprofv3 counter runs)")
+    ap.add_argument("--profile-copy", type=int, default=0, metavar="MB",
+                    help="only run --steps elementwise copies y = 2 x of MB megabytes each "
+                         "(calibrates the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the SIPG path is "
@@ -136,10 +255,10 @@ def main():
         sys.exit(2)
 
     import torch
-    from ddg_amd import sipg as D
     from ddg_amd.operator import SipgOperator
 
     torch.cuda.set_device(0)
+    dev = torch.device("cuda", 0)
     if args.copy:
         for mb in (8, 16, 24, 48, 96, 256, 1024):
             n = mb * (1 << 20) // 8
@@ -164,11 +283,30 @@ def main():
                 print(f"[sweep] N={N} {str(dt)[6:]} K={m.K} {t * 1e6:8.2f} us "
                       f"{m.K * m.Np / t / 1e9:7.1f} GDOF/s  {frac * 100:5.1f}% HBM", flush=True)
         return
+    if args.profile_copy:
+        n = args.profile_copy * (1 << 20) // 8
+        a = torch.randn(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(args.steps):
+            torch.mul(a, 2.0, out=b)
+        torch.cuda.synchronize()
+        print(f"[profile] {args.steps} copies of {args.profile_copy} MB", flush=True)
+        return
     m, bc = workload_mesh(args.n, order=args.order)
     K, Np = m.K, m.Np
+    if args.profile_apply:
+        dt = torch.float64 if args.profile_apply == "f64" else torch.float32
+        op = SipgOperator(m, bc, dtype=dt)
+        u = torch.randn(K * Np, dtype=torch.float64, device="cuda").to(dt)
+        y = torch.empty_like(u)
+        for _ in range(args.steps):
+            op.apply(u, y)
+        torch.cuda.synchronize()
+        print(f"[profile] {args.steps} applies {args.profile_apply} K={K} N={m.N}", flush=True)
+        return
+
     op = SipgOperator(m, bc)
     print(f"[bench] mesh K={K} N={m.N} Np={Np} dofs={K * Np}", file=sys.stderr, flush=True)
-
     g = torch.Generator(device="cuda").manual_seed(0)
     u = torch.randn(K * Np, dtype=torch.float64, device="cuda", generator=g)
     y = torch.empty_like(u)
@@ -178,42 +316,52 @@ def main():
     achieved = B / t_apply / 1e9
     print(f"[bench] apply f64: {t_apply * 1e6:.2f} us, {gdof:.1f} GDOF/s, {achieved:.0f} GB/s",
           file=sys.stderr, flush=True)
-
-    # float32 apply of the same operator (reported alongside)
     op32 = SipgOperator(m, bc, dtype=torch.float32)
     u32 = u.float()
-    y32 = torch.empty_like(u32)
-    t32, _ = time_apply(op32, u32, y32, args.steps, args.warmup)
+    t32, _ = time_apply(op32, u32, torch.empty_like(u32), args.steps, args.warmup)
+    print(f"[bench] apply f32: {t32 * 1e6:.2f} us", file=sys.stderr, flush=True)
+    del op, op32
 
-    # block-Jacobi CG solves to 1e-10
-    f, ud, q = problem_data(m)
-    pc = D.make_sipg_block_jacobi_preconditioner(m, bc_types=bc)
-    cfg = D.IterativeConfig(tol=1e-10, maxiter=20000)
-    lat, iters = [], []
-    ue = ud
-    us = np.zeros_like(ue)
-    for i in range(args.solves):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        us, info = D.solve_sipg_poisson_2d(m, f, bc_types=bc, u_dir=ud, q_neu=q, iterative=cfg,
-                                           preconditioner=pc, return_info=True)
-        torch.cuda.synchronize()
-        lat.append(time.perf_counter() - t0)
-        iters.append(info["iterations"])
-        print(f"[bench] BJ-PCG solve {i}: {lat[-1]:.3f} s, {info['iterations']} it, "
-              f"converged={info['converged']}", file=sys.stderr, flush=True)
-    err = float(np.sqrt(np.sum(m.J * (us - ue) ** 2) / np.sum(m.J * ue ** 2)))
+    # --- solves on the same mesh (hp-multigrid, tol 1e-10) ---
+    from ddg_amd import sipg as D
+    t0 = time.perf_counter()
+    mg64, fin = hpmg_for(m, 4, torch.float64)
+    mg32, _ = hpmg_for(fin, 4, torch.float32)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    print(f"[bench] hp-MG setup (f64 + f32): {t_setup:.2f} s, levels K={mg64.sizes} "
+          f"N={mg64.degrees}", file=sys.stderr, flush=True)
+    f, ud, q = problem_data(fin)
+    opb = D.get_operator(fin, bc)
+    b = opb.rhs(_flat(f, dev), u_dir=_flat(ud, dev), q_neu=_flat(q, dev))
+    cg1 = solver_block(fin, bc, b, mg64, mg32, args.solves, args.tol, args.inner_tol, ue=ud,
+                       log="config1 e^x cos3y")
+    cg1["setup_s"] = round(t_setup, 3)
+    fr = fourier_rhs(fin, seed=1)
+    b2 = opb.rhs(_flat(fr, dev))
+    cg2 = {"N4_2x%dx%d" % (args.n, args.n):
+           solver_block(fin, bc, b2, mg64, mg32, max(3, args.solves // 4), args.tol,
+                        args.inner_tol, log="config2 fourier N=4")}
+    if not args.no_config2:
+        m6, bc6 = workload_mesh(args.n // 2, N=6)
+        mg6_64, _ = hpmg_for(m6, 6, torch.float64)
+        mg6_32, _ = hpmg_for(m6, 6, torch.float32)
+        op6 = D.get_operator(m6, bc6)
+        b6 = op6.rhs(_flat(fourier_rhs(m6, seed=2), dev))
+        cg2["N6_2x%dx%d" % (args.n // 2, args.n // 2)] = solver_block(
+            m6, bc6, b6, mg6_64, mg6_32, max(3, args.solves // 4), args.tol, args.inner_tol,
+            log="config2 fourier N=6")
 
     cpu = None
     if not args.no_cpu:
         t_cpu, n_cpu = cpu_baseline(m, bc)
-        cpu = {"value": K * Np / t_cpu / 1e9, "unit": "GDOF/s", "cores": os.cpu_count(),
-               "kind": "port",
-               "sample": f"{n_cpu} float64 applies of the numpy restatement (oracle/sipg.py) on "
-                         f"the same 2x{args.n}x{args.n} N=4 mesh; numpy/BLAS threads = host cores"}
+        cpu = {"value": round(K * Np / t_cpu / 1e9, 5), "unit": "GDOF/s",
+               "cores": os.cpu_count(), "kind": "port",
+               "sample": f"{n_cpu} float64 applies of the numpy restatement (oracle/sipg.py) "
+                         f"on the same 2x{args.n}x{args.n} N=4 mesh, {t_cpu:.3f} s each; "
+                         "numpy/BLAS threads = host cores"}
         print(f"[bench] cpu oracle apply: {t_cpu:.3f} s", file=sys.stderr, flush=True)
 
-    lat_s = np.array(lat) if lat else np.array([np.nan])
     out = {
         "metric": "SIPG matvec GDOF/s (% HBM roofline); CG solves/s & p99 at 1e-10 rel err",
         "value": round(gdof, 3),
@@ -222,19 +370,18 @@ def main():
         "ms_per_step": round(t_apply * 1e3, 5),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"2x{args.n}x{args.n} jittered triangles ({args.order} order), N=4, "
-                               "Dirichlet x / Neumann y, tau_scale=200, float64 SIPG apply",
+        "config": {"workload": f"SIPG apply, 2x{args.n}x{args.n} jittered triangles "
+                               f"({args.order} order), N=4, Dirichlet x / Neumann y, "
+                               "tau_scale=200, float64",
                    "K": K, "Np": Np, "dofs": K * Np},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": read_traffic("sipg_apply_f64_n4_k131072_traffic.txt")},
         "cpu_baseline": cpu,
         "apply_f32": {"us": round(t32 * 1e6, 3), "GDOF/s": round(K * Np / t32 / 1e9, 2),
                       "frac": round(apply_bytes(K, Np, 4) / t32 / 1e9 / HBM_PEAK_GBS, 4)},
-        "cg": {"preconditioner": "block-Jacobi", "tol": 1e-10, "solves": len(lat),
-               "solves_per_s": round(len(lat) / float(lat_s.sum()), 4) if lat else None,
-               "p50_s": round(float(np.percentile(lat_s, 50)), 4),
-               "p99_s": round(float(np.percentile(lat_s, 99)), 4),
-               "iterations": iters, "rel_L2_err_vs_exact": err},
+        "cg": {"preconditioner": f"hp-multigrid levels K={mg64.sizes} N={mg64.degrees}",
+               "tol": args.tol, "inner_tol": args.inner_tol, "config1": cg1, "config2": cg2},
     }
     print(json.dumps(out), flush=True)
 

@@ -431,13 +431,18 @@ int ir_solve(Op& op64, Op& op32, double alpha, double shift, PC* pc32, const voi
   const double t = cfg.tol * cfg.tol * bb, a = cfg.atol * cfg.atol;
   const double atol2 = t > a ? t : a;
   ddgx_cg_cfg icfg{cfg.inner_tol, 0.0, cfg.inner_maxiter, cfg.check_every};
-  int outer = 0, inner = 0;
-  double rr = 0.0;
+  int outer = 0, inner = 0, stalled = 0;
+  double rr = 0.0, rr_prev = 0.0;
   double* r = (double*)op64.r;
   while (true) {
     CHECK(launch_apply(op64, x, r, nullptr, alpha, shift, M_RESID, nullptr, nullptr, s, b));
     CHECK(dot(op64, r, r, &rr, s));
     if (!(rr > atol2) || outer >= cfg.max_outer) break;
+    // the float64 residual has reached its round-off floor when two steps in
+    // a row fail to reduce it by half: further steps cannot meet the test
+    stalled = (outer > 0 && rr > 0.25 * rr_prev) ? stalled + 1 : 0;
+    if (stalled >= 2) break;
+    rr_prev = rr;
     const double nr = std::sqrt(rr);
     hipLaunchKernelGGL(scale_down_kernel, dim3(op64.grid_vec), dim3(256), 0, s, r,
                        (float*)op32.ir_b, 1.0 / nr, n);

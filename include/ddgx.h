@@ -153,6 +153,8 @@ int ddgx_pcg_solve(ddgx_op op, double alpha, double shift, ddgx_pc pc,
  * ddgx_pcg_solve):
  *   r = b - A x (float64); stop when <r,r> <= max(tol^2 <b,b>, atol^2)
  *   d = PCG_float32(A, r/|r|, pc32) to inner_tol;  x += |r| d (float64)
+ * The loop also ends, unconverged, when two steps in a row fail to halve
+ * ||r|| (the float64 residual is at its round-off floor).
  * op64/op32: the same operator in float64 and float32; pc32: float32
  * preconditioner (block-Jacobi or multigrid) or NULL.  b, x: float64. */
 typedef struct {

@@ -20,7 +20,7 @@ EXPORTS = (
     "ddgx_last_error", "ddgx_table_size", "ddgx_op_create", "ddgx_op_destroy",
     "ddgx_op_info", "ddgx_sipg_apply", "ddgx_sipg_rhs", "ddgx_block_jacobi_create",
     "ddgx_block_jacobi_blocks", "ddgx_pc_apply", "ddgx_pc_destroy", "ddgx_pcg_solve",
-    "ddgx_dot", "ddgx_mg_create", "ddgx_ir_solve",
+    "ddgx_dot", "ddgx_mg_create", "ddgx_ir_solve", "ddgx_sipg_param_grad",
 )
 
 
@@ -78,6 +78,7 @@ def lib():
         "ddgx_pcg_solve": (i32, [vp, dbl, dbl, vp, vp, vp, C.POINTER(CGConfig),
                                  C.POINTER(CGInfo), vp]),
         "ddgx_dot": (i32, [vp, vp, vp, C.POINTER(dbl), vp]),
+        "ddgx_sipg_param_grad": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp]),
         "ddgx_ir_solve": (i32, [vp, vp, dbl, dbl, vp, vp, vp, C.POINTER(IRConfig),
                                 C.POINTER(IRInfo), vp]),
         "ddgx_mg_create": (i32, [i32, C.POINTER(vp), C.POINTER(vp), C.POINTER(C.c_int32),

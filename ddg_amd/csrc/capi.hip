@@ -442,6 +442,18 @@ int ddgx_ir_solve(ddgx_op op64, ddgx_op op32, double alpha, double shift, ddgx_p
                   info, (hipStream_t)stream);
 }
 
+int ddgx_sipg_param_grad(ddgx_op op, const double* u, const double* lam, const double* u_dir,
+                         const double* kappa, const double* tau_scale, double* g_kappa,
+                         double* g_tau, void* stream) {
+  if (op == nullptr || u == nullptr || lam == nullptr || tau_scale == nullptr ||
+      g_kappa == nullptr || g_tau == nullptr) {
+    set_error("ddgx_sipg_param_grad: null argument");
+    return DDGX_ERR_INVALID;
+  }
+  return launch_param_grad(op->op, u, lam, u_dir, kappa, tau_scale, g_kappa, g_tau,
+                           (hipStream_t)stream);
+}
+
 int ddgx_dot(ddgx_op op, const void* a, const void* b, double* out, void* stream) {
   if (op == nullptr || a == nullptr || b == nullptr || out == nullptr) {
     set_error("ddgx_dot: null argument");

@@ -195,6 +195,9 @@ int launch_bj_apply(int N, int64_t K, int dtype, const void* binvT, const void* 
 int pcg_solve(Op& op, double alpha, double shift, PC* pc, const void* b, void* x,
               const ddgx_cg_cfg& cfg, ddgx_cg_info* info, hipStream_t s);
 int dot(Op& op, const void* a, const void* b, double* out, hipStream_t s);
+int launch_param_grad(const Op& op, const double* u, const double* lam, const double* udir,
+                      const double* kappa, const double* tau, double* gk, double* gt,
+                      hipStream_t s);
 int ir_solve(Op& op64, Op& op32, double alpha, double shift, PC* pc32, const void* b, void* x,
              const ddgx_ir_cfg& cfg, ddgx_ir_info* info, hipStream_t s);
 

@@ -40,12 +40,20 @@ class Multigrid(DevicePreconditioner):
     ``mats[l]``."""
 
     def __init__(self, levels, kinds, mats, *, tau_scale=200.0, alpha=0.0, nu_smooth=2,
-                 omega=0.7, regularize=False, eps=1.0e-8, dtype=torch.float64):
+                 omega=0.7, regularize=False, eps=1.0e-8, dtype=torch.float64, ops=None):
+        """``ops``: optional prebuilt float64 level operators (then ``dtype``
+        must be float64 and ``levels`` only names the meshes); otherwise the
+        operators come from the per-mesh cache of ``ddg_amd.sipg``."""
         from .sipg import get_operator
         if dtype not in (torch.float64, torch.float32):
             raise ValueError("dtype must be torch.float64 or torch.float32")
         self._levels = levels  # keeps the level meshes (and their cached operators) alive
-        self.ops = [get_operator(m, bc, tau_scale, kap, dtype) for m, bc, kap in levels]
+        if ops is not None:
+            if dtype != torch.float64 or len(ops) != len(levels):
+                raise ValueError("prebuilt level operators must be float64, one per level")
+            self.ops = list(ops)
+        else:
+            self.ops = [get_operator(m, bc, tau_scale, kap, dtype) for m, bc, kap in levels]
         self.ops64 = (self.ops if dtype == torch.float64 else
                       [get_operator(m, bc, tau_scale, kap, torch.float64) for m, bc, kap in levels])
         self.dtype = dtype

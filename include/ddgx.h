@@ -148,6 +148,21 @@ int ddgx_pcg_solve(ddgx_op op, double alpha, double shift, ddgx_pc pc,
                    const void* b, void* x, const ddgx_cg_cfg* cfg,
                    ddgx_cg_info* info, void* stream);
 
+/* Parameter gradient of the SIPG form a(u, lam; u_dir) with respect to the
+ * per-element diffusion coefficient kappa_k and penalty scale tau_scale_k
+ * (sipg.py:263 with a (K,) tau_scale; kappa is this library's extension,
+ * kappa = 1 being the reference operator):
+ *   g_kappa[k] = d a / d kappa_k,   g_tau[k] = d a / d tau_scale_k.
+ * With lam the adjoint solution (A lam = dL/du) the loss gradient is
+ * -g (the adjoint of the reference's jax.grad through
+ * lax.custom_linear_solve, sipg.py:604-609, 910-915).  kappa and tau_scale
+ * must be the values the float64 operator was built with (kappa NULL = 1).
+ * All pointers: device float64, caller element order (K or K*Np). */
+int ddgx_sipg_param_grad(ddgx_op op, const double* u, const double* lam,
+                         const double* u_dir, const double* kappa,
+                         const double* tau_scale, double* g_kappa, double* g_tau,
+                         void* stream);
+
 /* Mixed-precision iterative refinement (no counterpart in the reference,
  * whose solves are float64 only; it solves the same systems as
  * ddgx_pcg_solve):

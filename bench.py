@@ -213,6 +213,52 @@ def solver_block(m, bc, b, mg64, mg32, n_solves, tol, inner_tol, ref_x=None, ue=
     return res
 
 
+def serve_block(n_requests, slots, calib=200, load=0.8, sample=8, seed=0):
+    """The request stream: Helmholtz solves on 2 n^2 meshes, n in {16, 32, 64,
+    128}, N in {2..6}, k^2 in [0, 50], Fourier right-hand sides, mixed-precision
+    refinement to 1e-10.  A closed-loop burst of ``calib`` requests measures
+    the capacity C; then ``n_requests`` arrive as a Poisson stream at
+    ``load * C`` per second.  A sample of results is checked against a
+    float64 block-Jacobi PCG solve to tol 1e-13."""
+    import torch
+    from ddg_amd import serve as SV
+    srv = SV.SolveServer(SV.ServeConfig(slots=slots))
+    reqs = SV.make_requests(n_requests, seed=seed)
+    t0 = time.perf_counter()
+    srv.prepare(reqs)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    print(f"[bench] serve: {len(srv.classes)} resident classes built in {setup:.1f} s",
+          file=sys.stderr, flush=True)
+    burst = SV.poisson_arrivals(SV.make_requests(calib, seed=seed + 1), None)
+    cap = srv.run(burst).stats()
+    print(f"[bench] serve burst: {cap}", file=sys.stderr, flush=True)
+    rate = load * cap["solves_per_s"]
+    SV.poisson_arrivals(reqs, rate, seed=seed + 2)
+    rep = srv.run(reqs)
+    st = rep.stats()
+    print(f"[bench] serve stream at {rate:.1f}/s: {st}", file=sys.stderr, flush=True)
+    errs = []
+    for r in reqs[:sample]:
+        pcl = srv.problem_class(r.n, r.N)
+        x, info = srv.solve_one(r, srv.streams[0])
+        f = SV.fourier_field(pcl.x, pcl.y, r.seed)
+        b = pcl.op64.rhs(f)
+        xr, _ = pcl.op64.pcg(b, alpha=r.k2, pc=pcl.op64.block_jacobi(r.k2), tol=1e-13,
+                             maxiter=50000)
+        errs.append(float(torch.linalg.norm(x - xr) / torch.linalg.norm(xr)))
+    srv.close()
+    out = {"requests": st["requests"], "offered_rate_per_s": round(rate, 2),
+           "solves_per_s": st["solves_per_s"], "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
+           "service_p50_ms": st["service_p50_ms"], "service_p99_ms": st["service_p99_ms"],
+           "all_converged": st["all_converged"], "slots": slots,
+           "peak_slots_busy": st["peak_slots_busy"],
+           "closed_loop_capacity_per_s": cap["solves_per_s"], "setup_s": round(setup, 2),
+           "tol": 1e-10, "sample_max_rel_err_vs_fp64": max(errs) if errs else None,
+           "sample": f"first {len(errs)} requests re-solved by float64 block-Jacobi PCG to 1e-13"}
+    return out
+
+
 def read_traffic(name):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
     WRITE_SIZE, scaled by the same counters on a copy of known size), kept in
@@ -236,6 +282,10 @@ def main():
     ap.add_argument("--inner-tol", type=float, default=1e-4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-config2", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=2000,
+                    help="requests in the serving stream (0: skip the serving leg)")
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-only", action="store_true", help="run only the serving leg")
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--order", default="forest", choices=["forest", "rowmajor"])
     ap.add_argument("--sweep", action="store_true",
@@ -291,6 +341,10 @@ def main():
             torch.mul(a, 2.0, out=b)
         torch.cuda.synchronize()
         print(f"[profile] {args.steps} copies of {args.profile_copy} MB", flush=True)
+        return
+    if args.serve_only:
+        print(json.dumps({"serve": serve_block(args.serve_requests, args.serve_slots)}),
+              flush=True)
         return
     m, bc = workload_mesh(args.n, order=args.order)
     K, Np = m.K, m.Np
@@ -352,6 +406,10 @@ def main():
             m6, bc6, b6, mg6_64, mg6_32, max(3, args.solves // 4), args.tol, args.inner_tol,
             log="config2 fourier N=6")
 
+    serve = None
+    if args.serve_requests > 0:
+        serve = serve_block(args.serve_requests, args.serve_slots)
+
     cpu = None
     if not args.no_cpu:
         t_cpu, n_cpu = cpu_baseline(m, bc)
@@ -382,6 +440,7 @@ def main():
                       "frac": round(apply_bytes(K, Np, 4) / t32 / 1e9 / HBM_PEAK_GBS, 4)},
         "cg": {"preconditioner": f"hp-multigrid levels K={mg64.sizes} N={mg64.degrees}",
                "tol": args.tol, "inner_tol": args.inner_tol, "config1": cg1, "config2": cg2},
+        "serve": serve,
     }
     print(json.dumps(out), flush=True)
 

@@ -21,6 +21,8 @@ EXPORTS = (
     "ddgx_op_info", "ddgx_sipg_apply", "ddgx_sipg_rhs", "ddgx_block_jacobi_create",
     "ddgx_block_jacobi_blocks", "ddgx_pc_apply", "ddgx_pc_destroy", "ddgx_pcg_solve",
     "ddgx_dot", "ddgx_mg_create", "ddgx_ir_solve", "ddgx_sipg_param_grad",
+    "ddgx_solver_create", "ddgx_solver_bytes", "ddgx_solver_pcg", "ddgx_solver_ir",
+    "ddgx_solver_destroy",
 )
 
 
@@ -84,6 +86,13 @@ def lib():
         "ddgx_mg_create": (i32, [i32, C.POINTER(vp), C.POINTER(vp), C.POINTER(C.c_int32),
                                  C.POINTER(C.POINTER(dbl)), C.POINTER(MGConfig), vp,
                                  C.POINTER(vp)]),
+        "ddgx_solver_create": (i32, [vp, vp, vp, C.POINTER(vp)]),
+        "ddgx_solver_bytes": (i32, [vp, C.POINTER(i64)]),
+        "ddgx_solver_pcg": (i32, [vp, dbl, dbl, vp, vp, C.POINTER(CGConfig),
+                                  C.POINTER(CGInfo), vp]),
+        "ddgx_solver_ir": (i32, [vp, dbl, dbl, vp, vp, C.POINTER(IRConfig),
+                                 C.POINTER(IRInfo), vp]),
+        "ddgx_solver_destroy": (i32, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

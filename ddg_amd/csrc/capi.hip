@@ -16,6 +16,14 @@ struct ddgx_op_s {
 struct ddgx_pc_s {
   ddgx::PC pc;
 };
+struct ddgx_solver_s {
+  ddgx::Op* op = nullptr;     // operator of the solve (float64 for refinement)
+  ddgx::Op* op32 = nullptr;   // float32 twin (refinement solver) or null
+  ddgx::PC* pc = nullptr;     // preconditioner or null
+  ddgx::Work* w = nullptr;    // Krylov state of op
+  ddgx::Work* w32 = nullptr;  // Krylov state of op32
+  ddgx::MGWork* mgw = nullptr;  // V-cycle buffers when pc is a multigrid
+};
 
 namespace ddgx {
 
@@ -66,11 +74,10 @@ static int upload_as(int dtype, const double* src, size_t n, void** dst) {
 }
 
 static void free_op(Op& o) {
-  void* ptrs[] = {o.tab, o.rec, o.nbr, o.fcode, o.perm, o.hid, o.hab, o.fsj,
-                  o.r, o.p, o.z, o.Ap, o.partials, o.ir_b, o.ir_x, o.scal};
+  void* ptrs[] = {o.tab, o.rec, o.nbr, o.fcode, o.perm, o.hpos, o.fsj};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
-  if (o.scal_host) (void)hipHostFree(o.scal_host);
+  work_free(o.ws);
   o = Op{};
 }
 
@@ -127,12 +134,14 @@ int ddgx_op_create(int N, int64_t K, int dtype, const double* tables, const doub
     perm[i] = (int)k;
     inv[k] = (int)i;
   }
-  // per-tile halo lists (ids of out-of-tile neighbour elements) and the packed
-  // apply code of every face (neighbour trace slot: 0..63 in the tile, 64+h
-  // in the halo)
+  // per-tile halo lists (tile positions of out-of-tile neighbour elements)
+  // and the packed apply code of every face (neighbour trace slot: 0..63 in
+  // the tile, 64+h in the halo)
   const int64_t ntiles = (K + TILE - 1) / TILE;
-  std::vector<int> hid(ntiles * HALO_MAX, -1), hpos(ntiles * HALO_MAX, -1), fcode((size_t)K * 3);
+  std::vector<int> hall(ntiles * HALO_MAX, -1), fcode((size_t)K * 3);
   int max_halo = 0;
+  bool perm_id = true;
+  for (int64_t i = 0; i < K; ++i) perm_id = perm_id && perm[i] == i;
   for (int64_t t = 0; t < ntiles; ++t) {
     const int64_t i0 = t * TILE, i1 = std::min<int64_t>(K, i0 + TILE);
     int nh = 0;
@@ -147,15 +156,14 @@ int ddgx_op_create(int N, int64_t K, int dtype, const double* tables, const doub
             sl = (int)(in - i0);
           } else {
             int h = 0;
-            while (h < nh && hpos[t * HALO_MAX + h] != in) ++h;
+            while (h < nh && hall[t * HALO_MAX + h] != in) ++h;
             if (h == nh) {
               if (nh == HALO_MAX) {
                 set_error("ddgx_op_create: a 64-element tile has more than 64 out-of-tile "
                           "neighbours; pass an element order that keeps tiles compact");
                 return DDGX_ERR_INVALID;
               }
-              hpos[t * HALO_MAX + nh] = in;
-              hid[t * HALO_MAX + nh] = nbr_elem(c);
+              hall[t * HALO_MAX + nh] = in;
               ++nh;
             }
             sl = TILE + h;
@@ -167,17 +175,11 @@ int ddgx_op_create(int N, int64_t K, int dtype, const double* tables, const doub
     }
     max_halo = std::max(max_halo, nh);
   }
-  // halo face coefficients a_f, b_f gathered per tile: hab[t][c][h]
-  std::vector<double> hab((size_t)ntiles * 6 * HALO_MAX, 0.0);
+  // the smaller-LDS kernel variant when every halo fits its capacity
+  const int hcap = max_halo <= HALO_SMALL ? HALO_SMALL : HALO_MAX;
+  std::vector<int> hpos(ntiles * hcap);
   for (int64_t t = 0; t < ntiles; ++t)
-    for (int hh = 0; hh < HALO_MAX; ++hh) {
-      const int kid = hid[t * HALO_MAX + hh];
-      if (kid < 0) continue;
-      for (int f = 0; f < 3; ++f) {
-        hab[((size_t)t * 6 + f) * HALO_MAX + hh] = rec[(int64_t)kid * DDGX_REC_WORDS + R_A + f];
-        hab[((size_t)t * 6 + 3 + f) * HALO_MAX + hh] = rec[(int64_t)kid * DDGX_REC_WORDS + R_B + f];
-      }
-    }
+    for (int h = 0; h < hcap; ++h) hpos[t * hcap + h] = hall[t * HALO_MAX + h];
   auto* h = new ddgx_op_s();
   Op& o = h->op;
   o.N = N;
@@ -186,6 +188,8 @@ int ddgx_op_create(int N, int64_t K, int dtype, const double* tables, const doub
   o.Np = (N + 1) * (N + 2) / 2;
   o.Nfp = N + 1;
   o.max_halo = max_halo;
+  o.halo_cap = hcap;
+  o.perm_id = perm_id;
   int rc = DDGX_OK;
   auto put = [&](int** dst, const std::vector<int>& v, const char* what) -> bool {
     hipError_t e = hipMalloc((void**)dst, v.size() * sizeof(int));
@@ -219,14 +223,10 @@ int ddgx_op_create(int N, int64_t K, int dtype, const double* tables, const doub
     }
     if ((rc = upload_as(dtype, recT.data(), recT.size(), &o.rec)) != DDGX_OK) break;
     if ((rc = upload_as(dtype, fsjT.data(), fsjT.size(), &o.fsj)) != DDGX_OK) break;
-    if ((rc = upload_as(dtype, hab.data(), hab.size(), &o.hab)) != DDGX_OK) break;
     if (!put(&o.nbr, nbrT, "nbr") || !put(&o.fcode, fcode, "fcode") || !put(&o.perm, perm, "perm") ||
-        !put(&o.hid, hid, "hid"))
+        !put(&o.hpos, hpos, "hpos"))
       break;
-    {
-      const int cap = apply_grid(N, dtype);
-      o.grid_apply = (int)(ntiles < cap ? ntiles : cap);
-    }
+    o.grid_apply = (int)ntiles;
     const int64_t nvec = K * o.Np;
     int64_t gv = (nvec + 255) / 256;
     const int64_t cap = 4 * (int64_t)device_cu_count();
@@ -350,7 +350,8 @@ int ddgx_pc_apply(ddgx_pc pc, const void* r, void* z, void* stream) {
     return DDGX_ERR_INVALID;
   }
   const Op& o = *pc->pc.op;
-  if (pc->pc.type == 1) return mg_vcycle(*pc->pc.mg, r, z, nullptr, (hipStream_t)stream);
+  if (pc->pc.type == 1)
+    return mg_vcycle(*pc->pc.mg, *pc->pc.mg->ws, r, z, nullptr, (hipStream_t)stream);
   return launch_bj_apply(o.N, o.K, pc->pc.dtype, pc->pc.binvT, r, z, (hipStream_t)stream);
 }
 
@@ -413,7 +414,10 @@ int ddgx_pcg_solve(ddgx_op op, double alpha, double shift, ddgx_pc pc, const voi
     set_error("ddgx_pcg_solve: preconditioner built for a different mesh/order");
     return DDGX_ERR_INVALID;
   }
-  return pcg_solve(op->op, alpha, shift, pc ? &pc->pc : nullptr, b, x, *cfg, info,
+  Work* w = nullptr;
+  const int rc = default_work(op->op, false, &w);
+  if (rc != DDGX_OK) return rc;
+  return pcg_solve(op->op, *w, alpha, shift, pc ? &pc->pc : nullptr, nullptr, b, x, *cfg, info,
                    (hipStream_t)stream);
 }
 
@@ -438,8 +442,12 @@ int ddgx_ir_solve(ddgx_op op64, ddgx_op op32, double alpha, double shift, ddgx_p
     set_error("ddgx_ir_solve: bad refinement settings");
     return DDGX_ERR_INVALID;
   }
-  return ir_solve(op64->op, op32->op, alpha, shift, pc32 ? &pc32->pc : nullptr, b, x, *cfg,
-                  info, (hipStream_t)stream);
+  Work *w64 = nullptr, *w32 = nullptr;
+  int rc = default_work(op64->op, false, &w64);
+  if (rc == DDGX_OK) rc = default_work(op32->op, true, &w32);
+  if (rc != DDGX_OK) return rc;
+  return ir_solve(op64->op, *w64, op32->op, *w32, alpha, shift, pc32 ? &pc32->pc : nullptr,
+                  nullptr, b, x, *cfg, info, (hipStream_t)stream);
 }
 
 int ddgx_sipg_param_grad(ddgx_op op, const double* u, const double* lam, const double* u_dir,
@@ -459,7 +467,106 @@ int ddgx_dot(ddgx_op op, const void* a, const void* b, double* out, void* stream
     set_error("ddgx_dot: null argument");
     return DDGX_ERR_INVALID;
   }
-  return dot(op->op, a, b, out, (hipStream_t)stream);
+  Work* w = nullptr;
+  const int rc = default_work(op->op, false, &w);
+  if (rc != DDGX_OK) return rc;
+  return dot(op->op, *w, a, b, out, (hipStream_t)stream);
+}
+
+static void free_solver(ddgx_solver_s* h) {
+  work_free(h->w);
+  work_free(h->w32);
+  mgwork_free(h->mgw);
+  delete h;
+}
+
+int ddgx_solver_create(ddgx_op op, ddgx_op op32, ddgx_pc pc, ddgx_solver* out) {
+  if (op == nullptr || out == nullptr) {
+    set_error("ddgx_solver_create: null argument");
+    return DDGX_ERR_INVALID;
+  }
+  *out = nullptr;
+  const Op& a = op->op;
+  if (op32 != nullptr) {
+    const Op& c = op32->op;
+    if (a.dtype != DDGX_F64 || c.dtype != DDGX_F32 || a.K != c.K || a.N != c.N) {
+      set_error("ddgx_solver_create: refinement needs a float64 and a float32 operator of one "
+                "mesh and order");
+      return DDGX_ERR_INVALID;
+    }
+  }
+  if (pc != nullptr) {
+    const int want = op32 != nullptr ? DDGX_F32 : a.dtype;
+    if (pc->pc.dtype != want || pc->pc.op->K != a.K || pc->pc.op->N != a.N) {
+      set_error("ddgx_solver_create: preconditioner of another mesh, order or precision");
+      return DDGX_ERR_INVALID;
+    }
+  }
+  auto* h = new ddgx_solver_s();
+  h->op = &op->op;
+  h->op32 = op32 ? &op32->op : nullptr;
+  h->pc = pc ? &pc->pc : nullptr;
+  int rc = work_create(a, false, &h->w);
+  if (rc == DDGX_OK && op32 != nullptr) rc = work_create(op32->op, true, &h->w32);
+  if (rc == DDGX_OK && pc != nullptr && pc->pc.type == 1) rc = mgwork_create(*pc->pc.mg, &h->mgw);
+  if (rc != DDGX_OK) {
+    free_solver(h);
+    return rc;
+  }
+  *out = h;
+  return DDGX_OK;
+}
+
+int ddgx_solver_bytes(ddgx_solver sv, int64_t* bytes) {
+  if (sv == nullptr || bytes == nullptr) {
+    set_error("ddgx_solver_bytes: null argument");
+    return DDGX_ERR_INVALID;
+  }
+  *bytes = (int64_t)(sv->w->bytes + (sv->w32 ? sv->w32->bytes : 0) +
+                     (sv->mgw ? sv->mgw->bytes : 0));
+  return DDGX_OK;
+}
+
+int ddgx_solver_pcg(ddgx_solver sv, double alpha, double shift, const void* b, void* x,
+                    const ddgx_cg_cfg* cfg, ddgx_cg_info* info, void* stream) {
+  if (sv == nullptr || b == nullptr || x == nullptr || cfg == nullptr) {
+    set_error("ddgx_solver_pcg: null argument");
+    return DDGX_ERR_INVALID;
+  }
+  if (sv->op32 != nullptr) {
+    set_error("ddgx_solver_pcg: this solver was created for refinement (ddgx_solver_ir)");
+    return DDGX_ERR_INVALID;
+  }
+  if (cfg->maxiter < 0) {
+    set_error("ddgx_solver_pcg: maxiter must be >= 0");
+    return DDGX_ERR_INVALID;
+  }
+  return pcg_solve(*sv->op, *sv->w, alpha, shift, sv->pc, sv->mgw, b, x, *cfg, info,
+                   (hipStream_t)stream);
+}
+
+int ddgx_solver_ir(ddgx_solver sv, double alpha, double shift, const void* b, void* x,
+                   const ddgx_ir_cfg* cfg, ddgx_ir_info* info, void* stream) {
+  if (sv == nullptr || b == nullptr || x == nullptr || cfg == nullptr) {
+    set_error("ddgx_solver_ir: null argument");
+    return DDGX_ERR_INVALID;
+  }
+  if (sv->op32 == nullptr) {
+    set_error("ddgx_solver_ir: this solver has no float32 operator (use ddgx_solver_pcg)");
+    return DDGX_ERR_INVALID;
+  }
+  if (cfg->max_outer < 0 || cfg->inner_maxiter < 0 || !(cfg->inner_tol > 0.0)) {
+    set_error("ddgx_solver_ir: bad refinement settings");
+    return DDGX_ERR_INVALID;
+  }
+  return ir_solve(*sv->op, *sv->w, *sv->op32, *sv->w32, alpha, shift, sv->pc, sv->mgw, b, x,
+                  *cfg, info, (hipStream_t)stream);
+}
+
+int ddgx_solver_destroy(ddgx_solver sv) {
+  if (sv == nullptr) return DDGX_OK;
+  free_solver(sv);
+  return DDGX_OK;
 }
 
 }  // extern "C"

@@ -258,57 +258,96 @@ int launch_bj_apply(int N, int64_t K, int dtype, const void* binvT, const void* 
 
 static size_t dsize(int dtype) { return dtype == DDGX_F64 ? 8 : 4; }
 
-static int ensure_workspace(Op& op) {
-  if (op.r != nullptr) return DDGX_OK;
+void work_free(Work* w) {
+  if (w == nullptr) return;
+  void* ptrs[] = {w->r, w->p, w->z, w->Ap, w->partials, w->ir_b, w->ir_x, w->scal};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  if (w->scal_host) (void)hipHostFree(w->scal_host);
+  delete w;
+}
+
+int work_create(const Op& op, bool refine, Work** out) {
+  *out = nullptr;
+  Work* w = new Work();
+  w->K = op.K;
+  w->Np = op.Np;
+  w->dtype = op.dtype;
+  w->device = op.device;
   const size_t n = (size_t)op.K * op.Np * dsize(op.dtype);
-  DDGX_HIP(hipSetDevice(op.device));
-  if (hipMalloc(&op.r, n) != hipSuccess || hipMalloc(&op.p, n) != hipSuccess ||
-      hipMalloc(&op.z, n) != hipSuccess || hipMalloc(&op.Ap, n) != hipSuccess) {
-    set_error("device allocation of CG workspace failed");
-    return DDGX_ERR_NOMEM;
+  w->npart = op.grid_apply > op.grid_vec ? op.grid_apply : op.grid_vec;
+  const size_t np = 2 * (size_t)w->npart * sizeof(double);
+  hipError_t e = hipSetDevice(op.device);
+  if (e == hipSuccess) e = hipMalloc(&w->r, n);
+  if (e == hipSuccess) e = hipMalloc(&w->p, n);
+  if (e == hipSuccess) e = hipMalloc(&w->z, n);
+  if (e == hipSuccess) e = hipMalloc(&w->Ap, n);
+  if (e == hipSuccess) e = hipMalloc((void**)&w->partials, np);
+  if (e == hipSuccess) e = hipMalloc((void**)&w->scal, sizeof(CGScalars));
+  if (e == hipSuccess && refine) e = hipMalloc(&w->ir_b, (size_t)op.K * op.Np * sizeof(float));
+  if (e == hipSuccess && refine) e = hipMalloc(&w->ir_x, (size_t)op.K * op.Np * sizeof(float));
+  if (e == hipSuccess)
+    e = hipHostMalloc((void**)&w->scal_host, sizeof(CGScalars), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    work_free(w);
+    return hip_fail(e, "device allocation of CG workspace");
   }
-  const int g = op.grid_apply > op.grid_vec ? op.grid_apply : op.grid_vec;
-  DDGX_HIP(hipMalloc((void**)&op.partials, 2 * (size_t)g * sizeof(double)));
-  DDGX_HIP(hipMalloc((void**)&op.scal, sizeof(CGScalars)));
-  DDGX_HIP(hipHostMalloc((void**)&op.scal_host, sizeof(CGScalars), hipHostMallocDefault));
+  w->bytes = 4 * n + np + sizeof(CGScalars) +
+             (refine ? 2 * (size_t)op.K * op.Np * sizeof(float) : 0);
+  *out = w;
+  return DDGX_OK;
+}
+
+int default_work(Op& op, bool refine, Work** out) {
+  if (op.ws != nullptr && (!refine || op.ws->ir_b != nullptr)) {
+    *out = op.ws;
+    return DDGX_OK;
+  }
+  Work* w = nullptr;
+  const int rc = work_create(op, refine, &w);
+  if (rc != DDGX_OK) return rc;
+  work_free(op.ws);
+  op.ws = w;
+  *out = w;
   return DDGX_OK;
 }
 
 template <typename T>
-static int dot2(const Op& op, const void* a, const void* b, const void* c, const void* d,
-                const int* skip, hipStream_t s) {
+static int dot2(const Op& op, Work& w, const void* a, const void* b, const void* c,
+                const void* d, const int* skip, hipStream_t s) {
   const size_t n = (size_t)op.K * op.Np;
   hipLaunchKernelGGL((dot2_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (const T*)a,
-                     (const T*)b, (const T*)c, (const T*)d, n, op.partials, skip);
+                     (const T*)b, (const T*)c, (const T*)d, n, w.partials, skip);
   DDGX_HIP(hipGetLastError());
   return DDGX_OK;
 }
 
-static int finalize(const Op& op, int nparts, int mode, hipStream_t s) {
-  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, s, (const double*)op.partials,
-                     nparts, op.scal, mode);
+static int finalize(Work& w, int nparts, int mode, hipStream_t s) {
+  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, s, (const double*)w.partials,
+                     nparts, w.scal, mode);
   DDGX_HIP(hipGetLastError());
   return DDGX_OK;
 }
 
 template <typename T, int N>
-static int update_bj(const Op& op, void* x, const void* binvT, hipStream_t s) {
+static int update_bj(const Op& op, Work& w, void* x, const void* binvT, hipStream_t s) {
   hipLaunchKernelGGL((update_bj_kernel<T, N>), dim3(op.grid_vec), dim3(256), 0, s, (T*)x,
-                     (T*)op.r, (const T*)op.p, (const T*)op.Ap, (const T*)binvT, (T*)op.z,
-                     (const CGScalars*)op.scal, (int)op.K, op.partials);
+                     (T*)w.r, (const T*)w.p, (const T*)w.Ap, (const T*)binvT, (T*)w.z,
+                     (const CGScalars*)w.scal, (int)op.K, w.partials);
   DDGX_HIP(hipGetLastError());
   return DDGX_OK;
 }
 
 template <typename T>
-static int update_bj_dispatch(const Op& op, void* x, const void* binvT, hipStream_t s) {
+static int update_bj_dispatch(const Op& op, Work& w, void* x, const void* binvT,
+                              hipStream_t s) {
   switch (op.N) {
-    case 1: return update_bj<T, 1>(op, x, binvT, s);
-    case 2: return update_bj<T, 2>(op, x, binvT, s);
-    case 3: return update_bj<T, 3>(op, x, binvT, s);
-    case 4: return update_bj<T, 4>(op, x, binvT, s);
-    case 5: return update_bj<T, 5>(op, x, binvT, s);
-    case 6: return update_bj<T, 6>(op, x, binvT, s);
+    case 1: return update_bj<T, 1>(op, w, x, binvT, s);
+    case 2: return update_bj<T, 2>(op, w, x, binvT, s);
+    case 3: return update_bj<T, 3>(op, w, x, binvT, s);
+    case 4: return update_bj<T, 4>(op, w, x, binvT, s);
+    case 5: return update_bj<T, 5>(op, w, x, binvT, s);
+    case 6: return update_bj<T, 6>(op, w, x, binvT, s);
   }
   set_error("polynomial order not built");
   return DDGX_ERR_UNSUPPORTED;
@@ -320,67 +359,76 @@ static int update_bj_dispatch(const Op& op, void* x, const void* binvT, hipStrea
     if (_rc != DDGX_OK) return _rc; \
   } while (0)
 
+static int check_work(const Op& op, const Work& w) {
+  if (w.K != op.K || w.Np != op.Np || w.dtype != op.dtype) {
+    set_error("workspace does not match the operator's vector shape / dtype");
+    return DDGX_ERR_INVALID;
+  }
+  return DDGX_OK;
+}
+
 template <typename T>
-static int pcg_t(Op& op, double alpha, double shift, PC* pc, const void* b, void* x,
-                 const ddgx_cg_cfg& cfg, ddgx_cg_info* info, hipStream_t s) {
+static int pcg_t(const Op& op, Work& w, double alpha, double shift, const PC* pc, MGWork* mgw,
+                 const void* b, void* x, const ddgx_cg_cfg& cfg, ddgx_cg_info* info,
+                 hipStream_t s) {
   const size_t n = (size_t)op.K * op.Np;
   const bool has_pc = pc != nullptr;
   const bool bj = has_pc && pc->type == 0;  // block-Jacobi: fused into the update
-  CGScalars& init = *op.scal_host;  // pinned staging buffer
+  CGScalars& init = *w.scal_host;  // pinned staging buffer
   init = CGScalars{};
   init.tol = cfg.tol;
   init.atol = cfg.atol;
   init.maxiter = cfg.maxiter;
   init.use_pc = has_pc ? 1 : 0;
-  DDGX_HIP(hipMemcpyAsync(op.scal, &init, sizeof(CGScalars), hipMemcpyHostToDevice, s));
-  const int* done = &op.scal->done;
-  void* z = has_pc ? op.z : op.r;  // without a preconditioner z aliases r
+  DDGX_HIP(hipMemcpyAsync(w.scal, &init, sizeof(CGScalars), hipMemcpyHostToDevice, s));
+  const int* done = &w.scal->done;
+  void* z = has_pc ? w.z : w.r;  // without a preconditioner z aliases r
 
   // atol2 from <b,b>
-  CHECK(dot2<T>(op, b, b, nullptr, nullptr, nullptr, s));
-  CHECK(finalize(op, op.grid_vec, F_INIT_BB, s));
+  CHECK(dot2<T>(op, w, b, b, nullptr, nullptr, nullptr, s));
+  CHECK(finalize(w, op.grid_vec, F_INIT_BB, s));
   // r0 = b - A x0
-  CHECK(launch_apply(op, x, op.Ap, nullptr, alpha, shift, 0, nullptr, nullptr, s));
+  CHECK(launch_apply(op, x, w.Ap, nullptr, alpha, shift, 0, nullptr, nullptr, s));
   hipLaunchKernelGGL((residual_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (const T*)b,
-                     (const T*)op.Ap, (T*)op.r, n);
+                     (const T*)w.Ap, (T*)w.r, n);
   DDGX_HIP(hipGetLastError());
-  if (bj) CHECK(launch_bj_apply(op.N, op.K, op.dtype, pc->binvT, op.r, op.z, s));
-  else if (has_pc) CHECK(mg_vcycle(*pc->mg, op.r, op.z, nullptr, s));
+  if (bj) CHECK(launch_bj_apply(op.N, op.K, op.dtype, pc->binvT, w.r, w.z, s));
+  else if (has_pc) CHECK(mg_vcycle(*pc->mg, *mgw, w.r, w.z, nullptr, s));
   hipLaunchKernelGGL((copy_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (const T*)z,
-                     (T*)op.p, n);
+                     (T*)w.p, n);
   DDGX_HIP(hipGetLastError());
-  CHECK(dot2<T>(op, op.r, z, op.r, op.r, nullptr, s));
-  CHECK(finalize(op, op.grid_vec, F_INIT_GAMMA, s));
+  CHECK(dot2<T>(op, w, w.r, z, w.r, w.r, nullptr, s));
+  CHECK(finalize(w, op.grid_vec, F_INIT_GAMMA, s));
 
   const int every = cfg.check_every > 0 ? cfg.check_every : 16;
   int launched = 0;
   while (true) {
-    DDGX_HIP(hipMemcpyAsync(op.scal_host, op.scal, sizeof(CGScalars), hipMemcpyDeviceToHost, s));
+    DDGX_HIP(hipMemcpyAsync(w.scal_host, w.scal, sizeof(CGScalars), hipMemcpyDeviceToHost, s));
     DDGX_HIP(hipStreamSynchronize(s));
-    if (op.scal_host->done || launched >= cfg.maxiter) break;
+    if (w.scal_host->done || launched >= cfg.maxiter) break;
     for (int c = 0; c < every && launched < cfg.maxiter; ++c, ++launched) {
       // Ap = A p, partial <p, Ap>
-      CHECK(launch_apply(op, op.p, op.Ap, nullptr, alpha, shift, M_DOT, op.partials, done, s));
-      CHECK(finalize(op, op.grid_apply, F_ALPHA, s));
+      CHECK(launch_apply(op, w.p, w.Ap, nullptr, alpha, shift, M_DOT, w.partials, done, s));
+      CHECK(finalize(w, op.grid_apply, F_ALPHA, s));
       if (bj) {
-        CHECK(update_bj_dispatch<T>(op, x, pc->binvT, s));
+        CHECK(update_bj_dispatch<T>(op, w, x, pc->binvT, s));
       } else {
         hipLaunchKernelGGL((update_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (T*)x,
-                           (T*)op.r, (const T*)op.p, (const T*)op.Ap,
-                           (const CGScalars*)op.scal, n, has_pc ? nullptr : op.partials);
+                           (T*)w.r, (const T*)w.p, (const T*)w.Ap,
+                           (const CGScalars*)w.scal, n, has_pc ? nullptr : w.partials);
         DDGX_HIP(hipGetLastError());
         if (has_pc) {
-          CHECK(mg_vcycle(*pc->mg, op.r, op.z, done, s));
-          CHECK(dot2<T>(op, op.r, op.z, op.r, op.r, done, s));
+          CHECK(mg_vcycle(*pc->mg, *mgw, w.r, w.z, done, s));
+          CHECK(dot2<T>(op, w, w.r, w.z, w.r, w.r, done, s));
         }
       }
-      CHECK(finalize(op, op.grid_vec, F_BETA, s));
-      hipLaunchKernelGGL((pupdate_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (T*)op.p,
-                         (const T*)z, (const CGScalars*)op.scal, n);
+      CHECK(finalize(w, op.grid_vec, F_BETA, s));
+      hipLaunchKernelGGL((pupdate_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (T*)w.p,
+                         (const T*)z, (const CGScalars*)w.scal, n);
       DDGX_HIP(hipGetLastError());
     }
   }
-  const CGScalars& h = *op.scal_host;
+  const CGScalars& h = *w.scal_host;
   if (info != nullptr) {
     info->iterations = h.k;
     info->residual2 = h.rs;
@@ -390,15 +438,17 @@ static int pcg_t(Op& op, double alpha, double shift, PC* pc, const void* b, void
   return DDGX_OK;
 }
 
-int pcg_solve(Op& op, double alpha, double shift, PC* pc, const void* b, void* x,
-              const ddgx_cg_cfg& cfg, ddgx_cg_info* info, hipStream_t s) {
-  CHECK(ensure_workspace(op));
+int pcg_solve(const Op& op, Work& w, double alpha, double shift, const PC* pc, MGWork* mgw,
+              const void* b, void* x, const ddgx_cg_cfg& cfg, ddgx_cg_info* info,
+              hipStream_t s) {
+  CHECK(check_work(op, w));
   if (pc != nullptr && pc->dtype != op.dtype) {
     set_error("preconditioner dtype differs from operator dtype");
     return DDGX_ERR_INVALID;
   }
-  if (op.dtype == DDGX_F64) return pcg_t<double>(op, alpha, shift, pc, b, x, cfg, info, s);
-  return pcg_t<float>(op, alpha, shift, pc, b, x, cfg, info, s);
+  if (pc != nullptr && pc->type == 1 && mgw == nullptr) mgw = pc->mg->ws;
+  if (op.dtype == DDGX_F64) return pcg_t<double>(op, w, alpha, shift, pc, mgw, b, x, cfg, info, s);
+  return pcg_t<float>(op, w, alpha, shift, pc, mgw, b, x, cfg, info, s);
 }
 
 // rs = (float) (scale * r)
@@ -417,26 +467,27 @@ refine_update_kernel(double* __restrict__ x, const float* __restrict__ d, double
     x[i] += scale * (double)d[i];
 }
 
-int ir_solve(Op& op64, Op& op32, double alpha, double shift, PC* pc32, const void* b, void* x,
-             const ddgx_ir_cfg& cfg, ddgx_ir_info* info, hipStream_t s) {
-  CHECK(ensure_workspace(op64));
-  CHECK(ensure_workspace(op32));
-  const size_t n = (size_t)op64.K * op64.Np;
-  if (op32.ir_b == nullptr) {
-    DDGX_HIP(hipMalloc(&op32.ir_b, n * sizeof(float)));
-    DDGX_HIP(hipMalloc(&op32.ir_x, n * sizeof(float)));
+int ir_solve(const Op& op64, Work& w64, const Op& op32, Work& w32, double alpha, double shift,
+             const PC* pc32, MGWork* mgw, const void* b, void* x, const ddgx_ir_cfg& cfg,
+             ddgx_ir_info* info, hipStream_t s) {
+  CHECK(check_work(op64, w64));
+  CHECK(check_work(op32, w32));
+  if (w32.ir_b == nullptr) {
+    set_error("float32 workspace was created without refinement buffers");
+    return DDGX_ERR_INVALID;
   }
+  const size_t n = (size_t)op64.K * op64.Np;
   double bb = 0.0;
-  CHECK(dot(op64, b, b, &bb, s));
+  CHECK(dot(op64, w64, b, b, &bb, s));
   const double t = cfg.tol * cfg.tol * bb, a = cfg.atol * cfg.atol;
   const double atol2 = t > a ? t : a;
   ddgx_cg_cfg icfg{cfg.inner_tol, 0.0, cfg.inner_maxiter, cfg.check_every};
   int outer = 0, inner = 0, stalled = 0;
   double rr = 0.0, rr_prev = 0.0;
-  double* r = (double*)op64.r;
+  double* r = (double*)w64.r;
   while (true) {
     CHECK(launch_apply(op64, x, r, nullptr, alpha, shift, M_RESID, nullptr, nullptr, s, b));
-    CHECK(dot(op64, r, r, &rr, s));
+    CHECK(dot(op64, w64, r, r, &rr, s));
     if (!(rr > atol2) || outer >= cfg.max_outer) break;
     // the float64 residual has reached its round-off floor when two steps in
     // a row fail to reduce it by half: further steps cannot meet the test
@@ -445,14 +496,14 @@ int ir_solve(Op& op64, Op& op32, double alpha, double shift, PC* pc32, const voi
     rr_prev = rr;
     const double nr = std::sqrt(rr);
     hipLaunchKernelGGL(scale_down_kernel, dim3(op64.grid_vec), dim3(256), 0, s, r,
-                       (float*)op32.ir_b, 1.0 / nr, n);
+                       (float*)w32.ir_b, 1.0 / nr, n);
     DDGX_HIP(hipGetLastError());
-    DDGX_HIP(hipMemsetAsync(op32.ir_x, 0, n * sizeof(float), s));
+    DDGX_HIP(hipMemsetAsync(w32.ir_x, 0, n * sizeof(float), s));
     ddgx_cg_info ii{};
-    CHECK(pcg_solve(op32, alpha, shift, pc32, op32.ir_b, op32.ir_x, icfg, &ii, s));
+    CHECK(pcg_solve(op32, w32, alpha, shift, pc32, mgw, w32.ir_b, w32.ir_x, icfg, &ii, s));
     inner += ii.iterations;
     hipLaunchKernelGGL(refine_update_kernel, dim3(op64.grid_vec), dim3(256), 0, s, (double*)x,
-                       (const float*)op32.ir_x, nr, n);
+                       (const float*)w32.ir_x, nr, n);
     DDGX_HIP(hipGetLastError());
     ++outer;
   }
@@ -466,16 +517,16 @@ int ir_solve(Op& op64, Op& op32, double alpha, double shift, PC* pc32, const voi
   return DDGX_OK;
 }
 
-int dot(Op& op, const void* a, const void* b, double* out, hipStream_t s) {
-  CHECK(ensure_workspace(op));
+int dot(const Op& op, Work& w, const void* a, const void* b, double* out, hipStream_t s) {
+  CHECK(check_work(op, w));
   if (op.dtype == DDGX_F64)
-    CHECK(dot2<double>(op, a, b, nullptr, nullptr, nullptr, s));
+    CHECK(dot2<double>(op, w, a, b, nullptr, nullptr, nullptr, s));
   else
-    CHECK(dot2<float>(op, a, b, nullptr, nullptr, nullptr, s));
-  CHECK(finalize(op, op.grid_vec, F_SUM, s));
-  DDGX_HIP(hipMemcpyAsync(op.scal_host, op.scal, sizeof(CGScalars), hipMemcpyDeviceToHost, s));
+    CHECK(dot2<float>(op, w, a, b, nullptr, nullptr, nullptr, s));
+  CHECK(finalize(w, op.grid_vec, F_SUM, s));
+  DDGX_HIP(hipMemcpyAsync(w.scal_host, w.scal, sizeof(CGScalars), hipMemcpyDeviceToHost, s));
   DDGX_HIP(hipStreamSynchronize(s));
-  *out = op.scal_host->pAp;
+  *out = w.scal_host->pAp;
   return DDGX_OK;
 }
 

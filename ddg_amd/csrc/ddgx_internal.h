@@ -87,7 +87,8 @@ struct CGScalars {
 // k (0..K-1, 64 per tile) holds element perm[k].  Vectors (u, y, ...) keep
 // the caller's element order and are addressed through perm.
 constexpr int TILE = 64;      // elements per matrix-core workgroup
-constexpr int HALO_MAX = 64;  // out-of-tile neighbour elements per tile
+constexpr int HALO_MAX = 64;  // out-of-tile neighbour elements per tile (largest capacity)
+constexpr int HALO_SMALL = 32;  // capacity of the smaller-LDS variant of the apply
 // Packed per-face code of the apply: bits 0-7 LDS slot of the neighbour
 // trace (0..63 in the tile, 64+h halo), bits 8-9 neighbour face, bit 10
 // reversed node order, bits 11-12 face type.
@@ -110,23 +111,36 @@ struct Op {
   int* nbr = nullptr;    // 3 x K face codes (see above), tile order
   int* fcode = nullptr;  // 3 x K packed apply codes (fc_* below), tile order
   int* perm = nullptr;   // K: element id at each tile position
-  int* hid = nullptr;    // ntiles x HALO_MAX element ids of halo elements (-1: none)
-  void* hab = nullptr;   // ntiles x 6 x HALO_MAX halo face coefficients a_f, b_f (dtype)
+  bool perm_id = true;   // perm is the identity (the apply then skips it)
+  int* hpos = nullptr;   // ntiles x halo_cap tile positions of halo elements (-1: none)
   void* fsj = nullptr;   // K x 4 Neumann face Jacobians, tile order (dtype)
-  int grid_apply = 0;    // workgroups of one apply (64-element tiles)
+  int grid_apply = 0;    // workgroups of one apply: one per 64-element tile
   int max_halo = 0;      // largest halo of a tile
+  int halo_cap = 0;      // HALO_SMALL or HALO_MAX: halo entries stored per tile
   int grid_vec = 0;      // grid for vector kernels / reductions
-  // CG workspace
+  struct Work* ws = nullptr;  // default Krylov state (ddgx_pcg_solve / ddgx_ir_solve / ddgx_dot)
+};
+
+// Krylov state of one solve on an operator's vector shape and dtype.  The
+// operator itself is immutable after creation; everything a solve writes
+// lives here, so solves on one operator with different Work run at once.
+struct Work {
+  int64_t K = 0;
+  int Np = 0, dtype = 0, device = 0;
   void* r = nullptr;
   void* p = nullptr;
   void* z = nullptr;
   void* Ap = nullptr;
-  double* partials = nullptr;  // 2 * max(grid) doubles
-  void* ir_b = nullptr;        // refinement: scaled residual / correction (op dtype)
+  double* partials = nullptr;  // 2 * max(grid_apply, grid_vec) doubles
+  int npart = 0;
+  void* ir_b = nullptr;        // refinement (float32 work): scaled residual / correction
   void* ir_x = nullptr;
   CGScalars* scal = nullptr;
   CGScalars* scal_host = nullptr;  // pinned
+  size_t bytes = 0;            // device bytes held
 };
+int work_create(const Op& op, bool refine, Work** out);
+void work_free(Work* w);
 
 struct MG;
 
@@ -150,9 +164,17 @@ struct MGLevel {
   int tkind = 0;           // transfer to the next-coarser level: 0 = p, 1 = h
   int np_lo = 0;           // Np of the next-coarser level (p transfer)
   void* P = nullptr;       // p: Np x np_lo matrix; h: 4 x Np x Np child matrices (MG dtype)
-  void* b = nullptr;       // level right-hand side / solution / residual (MG dtype)
-  void* x = nullptr;
-  void* r = nullptr;
+};
+
+// Buffers one V-cycle writes (per-level right-hand side, iterate, residual
+// and the coarse-solve partial sums): one MGWork per concurrent solve.
+struct MGWork {
+  int nlev = 0;
+  void* b[16] = {};
+  void* x[16] = {};
+  void* r[16] = {};
+  double* gemv_part = nullptr;
+  size_t bytes = 0;
 };
 
 struct MG {
@@ -162,9 +184,11 @@ struct MG {
   MGLevel* lv = nullptr;
   int nlev = 0;
   double* Ainv = nullptr;       // coarsest level dense inverse (float64), n x n
-  double* gemv_part = nullptr;  // coarse solve partial sums
   int nc = 0;                   // unknowns of the coarsest level
+  MGWork* ws = nullptr;         // default cycle buffers (ddgx_pc_apply, ddgx_pcg_solve)
 };
+int mgwork_create(const MG& mg, MGWork** out);
+void mgwork_free(MGWork* w);
 
 void set_error(const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
@@ -179,27 +203,30 @@ int hip_fail(hipError_t e, const char* what);
 int launch_apply(const Op& op, const void* u, void* y, const void* udir, double alpha,
                  double shift, int mode, double* partial, const int* skip, hipStream_t s,
                  const void* bvec = nullptr);
-int mg_vcycle(MG& mg, const void* b, void* x, const int* skip, hipStream_t s);
+int mg_vcycle(const MG& mg, MGWork& w, const void* b, void* x, const int* skip, hipStream_t s);
 int mg_create(int nlev, Op* const* ops, Op* const* ops64, const int* tkind,
               const double* const* P, const ddgx_mg_cfg& cfg, hipStream_t s, MG** out);
 void mg_free(MG* mg);
 int launch_rhs(const Op& op, const void* f, const void* udir, const void* q, void* rhs,
                hipStream_t s);
-int apply_grid(int N, int dtype);
 int launch_block_probe(const Op& op, double* blocks, double alpha, hipStream_t s);
 int launch_invert_blocks(int N, int64_t K, int dtype, const double* blocks, void* binvT,
                          hipStream_t s);
 int launch_bj_apply(int N, int64_t K, int dtype, const void* binvT, const void* r, void* z,
                     hipStream_t s);
 
-int pcg_solve(Op& op, double alpha, double shift, PC* pc, const void* b, void* x,
-              const ddgx_cg_cfg& cfg, ddgx_cg_info* info, hipStream_t s);
-int dot(Op& op, const void* a, const void* b, double* out, hipStream_t s);
+int pcg_solve(const Op& op, Work& w, double alpha, double shift, const PC* pc, MGWork* mgw,
+              const void* b, void* x, const ddgx_cg_cfg& cfg, ddgx_cg_info* info,
+              hipStream_t s);
+int dot(const Op& op, Work& w, const void* a, const void* b, double* out, hipStream_t s);
 int launch_param_grad(const Op& op, const double* u, const double* lam, const double* udir,
                       const double* kappa, const double* tau, double* gk, double* gt,
                       hipStream_t s);
-int ir_solve(Op& op64, Op& op32, double alpha, double shift, PC* pc32, const void* b, void* x,
-             const ddgx_ir_cfg& cfg, ddgx_ir_info* info, hipStream_t s);
+int ir_solve(const Op& op64, Work& w64, const Op& op32, Work& w32, double alpha, double shift,
+             const PC* pc32, MGWork* mgw, const void* b, void* x, const ddgx_ir_cfg& cfg,
+             ddgx_ir_info* info, hipStream_t s);
+// default Work of an operator, created on first use
+int default_work(Op& op, bool refine, Work** out);
 
 int device_cu_count();
 

@@ -256,14 +256,15 @@ static int bj_smooth(const MGLevel& L, const void* r, void* x, double omega, int
   } while (0)
 
 template <typename T>
-static int coarse_solve(MG& mg, const void* b, void* x, const int* skip, hipStream_t s) {
+static int coarse_solve(const MG& mg, MGWork& w, const void* b, void* x, const int* skip,
+                        hipStream_t s) {
   const int n = mg.nc;
   const int nch = (n + GEMV_JC - 1) / GEMV_JC;
   hipLaunchKernelGGL((coarse_gemv_partial_kernel<T>), dim3((n + 255) / 256, nch), dim3(256), 0,
-                     s, (const double*)mg.Ainv, (const T*)b, mg.gemv_part, n, skip);
+                     s, (const double*)mg.Ainv, (const T*)b, w.gemv_part, n, skip);
   DDGX_HIP(hipGetLastError());
   hipLaunchKernelGGL((coarse_gemv_sum_kernel<T>), dim3((n + 255) / 256), dim3(256), 0, s,
-                     (const double*)mg.gemv_part, (T*)x, n, nch, skip);
+                     (const double*)w.gemv_part, (T*)x, n, nch, skip);
   DDGX_HIP(hipGetLastError());
   return DDGX_OK;
 }
@@ -289,40 +290,90 @@ static int transfer(const MGLevel& L, const MGLevel& C, bool prolong, const void
 }
 
 template <typename T>
-static int vcycle(MG& mg, int l, const void* b, void* x, const int* skip, hipStream_t s) {
-  if (l == mg.nlev - 1) return coarse_solve<T>(mg, b, x, skip, s);
-  MGLevel& L = mg.lv[l];
-  MGLevel& C = mg.lv[l + 1];
+static int vcycle(const MG& mg, MGWork& w, int l, const void* b, void* x, const int* skip,
+                  hipStream_t s) {
+  if (l == mg.nlev - 1) return coarse_solve<T>(mg, w, b, x, skip, s);
+  const MGLevel& L = mg.lv[l];
+  const MGLevel& C = mg.lv[l + 1];
   const Op& op = *L.op;
+  void* Lr = w.r[l];
+  void* Cb = w.b[l + 1];
+  void* Cx = w.x[l + 1];
   auto resid = [&]() {
-    return launch_apply(op, x, L.r, nullptr, mg.alpha, 0.0, M_RESID, nullptr, skip, s, b);
+    return launch_apply(op, x, Lr, nullptr, mg.alpha, 0.0, M_RESID, nullptr, skip, s, b);
   };
   if (mg.nu == 0) {
     const size_t n = (size_t)op.K * op.Np;
     hipLaunchKernelGGL((zero_kernel<T>), dim3(vec_grid(n)), dim3(256), 0, s, (T*)x, n, skip);
     DDGX_HIP(hipGetLastError());
-    MG_CHECK(transfer<T>(L, C, false, b, C.b, skip, s));
+    MG_CHECK(transfer<T>(L, C, false, b, Cb, skip, s));
   } else {
     MG_CHECK(bj_smooth<T>(L, b, x, mg.omega, 0, skip, s));
     for (int i = 1; i < mg.nu; ++i) {
       MG_CHECK(resid());
-      MG_CHECK(bj_smooth<T>(L, L.r, x, mg.omega, 1, skip, s));
+      MG_CHECK(bj_smooth<T>(L, Lr, x, mg.omega, 1, skip, s));
     }
     MG_CHECK(resid());
-    MG_CHECK(transfer<T>(L, C, false, L.r, C.b, skip, s));
+    MG_CHECK(transfer<T>(L, C, false, Lr, Cb, skip, s));
   }
-  MG_CHECK(vcycle<T>(mg, l + 1, C.b, C.x, skip, s));
-  MG_CHECK(transfer<T>(L, C, true, C.x, x, skip, s));
+  MG_CHECK(vcycle<T>(mg, w, l + 1, Cb, Cx, skip, s));
+  MG_CHECK(transfer<T>(L, C, true, Cx, x, skip, s));
   for (int i = 0; i < mg.nu; ++i) {
     MG_CHECK(resid());
-    MG_CHECK(bj_smooth<T>(L, L.r, x, mg.omega, 1, skip, s));
+    MG_CHECK(bj_smooth<T>(L, Lr, x, mg.omega, 1, skip, s));
   }
   return DDGX_OK;
 }
 
-int mg_vcycle(MG& mg, const void* b, void* x, const int* skip, hipStream_t s) {
-  if (mg.dtype == DDGX_F64) return vcycle<double>(mg, 0, b, x, skip, s);
-  return vcycle<float>(mg, 0, b, x, skip, s);
+int mg_vcycle(const MG& mg, MGWork& w, const void* b, void* x, const int* skip, hipStream_t s) {
+  if (w.nlev != mg.nlev) {
+    set_error("multigrid workspace does not match the hierarchy");
+    return DDGX_ERR_INVALID;
+  }
+  if (mg.dtype == DDGX_F64) return vcycle<double>(mg, w, 0, b, x, skip, s);
+  return vcycle<float>(mg, w, 0, b, x, skip, s);
+}
+
+void mgwork_free(MGWork* w) {
+  if (w == nullptr) return;
+  for (int l = 0; l < w->nlev; ++l) {
+    if (w->b[l]) (void)hipFree(w->b[l]);
+    if (w->x[l]) (void)hipFree(w->x[l]);
+    if (w->r[l]) (void)hipFree(w->r[l]);
+  }
+  if (w->gemv_part) (void)hipFree(w->gemv_part);
+  delete w;
+}
+
+// Level vectors of one cycle: b and x on levels 1.., r on every level but
+// the coarsest, and the coarse-solve partial sums.
+int mgwork_create(const MG& mg, MGWork** out) {
+  *out = nullptr;
+  MGWork* w = new MGWork();
+  w->nlev = mg.nlev;
+  hipError_t e = hipSuccess;
+  for (int l = 0; l < mg.nlev && e == hipSuccess; ++l) {
+    const size_t nv = (size_t)mg.lv[l].op->K * mg.lv[l].op->Np * dsz(mg.dtype);
+    if (l > 0) {
+      e = hipMalloc(&w->b[l], nv);
+      if (e == hipSuccess) e = hipMalloc(&w->x[l], nv);
+      w->bytes += 2 * nv;
+    }
+    if (e == hipSuccess && l + 1 < mg.nlev) {
+      e = hipMalloc(&w->r[l], nv);
+      w->bytes += nv;
+    }
+  }
+  const int nch = (mg.nc + GEMV_JC - 1) / GEMV_JC;
+  const size_t ng = (size_t)nch * mg.nc * sizeof(double);
+  if (e == hipSuccess) e = hipMalloc((void**)&w->gemv_part, ng);
+  w->bytes += ng;
+  if (e != hipSuccess) {
+    mgwork_free(w);
+    return hip_fail(e, "hipMalloc(multigrid cycle workspace)");
+  }
+  *out = w;
+  return DDGX_OK;
 }
 
 template <typename T>
@@ -340,15 +391,10 @@ void mg_free(MG* mg) {
     MGLevel& L = mg->lv[l];
     if (L.binvT) (void)hipFree(L.binvT);
     if (L.P) (void)hipFree(L.P);
-    if (L.r) (void)hipFree(L.r);
-    if (l > 0) {
-      if (L.b) (void)hipFree(L.b);
-      if (L.x) (void)hipFree(L.x);
-    }
   }
   delete[] mg->lv;
   if (mg->Ainv) (void)hipFree(mg->Ainv);
-  if (mg->gemv_part) (void)hipFree(mg->gemv_part);
+  mgwork_free(mg->ws);
   delete mg;
 }
 
@@ -471,19 +517,9 @@ int mg_create(int nlev, Op* const* ops, Op* const* ops64, const int* tkind,
       MGLevel& L = mg->lv[l];
       L.op = ops[l];
       L.alpha = cfg.alpha;
-      const size_t nv = (size_t)ops[l]->K * ops[l]->Np;
-      hipError_t he = hipSuccess;
-      if (l > 0) {
-        he = hipMalloc(&L.b, nv * dsz(dtype));
-        if (he == hipSuccess) he = hipMalloc(&L.x, nv * dsz(dtype));
-      }
-      if (l + 1 == nlev) {
-        if (he != hipSuccess) rc = hip_fail(he, "hipMalloc(level vectors)");
-        break;
-      }
-      if (he == hipSuccess) he = hipMalloc(&L.r, nv * dsz(dtype));
+      if (l + 1 == nlev) break;
       const size_t nb = (size_t)ops[l]->K * ops[l]->Np * ops[l]->Np;
-      if (he == hipSuccess) he = hipMalloc(&L.binvT, nb * dsz(dtype));
+      hipError_t he = hipMalloc(&L.binvT, nb * dsz(dtype));
       if (he != hipSuccess) { rc = hip_fail(he, "hipMalloc(level)"); break; }
       rc = launch_block_probe(*ops64[l], blocks, cfg.alpha, s);
       if (rc == DDGX_OK) rc = launch_invert_blocks(ops[l]->N, ops[l]->K, dtype, blocks, L.binvT, s);
@@ -508,9 +544,8 @@ int mg_create(int nlev, Op* const* ops, Op* const* ops64, const int* tkind,
       (void)hipFree(blocks);
       blocks = nullptr;
     }
-    const int nch = (mg->nc + GEMV_JC - 1) / GEMV_JC;
-    hipError_t he = hipMalloc((void**)&mg->gemv_part, (size_t)nch * mg->nc * sizeof(double));
-    if (he != hipSuccess) { rc = hip_fail(he, "hipMalloc(coarse workspace)"); break; }
+    rc = mgwork_create(*mg, &mg->ws);
+    if (rc != DDGX_OK) break;
     rc = build_coarse(*mg, *ops64[nlev - 1], cfg.alpha, cfg.regularize, cfg.eps, s);
   } while (false);
   if (blocks) (void)hipFree(blocks);

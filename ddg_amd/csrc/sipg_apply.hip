@@ -25,11 +25,18 @@ template <typename T, int N, int MODE>
 static int launch_t(const Op& op, const void* u, void* y, const void* udir, double alpha,
                     double shift, double* partial, const int* skip, hipStream_t s,
                     const void* bvec) {
-  // persistent workgroups over 64-element tiles; in dot mode each writes one partial
-  hipLaunchKernelGGL((sipg_apply_mfma_kernel<T, N, MODE>), dim3(op.grid_apply), dim3(256), 0, s,
-                     (const T*)op.frag, (const T*)op.rec, op.fcode, op.perm, op.hid,
-                     (const T*)op.hab, (const T*)u, (T*)y, (const T*)udir, (T)alpha, (T)shift,
-                     (int)op.K, partial, skip, (const T*)bvec);
+  // one workgroup per 64-element tile; in dot mode each writes one partial
+  const int* perm = op.perm_id ? nullptr : op.perm;
+  if (op.halo_cap == HALO_SMALL)
+    hipLaunchKernelGGL((sipg_apply_mfma_kernel<T, N, MODE, HALO_SMALL>), dim3(op.grid_apply),
+                       dim3(256), 0, s, (const T*)op.frag, (const T*)op.rec, op.fcode, perm,
+                       op.hpos, (const T*)u, (T*)y, (const T*)udir, (T)alpha, (T)shift,
+                       (int)op.K, partial, skip, (const T*)bvec);
+  else
+    hipLaunchKernelGGL((sipg_apply_mfma_kernel<T, N, MODE, HALO_MAX>), dim3(op.grid_apply),
+                       dim3(256), 0, s, (const T*)op.frag, (const T*)op.rec, op.fcode, perm,
+                       op.hpos, (const T*)u, (T*)y, (const T*)udir, (T)alpha, (T)shift,
+                       (int)op.K, partial, skip, (const T*)bvec);
   DDGX_HIP(hipGetLastError());
   return DDGX_OK;
 }
@@ -47,30 +54,6 @@ static int launch_mode(const Op& op, const void* u, void* y, const void* udir, d
       set_error("unsupported apply mode " + std::to_string(mode));
       return DDGX_ERR_UNSUPPORTED;
   }
-}
-
-// Persistent grid of the matrix-core apply: as many workgroups as fit on the
-// device at once (one per tile for small meshes).
-template <typename T, int N>
-static int apply_grid_t() {
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sipg_apply_mfma_kernel<T, N, 8>, 256,
-                                                   0) != hipSuccess ||
-      per_cu < 1)
-    per_cu = 1;
-  return per_cu * device_cu_count();
-}
-
-int apply_grid(int N, int dtype) {
-  switch (N) {
-    case 1: return dtype == DDGX_F64 ? apply_grid_t<double, 1>() : apply_grid_t<float, 1>();
-    case 2: return dtype == DDGX_F64 ? apply_grid_t<double, 2>() : apply_grid_t<float, 2>();
-    case 3: return dtype == DDGX_F64 ? apply_grid_t<double, 3>() : apply_grid_t<float, 3>();
-    case 4: return dtype == DDGX_F64 ? apply_grid_t<double, 4>() : apply_grid_t<float, 4>();
-    case 5: return dtype == DDGX_F64 ? apply_grid_t<double, 5>() : apply_grid_t<float, 5>();
-    case 6: return dtype == DDGX_F64 ? apply_grid_t<double, 6>() : apply_grid_t<float, 6>();
-  }
-  return 1;
 }
 
 int launch_apply(const Op& op, const void* u, void* y, const void* udir, double alpha,

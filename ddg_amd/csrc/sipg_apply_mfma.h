@@ -27,13 +27,13 @@
 // the per-face-node values P, a*X, b*X (9*Nfp rows); G = [E_f m1d_f^T | LR_f |
 // LS_f] lifts them.
 //
-// A workgroup is 4 waves; it processes tiles of 64 elements in a loop
-// (persistent workgroups, so the A fragments are loaded once per workgroup).
-// Face traces (u, dn) of the tile's elements and of its halo -- the out-of-tile neighbours, listed
-// per tile on the host -- are computed with the same GEMM 1 and exchanged
-// through LDS, so every face reads its neighbour trace from LDS through a
-// host-computed slot index.  All global loads of a tile are issued up front;
-// every element owns its output row (no atomics).
+// A workgroup is 4 waves and handles one tile of 64 elements; the grid has
+// one workgroup per tile, mapped XCD-contiguously (xcd_tile).  Face traces
+// (u, dn) of the tile's elements and of its halo -- the out-of-tile
+// neighbours, listed per tile on the host -- are computed with the same
+// GEMM 1 and exchanged through LDS, so every face reads its neighbour trace
+// from LDS through a host-computed slot index.  All global loads of a tile
+// are issued up front; every element owns its output row (no atomics).
 
 #pragma once
 
@@ -133,12 +133,23 @@ __device__ __forceinline__ void grad_and_traces(const T (&A1)[2 * MS<N>::NB][MS<
     }
 }
 
-template <typename T, int N, int MODE>
+// Tile handled by workgroup b of a G-workgroup launch.  Workgroups are
+// dispatched to the 8 XCDs round-robin (b % 8); each XCD gets one contiguous
+// run of tiles, so a tile's halo neighbours -- mostly the adjacent tiles of a
+// compact element order -- are fetched into the same XCD's L2.
+__device__ __forceinline__ int xcd_tile(int b, int G) {
+  constexpr int NX = 8;
+  const int q = G / NX, r = G % NX, x = b % NX;
+  return x * q + (x < r ? x : r) + b / NX;
+}
+
+// One 64-element tile per workgroup (grid = number of tiles).  HC is the
+// halo capacity of a tile (32 or 64 out-of-tile neighbour elements).
+template <typename T, int N, int MODE, int HC>
 __global__ void __launch_bounds__(256)
 sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
                        const int* __restrict__ fcode, const int* __restrict__ perm,
-                       const int* __restrict__ hid, const T* __restrict__ hab,
-                       const T* __restrict__ u,
+                       const int* __restrict__ hpos, const T* __restrict__ u,
                        T* __restrict__ y, const T* __restrict__ udir, T alpha, T shift, int K,
                        double* __restrict__ partial, const int* __restrict__ skip,
                        const T* __restrict__ bvec) {
@@ -149,8 +160,8 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
   constexpr int TRW = Q::TRW;
   constexpr bool HELM = MODE & M_HELM, UDIR = MODE & M_UDIR, DOT = MODE & M_DOT;
   constexpr bool RESID = MODE & M_RESID;
-  constexpr int XT = (TILE + HALO_MAX) * TRW;  // trace slots: tile then halo
-  constexpr int XF = TILE * Q::FS;             // face rows
+  constexpr int XT = (TILE + HC) * TRW;  // trace slots: tile then halo
+  constexpr int XF = TILE * Q::FS;       // face rows
   constexpr int XW = XT > XF ? XT : XF;
   // s_x holds the face traces during the face phase and, after a barrier,
   // the face rows F that feed GEMM 2.
@@ -164,37 +175,26 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
   const int g = lane >> 4, e = lane & 15;
-  const int el = 16 * wv + e;  // this lane's element within a tile
-  const int hi = 16 * wv + e;  // this lane's halo element within a tile
-  // A fragments: loaded once per workgroup from one of the table's replicas
-  // (all workgroups start at once; replicas spread those loads over L2)
-  const T* fr = frag + (size_t)(blockIdx.x % FRAG_COPIES) * Q::nfrag * 64;
-  T A1[2 * NB][KB];
-#pragma unroll
-  for (int b = 0; b < 2 * NB; ++b)
-#pragma unroll
-    for (int t = 0; t < KB; ++t) A1[b][t] = fr[(size_t)(Q::f1 + b * KB + t) * 64 + lane];
-  double dacc = 0.0;
-  const int ntiles = (K + TILE - 1) / TILE;
-
-  // persistent workgroups: each loops over tiles
-  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  const int el = 16 * wv + e;  // this lane's element (and halo entry) within the tile
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int k0 = tile * TILE;
   const int ne = min(TILE, K - k0);
-  const int k = k0 + el;       // this lane's tile position
+  const int k = k0 + el;  // this lane's tile position
   const bool valid = el < ne;
-  const int kg = valid ? perm[k] : 0;  // its element id
-  __syncthreads();  // the previous tile is done with LDS
+  const int kg = valid ? (perm ? perm[k] : k) : 0;  // its element id
 
   // ---- global loads of the tile: indices first, then nodal values --------
-  // halo: the tile's out-of-tile neighbours, 16 per wave, listed from slot 0
-  const bool hwave = hid[tile * HALO_MAX + 16 * wv] >= 0;  // uniform per wave
-  const int hg = hwave ? hid[tile * HALO_MAX + hi] : -1;
+  // halo: the tile's out-of-tile neighbours (tile positions), 16 per wave,
+  // listed from entry 0; a wave past the list skips the halo work
+  const int* hp = hpos + (size_t)tile * HC;
+  const bool hwave = 16 * wv < HC && hp[16 * wv] >= 0;  // uniform per wave
+  const int hk = hwave ? hp[el] : -1;
+  const int hg = hk >= 0 ? (perm ? perm[hk] : hk) : -1;
   FaceAB<T> abh{};
-  if (hwave && hg >= 0) {
-    const T* hb = hab + (size_t)tile * 6 * HALO_MAX + hi;
-    abh = FaceAB<T>{hb[0], hb[HALO_MAX], hb[2 * HALO_MAX], hb[3 * HALO_MAX], hb[4 * HALO_MAX],
-                    hb[5 * HALO_MAX]};
+  if (hk >= 0) {
+    const T* hr = rec + hk;
+    abh = FaceAB<T>{hr[(size_t)R_A * K], hr[(size_t)(R_A + 1) * K], hr[(size_t)(R_A + 2) * K],
+                    hr[(size_t)R_B * K], hr[(size_t)(R_B + 1) * K], hr[(size_t)(R_B + 2) * K]};
   }
   T U[NB][4], UH[NB][4];
 #pragma unroll
@@ -213,19 +213,14 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
     const int f = i / TILE, ee = i % TILE;
     s_fc[i] = ee < ne ? fcode[(size_t)f * K + k0 + ee] : (FT_NONE << 11);
   }
-  // GEMM 2 fragments (issued with the tile's other loads; L2 resident).
-  // The empty asm makes the pointer loop-variant so the compiler keeps these
-  // loads here instead of hoisting them (and 40 more registers) out of the loop.
-  const T* fr2 = fr;
-  asm volatile("" : "+s"(fr2));
-  T A2[NB][2 * KB + FK];
+  // GEMM 1 fragments, from one of the table's replicas (the many workgroups
+  // that start together spread these loads over L2 channels)
+  const T* fr = frag + (size_t)(blockIdx.x % FRAG_COPIES) * Q::nfrag * 64;
+  T A1[2 * NB][KB];
 #pragma unroll
-  for (int b = 0; b < NB; ++b) {
+  for (int b = 0; b < 2 * NB; ++b)
 #pragma unroll
-    for (int t = 0; t < 2 * KB; ++t) A2[b][t] = fr2[(size_t)(Q::f2 + b * 2 * KB + t) * 64 + lane];
-#pragma unroll
-    for (int t = 0; t < FK; ++t) A2[b][2 * KB + t] = fr2[(size_t)(Q::fF + b * FK + t) * 64 + lane];
-  }
+    for (int t = 0; t < KB; ++t) A1[b][t] = fr[(size_t)(Q::f1 + b * KB + t) * 64 + lane];
   __syncthreads();
 
   // ---- GEMM 1 and face traces: own elements, then the halo --------------
@@ -237,7 +232,7 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
   }
   if (hwave) {
     T HR[NB][4], HS[NB][4];
-    grad_and_traces<T, N>(A1, UH, HR, HS, abh, s_x + (TILE + hi) * TRW, g);
+    grad_and_traces<T, N>(A1, UH, HR, HS, abh, s_x + (TILE + el) * TRW, g);
   }
   // metric products (in place): Wr, Ws
 #pragma unroll
@@ -248,6 +243,19 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
       UR[b][r] = re[R_VRR] * a + re[R_VRS] * c;
       US[b][r] = re[R_VRS] * a + re[R_VSS] * c;
     }
+  // GEMM 2 fragments, issued here so their latency overlaps the face phase.
+  // The empty asm makes the pointer opaque so the compiler cannot move these
+  // loads (and their registers) up to the start of the kernel.
+  const T* fr2 = fr;
+  asm volatile("" : "+s"(fr2));
+  T A2[NB][2 * KB + FK];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+#pragma unroll
+    for (int t = 0; t < 2 * KB; ++t) A2[b][t] = fr2[(size_t)(Q::f2 + b * 2 * KB + t) * 64 + lane];
+#pragma unroll
+    for (int t = 0; t < FK; ++t) A2[b][2 * KB + t] = fr2[(size_t)(Q::fF + b * FK + t) * 64 + lane];
+  }
   __syncthreads();
 
   // ---- face phase: P, aX, bX per (element, face, node) ---------------------
@@ -272,10 +280,12 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
           uP = s_x[sl * TRW + (fn * Nfp + pn) * 2];
           dnP = -s_x[sl * TRW + (fn * Nfp + pn) * 2 + 1];
         } else if (ty == FT_DIRICHLET) {
-          if constexpr (UDIR)
-            uP = udir[(size_t)perm[k0 + e2] * Np + fmask<N>(f, p)];
-          else
+          if constexpr (UDIR) {
+            const int kk = k0 + e2;
+            uP = udir[(size_t)(perm ? perm[kk] : kk) * Np + fmask<N>(f, p)];
+          } else {
             uP = T(0);
+          }
           dnP = T(0);
         } else {
           uP = uM;
@@ -309,6 +319,7 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
   // ---- GEMM 2: volume + face lift (+ mass) -------------------------------
   T Jm = T(0);
   if constexpr (HELM) Jm = alpha * re[R_J];
+  double dacc = 0.0;
 #pragma unroll
   for (int bo = 0; bo < NB; ++bo) {
     v4 acc = {0, 0, 0, 0};
@@ -338,8 +349,6 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
       }
     }
   }
-
-  }  // tile loop
 
   if constexpr (DOT) {
     for (int off = 32; off > 0; off >>= 1) dacc += __shfl_down(dacc, off, 64);

@@ -196,6 +196,30 @@ int ddgx_ir_solve(ddgx_op op64, ddgx_op op32, double alpha, double shift, ddgx_p
  * dtype and length K*Np; result written to *out (host). */
 int ddgx_dot(ddgx_op op, const void* a, const void* b, double* out, void* stream);
 
+/* Solver context: private Krylov state (and multigrid cycle buffers) for
+ * solves with one operator and preconditioner.  No counterpart in the
+ * reference, whose JAX solves are functional and hold no state; it exists so
+ * that several solves on the same resident operators and preconditioner can
+ * run at once on different streams (the request loop in ddg_amd/serve.py).
+ * ddgx_pcg_solve / ddgx_ir_solve / ddgx_pc_apply use one default context per
+ * handle instead, so their concurrent calls on one handle race.
+ *   op32 == NULL: a PCG solver on op (pc of op's dtype or NULL);
+ *   op32 != NULL: a refinement solver (op float64, op32 its float32 twin, pc
+ *                 float32 or NULL).
+ * The operators and preconditioner must outlive the solver; one solver runs
+ * one solve at a time. */
+typedef struct ddgx_solver_s* ddgx_solver;
+int ddgx_solver_create(ddgx_op op, ddgx_op op32, ddgx_pc pc, ddgx_solver* out);
+/* Device bytes the solver holds (its share of a request's HBM footprint). */
+int ddgx_solver_bytes(ddgx_solver solver, int64_t* bytes);
+/* ddgx_pcg_solve on the solver's own state. */
+int ddgx_solver_pcg(ddgx_solver solver, double alpha, double shift, const void* b, void* x,
+                    const ddgx_cg_cfg* cfg, ddgx_cg_info* info, void* stream);
+/* ddgx_ir_solve on the solver's own state. */
+int ddgx_solver_ir(ddgx_solver solver, double alpha, double shift, const void* b, void* x,
+                   const ddgx_ir_cfg* cfg, ddgx_ir_info* info, void* stream);
+int ddgx_solver_destroy(ddgx_solver solver);
+
 #ifdef __cplusplus
 }
 #endif

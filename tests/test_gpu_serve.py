@@ -1,0 +1,83 @@
+"""Request loop on the GPU: every result is bitwise the same alone and in a
+mix of concurrent requests (stated tolerance: 0), served results meet the
+float64 tolerance against the oracle's dense solve, and admission keeps
+the slot and HBM limits."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import rel
+from oracle import mesh as om
+from oracle import sipg as S
+from ddg_amd.serve import (ServeConfig, SolveServer, make_requests, poisson_arrivals,
+                           fourier_field)
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def server(dev):
+    srv = SolveServer(ServeConfig(slots=4, keep_solutions=True, tol=1e-10))
+    yield srv
+    srv.close()
+
+
+def test_alone_equals_mix(server):
+    reqs = make_requests(24, seed=5, sizes=(8, 16, 32), orders=(2, 3, 4, 5), k2_range=(0, 50))
+    _say("serve: classes", sorted({(r.n, r.N) for r in reqs}))
+    server.prepare(reqs)
+    alone = {}
+    for r in reqs:
+        x, info = server.solve_one(r, server.streams[0])
+        torch.cuda.synchronize()
+        assert info.converged, (r, info.residual2, info.atol2)
+        alone[r.rid] = x.clone()
+    _say("serve: mix")
+    poisson_arrivals(reqs, None)
+    rep = server.run(reqs)
+    assert rep.peak_slots > 1, "requests never overlapped"
+    assert len(rep.results) == len(reqs)
+    for res in rep.results:
+        assert res.converged
+        assert torch.equal(res.x, alone[res.rid]), f"request {res.rid} differs in the mix"
+
+
+def test_results_match_oracle(server):
+    reqs = make_requests(3, seed=9, sizes=(8,), orders=(2, 3, 4), k2_range=(0, 50))
+    for r in reqs:
+        pc = server.problem_class(r.n, r.N)
+        m = pc.mesh
+        mo = om.build_mesh_2d(m.VX, m.VY, m.EToV, m.N)
+        _say(f"serve oracle n={r.n} N={r.N} k2={r.k2:.2f}")
+        x, info = server.solve_one(r, server.streams[1])
+        torch.cuda.synchronize()
+        f = fourier_field(pc.x, pc.y, r.seed).cpu().numpy().reshape(m.K, m.Np).T
+        b = S.mass_apply(mo, f).T.reshape(-1)
+        A = S.sipg_matrix_2d(mo, bc_types=pc.bc, alpha=r.k2)
+        xo = np.linalg.solve(A, b)
+        err = np.linalg.norm(x.cpu().numpy() - xo) / np.linalg.norm(xo)
+        assert info.converged and err < 1e-9, (err, info.outer)
+
+
+def test_poisson_stream_within_limits(server):
+    reqs = make_requests(40, seed=11, sizes=(8, 16), orders=(2, 3), k2_range=(0, 50))
+    poisson_arrivals(reqs, rate=400.0, seed=12)
+    server.prepare(reqs)
+    fp = max(server.problem_class(r.n, r.N).footprint for r in reqs)
+    server.cfg.keep_solutions = False
+    budget, server.cfg.hbm_budget = server.cfg.hbm_budget, 2 * fp
+    try:
+        rep = server.run(reqs)
+    finally:
+        server.cfg.hbm_budget = budget
+        server.cfg.keep_solutions = True
+    st = rep.stats()
+    _say("serve stream:", st)
+    assert st["requests"] == 40 and st["all_converged"]
+    assert rep.peak_bytes <= 2 * fp and rep.peak_slots <= 4
+    for res in rep.results:
+        assert res.start >= res.arrival and res.end >= res.start

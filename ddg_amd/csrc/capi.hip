@@ -74,7 +74,7 @@ static int upload_as(int dtype, const double* src, size_t n, void** dst) {
 }
 
 static void free_op(Op& o) {
-  void* ptrs[] = {o.tab, o.rec, o.nbr, o.fcode, o.perm, o.hpos, o.fsj};
+  void* ptrs[] = {o.tab, o.rec, o.nbr, o.fcode, o.perm, o.hpos, o.hab, o.fsj};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   work_free(o.ws);
@@ -180,6 +180,17 @@ int ddgx_op_create(int N, int64_t K, int dtype, const double* tables, const doub
   std::vector<int> hpos(ntiles * hcap);
   for (int64_t t = 0; t < ntiles; ++t)
     for (int h = 0; h < hcap; ++h) hpos[t * hcap + h] = hall[t * HALO_MAX + h];
+  // halo face coefficients a_f, b_f gathered per tile, hab[t][c][h], so the
+  // apply reads them with contiguous loads
+  std::vector<double> hab((size_t)ntiles * 6 * hcap, 0.0);
+  for (int64_t t = 0; t < ntiles; ++t)
+    for (int h = 0; h < hcap; ++h) {
+      const int pos = hpos[t * hcap + h];
+      if (pos < 0) continue;
+      const int64_t kid = perm[pos];
+      for (int c = 0; c < 6; ++c)
+        hab[((size_t)t * 6 + c) * hcap + h] = rec[kid * DDGX_REC_WORDS + R_A + c];
+    }
   auto* h = new ddgx_op_s();
   Op& o = h->op;
   o.N = N;
@@ -223,6 +234,7 @@ int ddgx_op_create(int N, int64_t K, int dtype, const double* tables, const doub
     }
     if ((rc = upload_as(dtype, recT.data(), recT.size(), &o.rec)) != DDGX_OK) break;
     if ((rc = upload_as(dtype, fsjT.data(), fsjT.size(), &o.fsj)) != DDGX_OK) break;
+    if ((rc = upload_as(dtype, hab.data(), hab.size(), &o.hab)) != DDGX_OK) break;
     if (!put(&o.nbr, nbrT, "nbr") || !put(&o.fcode, fcode, "fcode") || !put(&o.perm, perm, "perm") ||
         !put(&o.hpos, hpos, "hpos"))
       break;

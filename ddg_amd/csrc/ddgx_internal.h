@@ -113,6 +113,7 @@ struct Op {
   int* perm = nullptr;   // K: element id at each tile position
   bool perm_id = true;   // perm is the identity (the apply then skips it)
   int* hpos = nullptr;   // ntiles x halo_cap tile positions of halo elements (-1: none)
+  void* hab = nullptr;   // ntiles x 6 x halo_cap halo face coefficients a_f, b_f (dtype)
   void* fsj = nullptr;   // K x 4 Neumann face Jacobians, tile order (dtype)
   int grid_apply = 0;    // workgroups of one apply: one per 64-element tile
   int max_halo = 0;      // largest halo of a tile

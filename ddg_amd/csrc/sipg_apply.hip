@@ -30,12 +30,14 @@ static int launch_t(const Op& op, const void* u, void* y, const void* udir, doub
   if (op.halo_cap == HALO_SMALL)
     hipLaunchKernelGGL((sipg_apply_mfma_kernel<T, N, MODE, HALO_SMALL>), dim3(op.grid_apply),
                        dim3(256), 0, s, (const T*)op.frag, (const T*)op.rec, op.fcode, perm,
-                       op.hpos, (const T*)u, (T*)y, (const T*)udir, (T)alpha, (T)shift,
+                       op.hpos, (const T*)op.hab, (const T*)u, (T*)y, (const T*)udir, (T)alpha,
+                       (T)shift,
                        (int)op.K, partial, skip, (const T*)bvec);
   else
     hipLaunchKernelGGL((sipg_apply_mfma_kernel<T, N, MODE, HALO_MAX>), dim3(op.grid_apply),
                        dim3(256), 0, s, (const T*)op.frag, (const T*)op.rec, op.fcode, perm,
-                       op.hpos, (const T*)u, (T*)y, (const T*)udir, (T)alpha, (T)shift,
+                       op.hpos, (const T*)op.hab, (const T*)u, (T*)y, (const T*)udir, (T)alpha,
+                       (T)shift,
                        (int)op.K, partial, skip, (const T*)bvec);
   DDGX_HIP(hipGetLastError());
   return DDGX_OK;

@@ -74,7 +74,10 @@ struct MS {
   static constexpr int NF = 9 * Nfp;            // face rows: P, aX, bX
   static constexpr int FK = (NF + 3) / 4;       // k-steps over the face rows
   static constexpr int FS = 4 * FK + 1;         // LDS row stride of the face rows (odd)
-  static constexpr int TRW = 3 * Nfp * 2;       // trace words per element (u, dn per face node)
+  static constexpr int NS = 3 * Np + (3 * Np + 1) % 2;  // LDS words per element: u, ur, us (odd)
+  static constexpr int TRW = 3 * Nfp * 2;       // trace words per element: u, dn per face node
+  static constexpr int NFP3 = 3 * Nfp;          // face nodes of an element
+  static constexpr int FPW = NFP3 <= 8 ? 8 : NFP3 <= 16 ? 16 : 32;  // face-phase lanes per element
   static constexpr int f1 = 0;                  // GEMM1: 2*NB out blocks x KB
   static constexpr int f2 = f1 + 2 * NB * KB;   // GEMM2 volume: NB x 2*KB
   static constexpr int fF = f2 + NB * 2 * KB;   // GEMM2 face: NB x FK
@@ -133,6 +136,42 @@ __device__ __forceinline__ void grad_and_traces(const T (&A1)[2 * MS<N>::NB][MS<
     }
 }
 
+// GEMM 1 for 16 elements of a wave: [ur; us] = [Dr; Ds] u, and u, ur, us of
+// every node into the element's LDS slot (nodes in natural order).
+template <typename T, int N>
+__device__ __forceinline__ void grad_to_lds(const T (&A1)[2 * MS<N>::NB][MS<N>::KB],
+                                            const T (&U)[MS<N>::NB][4], T (&UR)[MS<N>::NB][4],
+                                            T (&US)[MS<N>::NB][4], T* sl, int g) {
+  using Q = MS<N>;
+  using MF = MfmaT<T>;
+  using v4 = typename MF::v4;
+  constexpr int NB = Q::NB, KB = Q::KB, Np = Q::Np;
+#pragma unroll
+  for (int b = 0; b < 2 * NB; ++b) {
+    v4 acc = {0, 0, 0, 0};
+#pragma unroll
+    for (int t = 0; t < KB; ++t) acc = MF::mma(A1[b][t], U[t / 4][t % 4], acc);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (b < NB)
+        UR[b][r] = acc[r];
+      else
+        US[b - NB][r] = acc[r];
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < NB; ++b)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int n = 16 * b + MF::crow(g, r);
+      if (n < Np) {
+        sl[n] = U[b][r];
+        sl[Np + n] = UR[b][r];
+        sl[2 * Np + n] = US[b][r];
+      }
+    }
+}
+
 // Tile handled by workgroup b of a G-workgroup launch.  Workgroups are
 // dispatched to the 8 XCDs round-robin (b % 8); each XCD gets one contiguous
 // run of tiles, so a tile's halo neighbours -- mostly the adjacent tiles of a
@@ -143,30 +182,46 @@ __device__ __forceinline__ int xcd_tile(int b, int G) {
   return x * q + (x < r ? x : r) + b / NX;
 }
 
+// Minimum waves per SIMD the register allocator is held to: 4 where the
+// operands fit in 128 registers; orders with two 16-row blocks (N >= 5)
+// need more registers per lane.
+template <typename T, int N>
+struct ApplyWaves {
+  static constexpr int value = N <= 4 ? 4 : (sizeof(T) == 8 ? 1 : 3);
+};
+
 // One 64-element tile per workgroup (grid = number of tiles).  HC is the
 // halo capacity of a tile (32 or 64 out-of-tile neighbour elements).
 template <typename T, int N, int MODE, int HC>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ApplyWaves<T, N>::value)))
 sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
                        const int* __restrict__ fcode, const int* __restrict__ perm,
-                       const int* __restrict__ hpos, const T* __restrict__ u,
-                       T* __restrict__ y, const T* __restrict__ udir, T alpha, T shift, int K,
-                       double* __restrict__ partial, const int* __restrict__ skip,
-                       const T* __restrict__ bvec) {
+                       const int* __restrict__ hpos, const T* __restrict__ hab,
+                       const T* __restrict__ u, T* __restrict__ y, const T* __restrict__ udir,
+                       T alpha, T shift, int K, double* __restrict__ partial,
+                       const int* __restrict__ skip, const T* __restrict__ bvec) {
   using Q = MS<N>;
   using MF = MfmaT<T>;
   using v4 = typename MF::v4;
   constexpr int Np = Q::Np, Nfp = Q::Nfp, NB = Q::NB, KB = Q::KB, FK = Q::FK;
-  constexpr int TRW = Q::TRW;
+  constexpr int NS = Q::NS, NFP3 = Q::NFP3, FPW = Q::FPW, TRW = Q::TRW;
+  // LDS layout of GEMM 1's output: float32 keeps u, ur, us of every node
+  // (the face phase forms the normal derivatives); float64, whose records
+  // are twice as large, keeps only u and dn at the face nodes so four
+  // workgroups fit in a compute unit's LDS.
+  constexpr bool NODE = sizeof(T) == 4;
+  constexpr int SW = NODE ? NS : TRW;       // LDS words per element slot
   constexpr bool HELM = MODE & M_HELM, UDIR = MODE & M_UDIR, DOT = MODE & M_DOT;
   constexpr bool RESID = MODE & M_RESID;
-  constexpr int XT = (TILE + HC) * TRW;  // trace slots: tile then halo
-  constexpr int XF = TILE * Q::FS;       // face rows
+  constexpr int RECS = DDGX_REC_WORDS + 1;  // LDS stride of a record (odd: no bank conflicts)
+  constexpr int XT = (TILE + HC) * SW;      // element slots: tile then halo
+  constexpr int XF = TILE * Q::FS;          // face rows
   constexpr int XW = XT > XF ? XT : XF;
-  // s_x holds the face traces during the face phase and, after a barrier,
-  // the face rows F that feed GEMM 2.
+  // s_x holds GEMM 1's output for the tile and its halo during the face
+  // phase and, after a barrier, the face rows F that feed GEMM 2.
   __shared__ T s_x[XW];
-  __shared__ T s_rec[TILE * DDGX_REC_WORDS];
+  __shared__ T s_rec[TILE * RECS];
+  __shared__ T s_hab[6 * HC];
   __shared__ int s_fc[3 * TILE];
   __shared__ double s_red[4];
 
@@ -190,12 +245,6 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
   const bool hwave = 16 * wv < HC && hp[16 * wv] >= 0;  // uniform per wave
   const int hk = hwave ? hp[el] : -1;
   const int hg = hk >= 0 ? (perm ? perm[hk] : hk) : -1;
-  FaceAB<T> abh{};
-  if (hk >= 0) {
-    const T* hr = rec + hk;
-    abh = FaceAB<T>{hr[(size_t)R_A * K], hr[(size_t)(R_A + 1) * K], hr[(size_t)(R_A + 2) * K],
-                    hr[(size_t)R_B * K], hr[(size_t)(R_B + 1) * K], hr[(size_t)(R_B + 2) * K]};
-  }
   T U[NB][4], UH[NB][4];
 #pragma unroll
   for (int b = 0; b < NB; ++b)
@@ -207,12 +256,13 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
     }
   for (int i = tid; i < DDGX_REC_WORDS * TILE; i += 256) {
     const int w = i / TILE, ee = i % TILE;
-    s_rec[ee * DDGX_REC_WORDS + w] = ee < ne ? rec[(size_t)w * K + k0 + ee] : T(0);
+    s_rec[ee * RECS + w] = ee < ne ? rec[(size_t)w * K + k0 + ee] : T(0);
   }
   for (int i = tid; i < 3 * TILE; i += 256) {
     const int f = i / TILE, ee = i % TILE;
     s_fc[i] = ee < ne ? fcode[(size_t)f * K + k0 + ee] : (FT_NONE << 11);
   }
+  for (int i = tid; i < 6 * HC; i += 256) s_hab[i] = hab[(size_t)tile * 6 * HC + i];
   // GEMM 1 fragments, from one of the table's replicas (the many workgroups
   // that start together spread these loads over L2 channels)
   const T* fr = frag + (size_t)(blockIdx.x % FRAG_COPIES) * Q::nfrag * 64;
@@ -223,16 +273,24 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
     for (int t = 0; t < KB; ++t) A1[b][t] = fr[(size_t)(Q::f1 + b * KB + t) * 64 + lane];
   __syncthreads();
 
-  // ---- GEMM 1 and face traces: own elements, then the halo --------------
-  const T* re = s_rec + el * DDGX_REC_WORDS;
+  // ---- GEMM 1: own elements and halo into LDS ----------------------------
+  const T* re = s_rec + el * RECS;
   T UR[NB][4], US[NB][4];
-  {
+  if constexpr (NODE) {
+    grad_to_lds<T, N>(A1, U, UR, US, s_x + el * SW, g);
+    if (hwave) {
+      T HR[NB][4], HS[NB][4];
+      grad_to_lds<T, N>(A1, UH, HR, HS, s_x + (TILE + el) * SW, g);
+    }
+  } else {
     const FaceAB<T> ab{re[R_A], re[R_A + 1], re[R_A + 2], re[R_B], re[R_B + 1], re[R_B + 2]};
-    grad_and_traces<T, N>(A1, U, UR, US, ab, s_x + el * TRW, g);
-  }
-  if (hwave) {
-    T HR[NB][4], HS[NB][4];
-    grad_and_traces<T, N>(A1, UH, HR, HS, abh, s_x + (TILE + el) * TRW, g);
+    grad_and_traces<T, N>(A1, U, UR, US, ab, s_x + el * SW, g);
+    if (hwave) {
+      const FaceAB<T> abh{s_hab[el], s_hab[HC + el], s_hab[2 * HC + el],
+                          s_hab[3 * HC + el], s_hab[4 * HC + el], s_hab[5 * HC + el]};
+      T HR[NB][4], HS[NB][4];
+      grad_and_traces<T, N>(A1, UH, HR, HS, abh, s_x + (TILE + el) * SW, g);
+    }
   }
   // metric products (in place): Wr, Ws
 #pragma unroll
@@ -243,71 +301,91 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
       UR[b][r] = re[R_VRR] * a + re[R_VRS] * c;
       US[b][r] = re[R_VRS] * a + re[R_VSS] * c;
     }
-  // GEMM 2 fragments, issued here so their latency overlaps the face phase.
-  // The empty asm makes the pointer opaque so the compiler cannot move these
-  // loads (and their registers) up to the start of the kernel.
-  const T* fr2 = fr;
-  asm volatile("" : "+s"(fr2));
+  // GEMM 2 fragments, issued here so their latency overlaps the face phase
   T A2[NB][2 * KB + FK];
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
 #pragma unroll
-    for (int t = 0; t < 2 * KB; ++t) A2[b][t] = fr2[(size_t)(Q::f2 + b * 2 * KB + t) * 64 + lane];
+    for (int t = 0; t < 2 * KB; ++t) A2[b][t] = fr[(size_t)(Q::f2 + b * 2 * KB + t) * 64 + lane];
 #pragma unroll
-    for (int t = 0; t < FK; ++t) A2[b][2 * KB + t] = fr2[(size_t)(Q::fF + b * FK + t) * 64 + lane];
+    for (int t = 0; t < FK; ++t) A2[b][2 * KB + t] = fr[(size_t)(Q::fF + b * FK + t) * 64 + lane];
   }
   __syncthreads();
 
-  // ---- face phase: P, aX, bX per (element, face, node) ---------------------
-  constexpr int NIT = (TILE * 3 * Nfp + 255) / 256;
-  T fv[NIT][3];
+  // ---- face phase: P, aX, bX per (element, face node) ---------------------
+  // thread -> one face node (f, p) of elements eg, eg + EG, ...
+  constexpr int EG = 256 / FPW, ITE = TILE / EG;
+  const int fp = tid % FPW, eg = tid / FPW;
+  const bool fpa = fp < NFP3;
+  const int ff = fpa ? fp / Nfp : 0, pp = fpa ? fp % Nfp : 0;
+  const int nM = fmask<N>(ff, pp);
+  T fv[ITE][3];
 #pragma unroll
-  for (int it = 0; it < NIT; ++it) {
-    const int item = tid + 256 * it;
+  for (int it = 0; it < ITE; ++it) {
+    const int e2 = eg + EG * it;
     fv[it][0] = fv[it][1] = fv[it][2] = T(0);
-    if (item < TILE * 3 * Nfp) {
-      const int e2 = item / (3 * Nfp), f = (item / Nfp) % 3, p = item % Nfp;
-      if (e2 < ne) {
-        const T* rr = s_rec + e2 * DDGX_REC_WORDS;
-        const T uM = s_x[e2 * TRW + (f * Nfp + p) * 2];
-        const T dnM = s_x[e2 * TRW + (f * Nfp + p) * 2 + 1];
-        const int c = s_fc[f * TILE + e2];
-        const int ty = fc_type(c);
-        T uP, dnP;
-        if (ty == FT_INTERIOR) {
-          const int sl = fc_slot(c), fn = fc_face(c);
-          const int pn = fc_rev(c) ? (Nfp - 1 - p) : p;
-          uP = s_x[sl * TRW + (fn * Nfp + pn) * 2];
-          dnP = -s_x[sl * TRW + (fn * Nfp + pn) * 2 + 1];
-        } else if (ty == FT_DIRICHLET) {
-          if constexpr (UDIR) {
-            const int kk = k0 + e2;
-            uP = udir[(size_t)(perm ? perm[kk] : kk) * Np + fmask<N>(f, p)];
-          } else {
-            uP = T(0);
-          }
-          dnP = T(0);
-        } else {
-          uP = uM;
-          dnP = T(0);
-        }
-        const T ju = uM - uP;
-        const T X = -rr[R_SJW + f] * ju;
-        fv[it][0] = rr[R_PEN + f] * ju - rr[R_SJW + f] * (dnM + dnP);
-        fv[it][1] = rr[R_A + f] * X;
-        fv[it][2] = rr[R_B + f] * X;
+    if (fpa && e2 < ne) {
+      const T* rr = s_rec + e2 * RECS;
+      const T* xm = s_x + e2 * SW;
+      const T aM = rr[R_A + ff], bM = rr[R_B + ff];
+      T uM, dnM;
+      if constexpr (NODE) {
+        uM = xm[nM];
+        dnM = aM * xm[Np + nM] + bM * xm[2 * Np + nM];
+      } else {
+        uM = xm[fp * 2];
+        dnM = xm[fp * 2 + 1];
       }
+      const int c = s_fc[ff * TILE + e2];
+      const int ty = fc_type(c);
+      T uP, dnP;
+      if (ty == FT_INTERIOR) {
+        const int sl = fc_slot(c), fn = fc_face(c);
+        const int pn = fc_rev(c) ? (Nfp - 1 - pp) : pp;
+        const T* xp = s_x + sl * SW;
+        if constexpr (NODE) {
+          const int nP = fmask<N>(fn, pn);
+          T aP, bP;  // the neighbour's own face coefficients (its outward normal)
+          if (sl < TILE) {
+            aP = s_rec[sl * RECS + R_A + fn];
+            bP = s_rec[sl * RECS + R_B + fn];
+          } else {
+            aP = s_hab[fn * HC + sl - TILE];
+            bP = s_hab[(3 + fn) * HC + sl - TILE];
+          }
+          uP = xp[nP];
+          dnP = -(aP * xp[Np + nP] + bP * xp[2 * Np + nP]);
+        } else {
+          uP = xp[(fn * Nfp + pn) * 2];
+          dnP = -xp[(fn * Nfp + pn) * 2 + 1];
+        }
+      } else if (ty == FT_DIRICHLET) {
+        if constexpr (UDIR) {
+          const int kk = k0 + e2;
+          uP = udir[(size_t)(perm ? perm[kk] : kk) * Np + nM];
+        } else {
+          uP = T(0);
+        }
+        dnP = T(0);
+      } else {
+        uP = uM;
+        dnP = T(0);
+      }
+      const T ju = uM - uP;
+      const T X = -rr[R_SJW + ff] * ju;
+      fv[it][0] = rr[R_PEN + ff] * ju - rr[R_SJW + ff] * (dnM + dnP);
+      fv[it][1] = aM * X;
+      fv[it][2] = bM * X;
     }
   }
-  __syncthreads();  // traces consumed: s_x becomes the face rows
+  __syncthreads();  // nodal values consumed: s_x becomes the face rows
 #pragma unroll
-  for (int it = 0; it < NIT; ++it) {
-    const int item = tid + 256 * it;
-    if (item < TILE * 3 * Nfp) {
-      const int e2 = item / (3 * Nfp), fp = item % (3 * Nfp);
+  for (int it = 0; it < ITE; ++it) {
+    const int e2 = eg + EG * it;
+    if (fpa) {
       s_x[e2 * Q::FS + fp] = fv[it][0];
-      s_x[e2 * Q::FS + 3 * Nfp + fp] = fv[it][1];
-      s_x[e2 * Q::FS + 6 * Nfp + fp] = fv[it][2];
+      s_x[e2 * Q::FS + NFP3 + fp] = fv[it][1];
+      s_x[e2 * Q::FS + 2 * NFP3 + fp] = fv[it][2];
     }
   }
   constexpr int PADC = 4 * FK - Q::NF;
@@ -322,13 +400,16 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
   double dacc = 0.0;
 #pragma unroll
   for (int bo = 0; bo < NB; ++bo) {
-    v4 acc = {0, 0, 0, 0};
+    // three independent accumulator chains (volume r, volume s, faces) so
+    // consecutive matrix-core products do not wait on each other
+    v4 acc = {0, 0, 0, 0}, acs = {0, 0, 0, 0}, acf = {0, 0, 0, 0};
 #pragma unroll
-    for (int t = 0; t < KB; ++t) acc = MF::mma(A2[bo][t], UR[t / 4][t % 4], acc);
+    for (int t = 0; t < KB; ++t) {
+      acc = MF::mma(A2[bo][t], UR[t / 4][t % 4], acc);
+      acs = MF::mma(A2[bo][KB + t], US[t / 4][t % 4], acs);
+    }
 #pragma unroll
-    for (int t = 0; t < KB; ++t) acc = MF::mma(A2[bo][KB + t], US[t / 4][t % 4], acc);
-#pragma unroll
-    for (int t = 0; t < FK; ++t) acc = MF::mma(A2[bo][2 * KB + t], s_x[el * Q::FS + 4 * t + g], acc);
+    for (int t = 0; t < FK; ++t) acf = MF::mma(A2[bo][2 * KB + t], s_x[el * Q::FS + 4 * t + g], acf);
     if constexpr (HELM) {
 #pragma unroll
       for (int t = 0; t < KB; ++t)
@@ -337,7 +418,7 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int n = 16 * bo + MF::crow(g, r);
-      T v = acc[r];
+      T v = acc[r] + acs[r] + acf[r];
       if constexpr (HELM) v += shift * U[bo][r];
       if (valid && n < Np) {
         const size_t gi = (size_t)kg * Np + n;

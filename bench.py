@@ -202,6 +202,8 @@ def solver_block(m, bc, b, mg64, mg32, n_solves, tol, inner_tol, ref_x=None, ue=
         if "outer" in info:
             d["outer"] = info["outer"]
         d["converged"] = info["converged"]
+        if "at_floor" in info:
+            d["at_residual_floor"] = info["at_floor"]
         d["rel_err_vs_fp64_tight"] = float(torch.linalg.norm(x - ref_x)) / nref
         if ue is not None:
             xs = x.cpu().numpy().reshape(m.K, m.Np).T
@@ -251,7 +253,8 @@ def serve_block(n_requests, slots, calib=200, load=0.8, sample=8, seed=0):
     out = {"requests": st["requests"], "offered_rate_per_s": round(rate, 2),
            "solves_per_s": st["solves_per_s"], "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
            "service_p50_ms": st["service_p50_ms"], "service_p99_ms": st["service_p99_ms"],
-           "all_converged": st["all_converged"], "slots": slots,
+           "converged": st["converged"], "at_residual_floor": st["at_residual_floor"],
+           "max_rel_residual": st["max_rel_residual"], "slots": slots,
            "peak_slots_busy": st["peak_slots_busy"],
            "closed_loop_capacity_per_s": cap["solves_per_s"], "setup_s": round(setup, 2),
            "tol": 1e-10, "sample_max_rel_err_vs_fp64": max(errs) if errs else None,
@@ -295,6 +298,8 @@ def main():
                     help="time a device-to-device copy at several sizes and exit")
     ap.add_argument("--profile-apply", choices=["f64", "f32"], default=None,
                     help="only run --steps eager applies (for rocprofv3 counter runs)")
+    ap.add_argument("--profile-solve", action="store_true",
+                    help="only run a few config-1 solves (FP64 MG-PCG and mixed) for rocprofv3")
     ap.add_argument("--profile-copy", type=int, default=0, metavar="MB",
                     help="only run --steps elementwise copies y = 2 x of MB megabytes each "
                          "(calibrates the HBM byte counters)")
@@ -357,6 +362,30 @@ def main():
             op.apply(u, y)
         torch.cuda.synchronize()
         print(f"[profile] {args.steps} applies {args.profile_apply} K={K} N={m.N}", flush=True)
+        return
+
+    if args.profile_solve:
+        from ddg_amd import sipg as D
+        from ddg_amd.operator import refine_solve
+        mg64, fin = hpmg_for(m, 4, torch.float64)
+        mg32, _ = hpmg_for(fin, 4, torch.float32)
+        f, ud, q = problem_data(fin)
+        op64 = D.get_operator(fin, bc)
+        op32 = D.get_operator(fin, bc, dtype=torch.float32)
+        b = op64.rhs(_flat(f, dev), u_dir=_flat(ud, dev), q_neu=_flat(q, dev))
+        for _ in range(max(1, args.solves)):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _, i64 = op64.pcg(b, pc=mg64, tol=args.tol, maxiter=5000)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            _, imx = refine_solve(op64, op32, b, pc32=mg32, tol=args.tol, inner_tol=args.inner_tol,
+                                  inner_maxiter=2000)
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            print(f"[profile] fp64 {1e3 * (t1 - t0):.1f} ms {i64['iterations']} it; mixed "
+                  f"{1e3 * (t2 - t1):.1f} ms {imx['iterations']} it {imx['outer']} outer",
+                  flush=True)
         return
 
     op = SipgOperator(m, bc)

@@ -48,7 +48,7 @@ class IRConfig(C.Structure):
 
 class IRInfo(C.Structure):
     _fields_ = [("outer", C.c_int), ("inner_total", C.c_int), ("converged", C.c_int),
-                ("residual2", C.c_double), ("atol2", C.c_double)]
+                ("residual2", C.c_double), ("atol2", C.c_double), ("at_floor", C.c_int)]
 
 
 class DdgxError(RuntimeError):

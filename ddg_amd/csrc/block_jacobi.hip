@@ -7,8 +7,9 @@
 // zero -- exactly what a colour probe sees, since face neighbours never share
 // a colour -- so Np launches give every block's column j.  The blocks are
 // inverted in float64 by Gauss-Jordan elimination (SPD blocks, no pivoting),
-// one wavefront per element with the augmented block in LDS, and stored
-// transposed in the precision of the vectors they will multiply.
+// one wavefront per element with the augmented block in LDS, and stored as
+// packed symmetric upper triangles in the precision of the vectors they
+// will multiply.
 
 #include "ddgx_internal.h"
 
@@ -108,7 +109,7 @@ int launch_block_probe(const Op& op, double* blocks, double alpha, hipStream_t s
 
 template <typename T, int N>
 __global__ void __launch_bounds__(64)
-invert_blocks_kernel(const double* __restrict__ blocks, T* __restrict__ binvT, int K) {
+invert_blocks_kernel(const double* __restrict__ blocks, T* __restrict__ binv, int K) {
   constexpr int Np = Shape<N>::Np;
   constexpr int W = 2 * Np;  // augmented width, <= 56
   __shared__ double A[Np][W + 1];
@@ -136,37 +137,44 @@ invert_blocks_kernel(const double* __restrict__ blocks, T* __restrict__ binvT, i
       }
       __syncthreads();
     }
-    T* out = binvT + (size_t)k * Np * Np;
+    // upper triangle of the symmetric part, packed (the inverse of a
+    // symmetric block, symmetric up to rounding)
+    T* out = binv + (size_t)k * SymPack<N>::NPK;
     for (int idx = t; idx < Np * Np; idx += 64) {
-      const int j = idx / Np, i = idx % Np;  // out[j][i] = inv[i][j]
-      out[idx] = (T)A[i][Np + j];
+      const int i = idx / Np, j = idx % Np;
+      if (i <= j) out[SymPack<N>::pk(i, j)] = (T)(0.5 * (A[i][Np + j] + A[j][Np + i]));
     }
   }
 }
 
 template <int N>
-static int invert_t(int64_t K, int dtype, const double* blocks, void* binvT, hipStream_t s) {
+static int invert_t(int64_t K, int dtype, const double* blocks, void* binv, hipStream_t s) {
   int grid = 16 * device_cu_count();
   if ((int64_t)grid > K) grid = (int)K;
   if (dtype == DDGX_F64)
     hipLaunchKernelGGL((invert_blocks_kernel<double, N>), dim3(grid), dim3(64), 0, s, blocks,
-                       (double*)binvT, (int)K);
+                       (double*)binv, (int)K);
   else
     hipLaunchKernelGGL((invert_blocks_kernel<float, N>), dim3(grid), dim3(64), 0, s, blocks,
-                       (float*)binvT, (int)K);
+                       (float*)binv, (int)K);
   DDGX_HIP(hipGetLastError());
   return DDGX_OK;
 }
 
-int launch_invert_blocks(int N, int64_t K, int dtype, const double* blocks, void* binvT,
+size_t packed_block_words(int N) {
+  const size_t Np = (size_t)(N + 1) * (N + 2) / 2;
+  return Np * (Np + 1) / 2;
+}
+
+int launch_invert_blocks(int N, int64_t K, int dtype, const double* blocks, void* binv,
                          hipStream_t s) {
   switch (N) {
-    case 1: return invert_t<1>(K, dtype, blocks, binvT, s);
-    case 2: return invert_t<2>(K, dtype, blocks, binvT, s);
-    case 3: return invert_t<3>(K, dtype, blocks, binvT, s);
-    case 4: return invert_t<4>(K, dtype, blocks, binvT, s);
-    case 5: return invert_t<5>(K, dtype, blocks, binvT, s);
-    case 6: return invert_t<6>(K, dtype, blocks, binvT, s);
+    case 1: return invert_t<1>(K, dtype, blocks, binv, s);
+    case 2: return invert_t<2>(K, dtype, blocks, binv, s);
+    case 3: return invert_t<3>(K, dtype, blocks, binv, s);
+    case 4: return invert_t<4>(K, dtype, blocks, binv, s);
+    case 5: return invert_t<5>(K, dtype, blocks, binv, s);
+    case 6: return invert_t<6>(K, dtype, blocks, binv, s);
   }
   set_error("polynomial order not built");
   return DDGX_ERR_UNSUPPORTED;

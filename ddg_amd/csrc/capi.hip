@@ -36,12 +36,17 @@ int hip_fail(hipError_t e, const char* what) {
   return e == hipErrorOutOfMemory ? DDGX_ERR_NOMEM : DDGX_ERR_HIP;
 }
 
+// Compute units of the current device, queried once per device (launch code
+// runs inside stream capture, where device queries are best avoided).
 int device_cu_count() {
+  static int cached[64] = {0};
   int dev = 0, cus = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 256;
+  if (dev >= 0 && dev < 64 && cached[dev] > 0) return cached[dev];
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
       cus <= 0)
     return 256;
+  if (dev >= 0 && dev < 64) cached[dev] = cus;
   return cus;
 }
 
@@ -318,18 +323,18 @@ int ddgx_block_jacobi_create(ddgx_op op64, double alpha, int dtype, void* stream
   do {
     hipError_t e = hipMalloc((void**)&pc.blocks, nb * sizeof(double));
     if (e != hipSuccess) { rc = hip_fail(e, "hipMalloc(blocks)"); break; }
-    e = hipMalloc(&pc.binvT, nb * (dtype == DDGX_F64 ? 8 : 4));
-    if (e != hipSuccess) { rc = hip_fail(e, "hipMalloc(binvT)"); break; }
+    e = hipMalloc(&pc.binv, (size_t)o.K * packed_block_words(o.N) * (dtype == DDGX_F64 ? 8 : 4));
+    if (e != hipSuccess) { rc = hip_fail(e, "hipMalloc(binv)"); break; }
     rc = launch_block_probe(o, pc.blocks, alpha, s);
     if (rc != DDGX_OK) break;
-    rc = launch_invert_blocks(o.N, o.K, dtype, pc.blocks, pc.binvT, s);
+    rc = launch_invert_blocks(o.N, o.K, dtype, pc.blocks, pc.binv, s);
     if (rc != DDGX_OK) break;
     e = hipStreamSynchronize(s);
     if (e != hipSuccess) { rc = hip_fail(e, "block-Jacobi build"); break; }
   } while (false);
   if (rc != DDGX_OK) {
     if (pc.blocks) (void)hipFree(pc.blocks);
-    if (pc.binvT) (void)hipFree(pc.binvT);
+    if (pc.binv) (void)hipFree(pc.binv);
     delete h;
     return rc;
   }
@@ -364,13 +369,13 @@ int ddgx_pc_apply(ddgx_pc pc, const void* r, void* z, void* stream) {
   const Op& o = *pc->pc.op;
   if (pc->pc.type == 1)
     return mg_vcycle(*pc->pc.mg, *pc->pc.mg->ws, r, z, nullptr, (hipStream_t)stream);
-  return launch_bj_apply(o.N, o.K, pc->pc.dtype, pc->pc.binvT, r, z, (hipStream_t)stream);
+  return launch_bj_apply(o.N, o.K, pc->pc.dtype, pc->pc.binv, r, z, (hipStream_t)stream);
 }
 
 int ddgx_pc_destroy(ddgx_pc pc) {
   if (pc == nullptr) return DDGX_OK;
   if (pc->pc.blocks) (void)hipFree(pc->pc.blocks);
-  if (pc->pc.binvT) (void)hipFree(pc->pc.binvT);
+  if (pc->pc.binv) (void)hipFree(pc->pc.binv);
   if (pc->pc.mg) mg_free(pc->pc.mg);
   delete pc;
   return DDGX_OK;
@@ -520,6 +525,9 @@ int ddgx_solver_create(ddgx_op op, ddgx_op op32, ddgx_pc pc, ddgx_solver* out) {
   h->pc = pc ? &pc->pc : nullptr;
   int rc = work_create(a, false, &h->w);
   if (rc == DDGX_OK && op32 != nullptr) rc = work_create(op32->op, true, &h->w32);
+  // a solver context replays captured chunks of CG iterations
+  if (rc == DDGX_OK) h->w->use_graph = true;
+  if (rc == DDGX_OK && h->w32) h->w32->use_graph = true;
   if (rc == DDGX_OK && pc != nullptr && pc->pc.type == 1) rc = mgwork_create(*pc->pc.mg, &h->mgw);
   if (rc != DDGX_OK) {
     free_solver(h);

@@ -144,12 +144,13 @@ update_kernel(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
 template <typename T, int N>
 __global__ void __launch_bounds__(256)
 update_bj_kernel(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
-                 const T* __restrict__ Ap, const T* __restrict__ binvT, T* __restrict__ z,
+                 const T* __restrict__ Ap, const T* __restrict__ binv, T* __restrict__ z,
                  const CGScalars* __restrict__ sc, int K, double* __restrict__ partial) {
   using S = Shape<N>;
   constexpr int Np = S::Np, NL = S::NL, EPB = S::EPB;
   if (sc->done) return;
   __shared__ T s_r[EPB * Np];
+  __shared__ T s_b[EPB * SymPack<N>::NPK];
   __shared__ double s_red[4];
   const T alpha = (T)sc->alpha;
   const int tid = threadIdx.x, e = tid / NL, li = tid % NL;
@@ -166,12 +167,10 @@ update_bj_kernel(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
       r[g] = ri;
       s_r[i] = ri;
     }
+    stage_blocks<T, N>(binv, s_b, k0, ne, tid);
     __syncthreads();
     if (li < Np && e < ne) {
-      const T* B = binvT + (size_t)(k0 + e) * Np * Np;
-      T acc = 0;
-#pragma unroll
-      for (int j = 0; j < Np; ++j) acc += B[j * Np + li] * s_r[e * Np + j];
+      const T acc = block_row<T, N>(s_b, s_r, e, li);
       z[(size_t)(k0 + e) * Np + li] = acc;
       const double ri = (double)s_r[e * Np + li];
       rz += ri * (double)acc;
@@ -189,10 +188,11 @@ update_bj_kernel(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
 // standalone block-Jacobi apply z = Binv r
 template <typename T, int N>
 __global__ void __launch_bounds__(256)
-bj_apply_kernel(const T* __restrict__ binvT, const T* __restrict__ r, T* __restrict__ z, int K) {
+bj_apply_kernel(const T* __restrict__ binv, const T* __restrict__ r, T* __restrict__ z, int K) {
   using S = Shape<N>;
   constexpr int Np = S::Np, NL = S::NL, EPB = S::EPB;
   __shared__ T s_r[EPB * Np];
+  __shared__ T s_b[EPB * SymPack<N>::NPK];
   const int tid = threadIdx.x, e = tid / NL, li = tid % NL;
   const int ntiles = (K + EPB - 1) / EPB;
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -200,14 +200,9 @@ bj_apply_kernel(const T* __restrict__ binvT, const T* __restrict__ r, T* __restr
     const int ne = min(EPB, K - k0);
     __syncthreads();
     for (int i = tid; i < ne * Np; i += 256) s_r[i] = r[(size_t)k0 * Np + i];
+    stage_blocks<T, N>(binv, s_b, k0, ne, tid);
     __syncthreads();
-    if (li < Np && e < ne) {
-      const T* B = binvT + (size_t)(k0 + e) * Np * Np;
-      T acc = 0;
-#pragma unroll
-      for (int j = 0; j < Np; ++j) acc += B[j * Np + li] * s_r[e * Np + j];
-      z[(size_t)(k0 + e) * Np + li] = acc;
-    }
+    if (li < Np && e < ne) z[(size_t)(k0 + e) * Np + li] = block_row<T, N>(s_b, s_r, e, li);
   }
 }
 
@@ -227,30 +222,30 @@ pupdate_kernel(T* __restrict__ p, const T* __restrict__ z, const CGScalars* __re
 // host side
 // ---------------------------------------------------------------------------
 template <int N>
-static int bj_apply_t(int64_t K, int dtype, const void* binvT, const void* r, void* z,
+static int bj_apply_t(int64_t K, int dtype, const void* binv, const void* r, void* z,
                       hipStream_t s) {
   const int ntiles = (int)((K + Shape<N>::EPB - 1) / Shape<N>::EPB);
   int grid = 8 * device_cu_count();
   grid = ntiles < grid ? ntiles : grid;
   if (dtype == DDGX_F64)
     hipLaunchKernelGGL((bj_apply_kernel<double, N>), dim3(grid), dim3(256), 0, s,
-                       (const double*)binvT, (const double*)r, (double*)z, (int)K);
+                       (const double*)binv, (const double*)r, (double*)z, (int)K);
   else
     hipLaunchKernelGGL((bj_apply_kernel<float, N>), dim3(grid), dim3(256), 0, s,
-                       (const float*)binvT, (const float*)r, (float*)z, (int)K);
+                       (const float*)binv, (const float*)r, (float*)z, (int)K);
   DDGX_HIP(hipGetLastError());
   return DDGX_OK;
 }
 
-int launch_bj_apply(int N, int64_t K, int dtype, const void* binvT, const void* r, void* z,
+int launch_bj_apply(int N, int64_t K, int dtype, const void* binv, const void* r, void* z,
                     hipStream_t s) {
   switch (N) {
-    case 1: return bj_apply_t<1>(K, dtype, binvT, r, z, s);
-    case 2: return bj_apply_t<2>(K, dtype, binvT, r, z, s);
-    case 3: return bj_apply_t<3>(K, dtype, binvT, r, z, s);
-    case 4: return bj_apply_t<4>(K, dtype, binvT, r, z, s);
-    case 5: return bj_apply_t<5>(K, dtype, binvT, r, z, s);
-    case 6: return bj_apply_t<6>(K, dtype, binvT, r, z, s);
+    case 1: return bj_apply_t<1>(K, dtype, binv, r, z, s);
+    case 2: return bj_apply_t<2>(K, dtype, binv, r, z, s);
+    case 3: return bj_apply_t<3>(K, dtype, binv, r, z, s);
+    case 4: return bj_apply_t<4>(K, dtype, binv, r, z, s);
+    case 5: return bj_apply_t<5>(K, dtype, binv, r, z, s);
+    case 6: return bj_apply_t<6>(K, dtype, binv, r, z, s);
   }
   set_error("polynomial order not built");
   return DDGX_ERR_UNSUPPORTED;
@@ -260,6 +255,7 @@ static size_t dsize(int dtype) { return dtype == DDGX_F64 ? 8 : 4; }
 
 void work_free(Work* w) {
   if (w == nullptr) return;
+  if (w->chunk) (void)hipGraphExecDestroy(w->chunk);
   void* ptrs[] = {w->r, w->p, w->z, w->Ap, w->partials, w->ir_b, w->ir_x, w->scal};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
@@ -330,24 +326,24 @@ static int finalize(Work& w, int nparts, int mode, hipStream_t s) {
 }
 
 template <typename T, int N>
-static int update_bj(const Op& op, Work& w, void* x, const void* binvT, hipStream_t s) {
+static int update_bj(const Op& op, Work& w, void* x, const void* binv, hipStream_t s) {
   hipLaunchKernelGGL((update_bj_kernel<T, N>), dim3(op.grid_vec), dim3(256), 0, s, (T*)x,
-                     (T*)w.r, (const T*)w.p, (const T*)w.Ap, (const T*)binvT, (T*)w.z,
+                     (T*)w.r, (const T*)w.p, (const T*)w.Ap, (const T*)binv, (T*)w.z,
                      (const CGScalars*)w.scal, (int)op.K, w.partials);
   DDGX_HIP(hipGetLastError());
   return DDGX_OK;
 }
 
 template <typename T>
-static int update_bj_dispatch(const Op& op, Work& w, void* x, const void* binvT,
+static int update_bj_dispatch(const Op& op, Work& w, void* x, const void* binv,
                               hipStream_t s) {
   switch (op.N) {
-    case 1: return update_bj<T, 1>(op, w, x, binvT, s);
-    case 2: return update_bj<T, 2>(op, w, x, binvT, s);
-    case 3: return update_bj<T, 3>(op, w, x, binvT, s);
-    case 4: return update_bj<T, 4>(op, w, x, binvT, s);
-    case 5: return update_bj<T, 5>(op, w, x, binvT, s);
-    case 6: return update_bj<T, 6>(op, w, x, binvT, s);
+    case 1: return update_bj<T, 1>(op, w, x, binv, s);
+    case 2: return update_bj<T, 2>(op, w, x, binv, s);
+    case 3: return update_bj<T, 3>(op, w, x, binv, s);
+    case 4: return update_bj<T, 4>(op, w, x, binv, s);
+    case 5: return update_bj<T, 5>(op, w, x, binv, s);
+    case 6: return update_bj<T, 6>(op, w, x, binv, s);
   }
   set_error("polynomial order not built");
   return DDGX_ERR_UNSUPPORTED;
@@ -363,6 +359,43 @@ static int check_work(const Op& op, const Work& w) {
   if (w.K != op.K || w.Np != op.Np || w.dtype != op.dtype) {
     set_error("workspace does not match the operator's vector shape / dtype");
     return DDGX_ERR_INVALID;
+  }
+  return DDGX_OK;
+}
+
+// Record `every` calls of `iteration` on stream s into a graph and make it
+// the context's chunk (updating the existing executable graph in place when
+// the topology is unchanged).
+template <typename F>
+static int capture_chunk(Work& w, int every, F&& iteration, hipStream_t s) {
+  DDGX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  int rc = DDGX_OK;
+  for (int c = 0; c < every && rc == DDGX_OK; ++c) rc = iteration();
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(s, &g);
+  if (rc != DDGX_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (ec != hipSuccess) return hip_fail(ec, "hipStreamEndCapture");
+  bool updated = false;
+  if (w.chunk) {
+    hipGraphNode_t bad = nullptr;
+    hipGraphExecUpdateResult res;
+    updated = hipGraphExecUpdate(w.chunk, g, &bad, &res) == hipSuccess &&
+              res == hipGraphExecUpdateSuccess;
+    if (!updated) {
+      (void)hipGetLastError();
+      (void)hipGraphExecDestroy(w.chunk);
+      w.chunk = nullptr;
+    }
+  }
+  hipError_t e = hipSuccess;
+  if (!updated) e = hipGraphInstantiate(&w.chunk, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    w.chunk = nullptr;
+    return hip_fail(e, "hipGraphInstantiate");
   }
   return DDGX_OK;
 }
@@ -392,7 +425,7 @@ static int pcg_t(const Op& op, Work& w, double alpha, double shift, const PC* pc
   hipLaunchKernelGGL((residual_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (const T*)b,
                      (const T*)w.Ap, (T*)w.r, n);
   DDGX_HIP(hipGetLastError());
-  if (bj) CHECK(launch_bj_apply(op.N, op.K, op.dtype, pc->binvT, w.r, w.z, s));
+  if (bj) CHECK(launch_bj_apply(op.N, op.K, op.dtype, pc->binv, w.r, w.z, s));
   else if (has_pc) CHECK(mg_vcycle(*pc->mg, *mgw, w.r, w.z, nullptr, s));
   hipLaunchKernelGGL((copy_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (const T*)z,
                      (T*)w.p, n);
@@ -400,32 +433,58 @@ static int pcg_t(const Op& op, Work& w, double alpha, double shift, const PC* pc
   CHECK(dot2<T>(op, w, w.r, z, w.r, w.r, nullptr, s));
   CHECK(finalize(w, op.grid_vec, F_INIT_GAMMA, s));
 
+  // one CG iteration; every kernel returns at once when the device stop
+  // flag (converged or k == maxiter) is set
+  auto iteration = [&]() -> int {
+    // Ap = A p, partial <p, Ap>
+    CHECK(launch_apply(op, w.p, w.Ap, nullptr, alpha, shift, M_DOT, w.partials, done, s));
+    CHECK(finalize(w, op.grid_apply, F_ALPHA, s));
+    if (bj) {
+      CHECK(update_bj_dispatch<T>(op, w, x, pc->binv, s));
+    } else {
+      hipLaunchKernelGGL((update_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (T*)x,
+                         (T*)w.r, (const T*)w.p, (const T*)w.Ap,
+                         (const CGScalars*)w.scal, n, has_pc ? nullptr : w.partials);
+      DDGX_HIP(hipGetLastError());
+      if (has_pc) {
+        CHECK(mg_vcycle(*pc->mg, *mgw, w.r, w.z, done, s));
+        CHECK(dot2<T>(op, w, w.r, w.z, w.r, w.r, done, s));
+      }
+    }
+    CHECK(finalize(w, op.grid_vec, F_BETA, s));
+    hipLaunchKernelGGL((pupdate_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (T*)w.p,
+                       (const T*)z, (const CGScalars*)w.scal, n);
+    DDGX_HIP(hipGetLastError());
+    return DDGX_OK;
+  };
   const int every = cfg.check_every > 0 ? cfg.check_every : 16;
+  // A solver context on a non-null stream replays a captured chunk of
+  // `every` iterations as one graph launch (captured again when the vectors,
+  // coefficients or preconditioner change).
+  const bool graph = w.use_graph && s != nullptr && cfg.maxiter > 0;
+  if (graph) {
+    const bool same = w.chunk && w.g_x == x && w.g_pc == (const void*)pc &&
+                      w.g_mgw == (const void*)mgw && w.g_alpha == alpha &&
+                      w.g_shift == shift && w.g_every == every && w.g_stream == s;
+    if (!same) CHECK(capture_chunk(w, every, iteration, s));
+    w.g_x = x;
+    w.g_pc = pc;
+    w.g_mgw = mgw;
+    w.g_alpha = alpha;
+    w.g_shift = shift;
+    w.g_every = every;
+    w.g_stream = s;
+  }
   int launched = 0;
   while (true) {
     DDGX_HIP(hipMemcpyAsync(w.scal_host, w.scal, sizeof(CGScalars), hipMemcpyDeviceToHost, s));
     DDGX_HIP(hipStreamSynchronize(s));
     if (w.scal_host->done || launched >= cfg.maxiter) break;
-    for (int c = 0; c < every && launched < cfg.maxiter; ++c, ++launched) {
-      // Ap = A p, partial <p, Ap>
-      CHECK(launch_apply(op, w.p, w.Ap, nullptr, alpha, shift, M_DOT, w.partials, done, s));
-      CHECK(finalize(w, op.grid_apply, F_ALPHA, s));
-      if (bj) {
-        CHECK(update_bj_dispatch<T>(op, w, x, pc->binvT, s));
-      } else {
-        hipLaunchKernelGGL((update_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (T*)x,
-                           (T*)w.r, (const T*)w.p, (const T*)w.Ap,
-                           (const CGScalars*)w.scal, n, has_pc ? nullptr : w.partials);
-        DDGX_HIP(hipGetLastError());
-        if (has_pc) {
-          CHECK(mg_vcycle(*pc->mg, *mgw, w.r, w.z, done, s));
-          CHECK(dot2<T>(op, w, w.r, w.z, w.r, w.r, done, s));
-        }
-      }
-      CHECK(finalize(w, op.grid_vec, F_BETA, s));
-      hipLaunchKernelGGL((pupdate_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (T*)w.p,
-                         (const T*)z, (const CGScalars*)w.scal, n);
-      DDGX_HIP(hipGetLastError());
+    if (graph) {
+      DDGX_HIP(hipGraphLaunch(w.chunk, s));
+      launched += every;
+    } else {
+      for (int c = 0; c < every && launched < cfg.maxiter; ++c, ++launched) CHECK(iteration());
     }
   }
   const CGScalars& h = *w.scal_host;
@@ -482,17 +541,23 @@ int ir_solve(const Op& op64, Work& w64, const Op& op32, Work& w32, double alpha,
   const double t = cfg.tol * cfg.tol * bb, a = cfg.atol * cfg.atol;
   const double atol2 = t > a ? t : a;
   ddgx_cg_cfg icfg{cfg.inner_tol, 0.0, cfg.inner_maxiter, cfg.check_every};
-  int outer = 0, inner = 0, stalled = 0;
+  int outer = 0, inner = 0, stalled = 0, floor_hit = 0;
   double rr = 0.0, rr_prev = 0.0;
   double* r = (double*)w64.r;
   while (true) {
     CHECK(launch_apply(op64, x, r, nullptr, alpha, shift, M_RESID, nullptr, nullptr, s, b));
     CHECK(dot(op64, w64, r, r, &rr, s));
     if (!(rr > atol2) || outer >= cfg.max_outer) break;
-    // the float64 residual has reached its round-off floor when two steps in
-    // a row fail to reduce it by half: further steps cannot meet the test
-    stalled = (outer > 0 && rr > 0.25 * rr_prev) ? stalled + 1 : 0;
-    if (stalled >= 2) break;
+    // A step reduces the float64 residual by about max(inner_tol, eps32 *
+    // cond(A)), not always monotonically; the residual has met its round-off
+    // floor when two steps in a row fail to halve it.
+    if (outer > 0) {
+      stalled = rr > 0.25 * rr_prev ? stalled + 1 : 0;
+      if (stalled >= 2) {
+        floor_hit = 1;
+        break;
+      }
+    }
     rr_prev = rr;
     const double nr = std::sqrt(rr);
     hipLaunchKernelGGL(scale_down_kernel, dim3(op64.grid_vec), dim3(256), 0, s, r,
@@ -513,6 +578,7 @@ int ir_solve(const Op& op64, Work& w64, const Op& op32, Work& w32, double alpha,
     info->residual2 = rr;
     info->atol2 = atol2;
     info->converged = rr <= atol2 ? 1 : 0;
+    info->at_floor = floor_hit;
   }
   return DDGX_OK;
 }

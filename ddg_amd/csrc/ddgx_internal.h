@@ -40,6 +40,39 @@ struct Shape {
   static constexpr int size = oLS + 3 * Nfp * RS;
 };
 
+// Block-Jacobi inverse blocks are symmetric and stored packed: entry (i, j),
+// i <= j, of element k at binv[k * NPK + pk(i, j)], rows of the upper
+// triangle one after another.
+template <int N>
+struct SymPack {
+  static constexpr int Np = Shape<N>::Np;
+  static constexpr int NPK = Np * (Np + 1) / 2;
+  __host__ __device__ static constexpr int pk(int i, int j) {
+    return i <= j ? i * Np - i * (i - 1) / 2 + (j - i) : j * Np - j * (j - 1) / 2 + (i - j);
+  }
+};
+
+// Inverse blocks of the EPB elements of a tile into LDS (contiguous loads).
+template <typename T, int N>
+__device__ __forceinline__ void stage_blocks(const T* __restrict__ binv, T* s_b, int k0, int ne,
+                                             int tid) {
+  constexpr int NPK = SymPack<N>::NPK;
+  const T* src = binv + (size_t)k0 * NPK;
+  for (int i = tid; i < ne * NPK; i += 256) s_b[i] = src[i];
+}
+
+// Row li of B_e r_e from the staged packed block and the staged vector.
+template <typename T, int N>
+__device__ __forceinline__ T block_row(const T* s_b, const T* s_r, int e, int li) {
+  constexpr int Np = Shape<N>::Np, NPK = SymPack<N>::NPK;
+  const T* b = s_b + e * NPK;
+  const T* v = s_r + e * Np;
+  T acc = 0;
+#pragma unroll
+  for (int j = 0; j < Np; ++j) acc += b[SymPack<N>::pk(li, j)] * v[j];
+  return acc;
+}
+
 // Node index of face node p on face f.  Nodes are ordered in rows of
 // constant s (row a has N+1-a nodes, starting at a(N+1) - a(a-1)/2):
 // face 0 = row 0, face 1 = last node of each row, face 2 = first node of
@@ -139,6 +172,15 @@ struct Work {
   CGScalars* scal = nullptr;
   CGScalars* scal_host = nullptr;  // pinned
   size_t bytes = 0;            // device bytes held
+  // Captured chunk of CG iterations (HIP graph) and what it was captured for
+  bool use_graph = false;
+  hipGraphExec_t chunk = nullptr;
+  const void* g_x = nullptr;
+  const void* g_pc = nullptr;
+  const void* g_mgw = nullptr;
+  double g_alpha = 0.0, g_shift = 0.0;
+  int g_every = 0;
+  hipStream_t g_stream = nullptr;
 };
 int work_create(const Op& op, bool refine, Work** out);
 void work_free(Work* w);
@@ -148,8 +190,8 @@ struct MG;
 struct PC {
   int type = 0;              // 0 block-Jacobi, 1 multigrid V-cycle
   Op* op = nullptr;          // BJ: the float64 operator the blocks came from
-  int dtype = 0;             // precision of binvT and of the vectors
-  void* binvT = nullptr;     // K x Np x Np, transposed inverse blocks (dtype)
+  int dtype = 0;             // precision of binv and of the vectors
+  void* binv = nullptr;      // K x Np(Np+1)/2 packed symmetric inverse blocks (dtype)
   double* blocks = nullptr;  // K x Np x Np float64 blocks (kept for inspection)
   MG* mg = nullptr;          // type 1
   int64_t K = 0;             // vector shape the PC applies to
@@ -160,7 +202,7 @@ struct PC {
 // solved directly with a dense float64 inverse.
 struct MGLevel {
   Op* op = nullptr;        // operator at this level (MG dtype)
-  void* binvT = nullptr;   // block-Jacobi inverse blocks of this level (MG dtype)
+  void* binv = nullptr;    // packed block-Jacobi inverse blocks of this level (MG dtype)
   double alpha = 0.0;      // Helmholtz coefficient of this level's operator
   int tkind = 0;           // transfer to the next-coarser level: 0 = p, 1 = h
   int np_lo = 0;           // Np of the next-coarser level (p transfer)
@@ -211,10 +253,11 @@ void mg_free(MG* mg);
 int launch_rhs(const Op& op, const void* f, const void* udir, const void* q, void* rhs,
                hipStream_t s);
 int launch_block_probe(const Op& op, double* blocks, double alpha, hipStream_t s);
-int launch_invert_blocks(int N, int64_t K, int dtype, const double* blocks, void* binvT,
+int launch_invert_blocks(int N, int64_t K, int dtype, const double* blocks, void* binv,
                          hipStream_t s);
-int launch_bj_apply(int N, int64_t K, int dtype, const void* binvT, const void* r, void* z,
+int launch_bj_apply(int N, int64_t K, int dtype, const void* binv, const void* r, void* z,
                     hipStream_t s);
+size_t packed_block_words(int N);
 
 int pcg_solve(const Op& op, Work& w, double alpha, double shift, const PC* pc, MGWork* mgw,
               const void* b, void* x, const ddgx_cg_cfg& cfg, ddgx_cg_info* info,

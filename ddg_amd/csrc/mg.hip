@@ -39,12 +39,13 @@ constexpr int MG_MAX_COARSE = 8192;
 // x = (accumulate ? x : 0) + omega * Binv r, element by element
 template <typename T, int N>
 __global__ void __launch_bounds__(256)
-bj_smooth_kernel(const T* __restrict__ binvT, const T* __restrict__ r, T* __restrict__ x, int K,
+bj_smooth_kernel(const T* __restrict__ binv, const T* __restrict__ r, T* __restrict__ x, int K,
                  T omega, int accumulate, const int* __restrict__ skip) {
   using S = Shape<N>;
   constexpr int Np = S::Np, NL = S::NL, EPB = S::EPB;
   if (skip != nullptr && *skip) return;
   __shared__ T s_r[EPB * Np];
+  __shared__ T s_b[EPB * SymPack<N>::NPK];
   const int tid = threadIdx.x, e = tid / NL, li = tid % NL;
   const int ntiles = (K + EPB - 1) / EPB;
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -52,12 +53,10 @@ bj_smooth_kernel(const T* __restrict__ binvT, const T* __restrict__ r, T* __rest
     const int ne = min(EPB, K - k0);
     __syncthreads();
     for (int i = tid; i < ne * Np; i += 256) s_r[i] = r[(size_t)k0 * Np + i];
+    stage_blocks<T, N>(binv, s_b, k0, ne, tid);
     __syncthreads();
     if (li < Np && e < ne) {
-      const T* B = binvT + (size_t)(k0 + e) * Np * Np;
-      T acc = 0;
-#pragma unroll
-      for (int j = 0; j < Np; ++j) acc += B[j * Np + li] * s_r[e * Np + j];
+      const T acc = block_row<T, N>(s_b, s_r, e, li);
       const size_t g = (size_t)(k0 + e) * Np + li;
       x[g] = accumulate ? x[g] + omega * acc : omega * acc;
     }
@@ -229,7 +228,7 @@ static int bj_smooth_t(const MGLevel& L, const void* r, void* x, double omega, i
   int grid = 8 * device_cu_count();
   grid = ntiles < grid ? ntiles : grid;
   hipLaunchKernelGGL((bj_smooth_kernel<T, N>), dim3(grid), dim3(256), 0, s,
-                     (const T*)L.binvT, (const T*)r, (T*)x, K, (T)omega, acc, skip);
+                     (const T*)L.binv, (const T*)r, (T*)x, K, (T)omega, acc, skip);
   DDGX_HIP(hipGetLastError());
   return DDGX_OK;
 }
@@ -389,7 +388,7 @@ void mg_free(MG* mg) {
   if (mg == nullptr) return;
   for (int l = 0; l < mg->nlev; ++l) {
     MGLevel& L = mg->lv[l];
-    if (L.binvT) (void)hipFree(L.binvT);
+    if (L.binv) (void)hipFree(L.binv);
     if (L.P) (void)hipFree(L.P);
   }
   delete[] mg->lv;
@@ -518,11 +517,11 @@ int mg_create(int nlev, Op* const* ops, Op* const* ops64, const int* tkind,
       L.op = ops[l];
       L.alpha = cfg.alpha;
       if (l + 1 == nlev) break;
-      const size_t nb = (size_t)ops[l]->K * ops[l]->Np * ops[l]->Np;
-      hipError_t he = hipMalloc(&L.binvT, nb * dsz(dtype));
+      const size_t nb = (size_t)ops[l]->K * packed_block_words(ops[l]->N);
+      hipError_t he = hipMalloc(&L.binv, nb * dsz(dtype));
       if (he != hipSuccess) { rc = hip_fail(he, "hipMalloc(level)"); break; }
       rc = launch_block_probe(*ops64[l], blocks, cfg.alpha, s);
-      if (rc == DDGX_OK) rc = launch_invert_blocks(ops[l]->N, ops[l]->K, dtype, blocks, L.binvT, s);
+      if (rc == DDGX_OK) rc = launch_invert_blocks(ops[l]->N, ops[l]->K, dtype, blocks, L.binv, s);
       if (rc != DDGX_OK) break;
       he = hipStreamSynchronize(s);  // blocks is reused by the next level
       if (he != hipSuccess) { rc = hip_fail(he, "block-Jacobi build"); break; }

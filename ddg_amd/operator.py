@@ -307,8 +307,8 @@ def refine_solve(op64: SipgOperator, op32: SipgOperator, b, x=None, *, alpha=0.0
                                         pch, _ptr(b), _ptr(x), C.byref(cfg), C.byref(info),
                                         _stream()), "ddgx_ir_solve")
     return x, {"outer": info.outer, "iterations": info.inner_total,
-               "converged": bool(info.converged), "residual2": info.residual2,
-               "atol2": info.atol2}
+               "converged": bool(info.converged), "at_floor": bool(info.at_floor),
+               "residual2": info.residual2, "atol2": info.atol2}
 
 
 class DevicePreconditioner:

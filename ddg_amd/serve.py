@@ -62,9 +62,10 @@ class Result:
     end: float          # solution ready on the host side
     outer: int
     iterations: int
-    converged: bool
+    converged: bool     # float64 residual met the tolerance
     residual2: float
     atol2: float
+    at_floor: bool = False  # stopped at the float64 residual's round-off floor
     x: object = None    # device solution (kept only when asked for)
 
     @property
@@ -263,25 +264,39 @@ class ProblemClass:
 
 
 class SolverContext:
-    """A ``ddgx_solver`` refinement context (float64 op, float32 twin,
-    float32 preconditioner)."""
+    """A ``ddgx_solver`` context: private Krylov state for solves with one
+    operator and preconditioner.  ``op32`` given: a refinement context
+    (float64 ``op``, float32 twin, float32 ``pc``), used by :meth:`solve`;
+    else a PCG context on ``op`` (:meth:`pcg`).  On a non-default stream the
+    context replays captured chunks of CG iterations as HIP graphs."""
 
-    def __init__(self, op64, op32, pc32):
+    def __init__(self, op, op32=None, pc=None):
         L = _lib.lib()
         h = C.c_void_p()
-        _lib.check(L.ddgx_solver_create(op64.handle, op32.handle, pc32.handle, C.byref(h)),
+        _lib.check(L.ddgx_solver_create(op.handle, None if op32 is None else op32.handle,
+                                        None if pc is None else pc.handle, C.byref(h)),
                    "ddgx_solver_create")
         self._h = h
         nb = C.c_int64()
         _lib.check(L.ddgx_solver_bytes(h, C.byref(nb)), "ddgx_solver_bytes")
         self.nbytes = int(nb.value)
-        self._keep = (op64, op32, pc32)
+        self._keep = (op, op32, pc)
 
     def solve(self, b, x, alpha, cfg, stream):
+        """Mixed-precision refinement (``ddgx_solver_ir``); returns IRInfo."""
         info = _lib.IRInfo()
         _lib.check(_lib.lib().ddgx_solver_ir(self._h, float(alpha), 0.0, C.c_void_p(b.data_ptr()),
                                              C.c_void_p(x.data_ptr()), C.byref(cfg),
                                              C.byref(info), C.c_void_p(stream)), "ddgx_solver_ir")
+        return info
+
+    def pcg(self, b, x, alpha, cfg, stream, shift=0.0):
+        """Preconditioned CG (``ddgx_solver_pcg``); returns CGInfo."""
+        info = _lib.CGInfo()
+        _lib.check(_lib.lib().ddgx_solver_pcg(self._h, float(alpha), float(shift),
+                                              C.c_void_p(b.data_ptr()), C.c_void_p(x.data_ptr()),
+                                              C.byref(cfg), C.byref(info), C.c_void_p(stream)),
+                   "ddgx_solver_pcg")
         return info
 
     def close(self):
@@ -315,6 +330,7 @@ class ServeConfig:
 class ServeReport:
     results: list = field(default_factory=list)
     wall: float = 0.0
+    tol: float = 1e-10
     peak_slots: int = 0
     peak_bytes: int = 0
 
@@ -327,7 +343,11 @@ class ServeReport:
                 "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
                 "service_p50_ms": round(float(np.percentile(svc, 50)) * 1e3, 3),
                 "service_p99_ms": round(float(np.percentile(svc, 99)) * 1e3, 3),
-                "all_converged": bool(all(r.converged for r in self.results)),
+                "converged": int(sum(r.converged for r in self.results)),
+                "at_residual_floor": int(sum(r.at_floor and not r.converged for r in self.results)),
+                "all_solved": bool(all(r.converged or r.at_floor for r in self.results)),
+                "max_rel_residual": float(max(np.sqrt(r.residual2 / r.atol2)
+                                              for r in self.results)) * self.tol,
                 "peak_slots_busy": self.peak_slots,
                 "peak_request_bytes": self.peak_bytes}
 
@@ -384,7 +404,7 @@ class SolveServer:
         torch.cuda.synchronize()
         cfg = self.cfg
         adm = Admission(cfg.slots, cfg.hbm_budget)
-        report = ServeReport()
+        report = ServeReport(tol=cfg.tol)
         work = queue.Queue()
         cond = threading.Condition()
         errors = []
@@ -406,7 +426,8 @@ class SolveServer:
                     x, info = self.solve_one(req, self.streams[slot])
                     res = Result(req.rid, req.arrival, t_admit, clock(), info.outer,
                                  info.inner_total, bool(info.converged), info.residual2,
-                                 info.atol2, x if cfg.keep_solutions else None)
+                                 info.atol2, bool(info.at_floor),
+                                 x if cfg.keep_solutions else None)
                 except Exception as e:  # surfaced after the loop
                     errors.append(e)
                 with cond:

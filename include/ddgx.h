@@ -168,8 +168,8 @@ int ddgx_sipg_param_grad(ddgx_op op, const double* u, const double* lam,
  * ddgx_pcg_solve):
  *   r = b - A x (float64); stop when <r,r> <= max(tol^2 <b,b>, atol^2)
  *   d = PCG_float32(A, r/|r|, pc32) to inner_tol;  x += |r| d (float64)
- * The loop also ends, unconverged, when two steps in a row fail to halve
- * ||r|| (the float64 residual is at its round-off floor).
+ * The loop also ends, unconverged, at the float64 residual's round-off
+ * floor: when two steps in a row fail to halve ||r|| (info->at_floor = 1).
  * op64/op32: the same operator in float64 and float32; pc32: float32
  * preconditioner (block-Jacobi or multigrid) or NULL.  b, x: float64. */
 typedef struct {
@@ -186,6 +186,7 @@ typedef struct {
   int converged;      /* outer stopping rule met                              */
   double residual2;   /* final <r,r> of the float64 residual                  */
   double atol2;
+  int at_floor;       /* ended at the float64 residual's round-off floor      */
 } ddgx_ir_info;
 
 int ddgx_ir_solve(ddgx_op op64, ddgx_op op32, double alpha, double shift, ddgx_pc pc32,

@@ -1,0 +1,74 @@
+"""Solver contexts on a side stream replay captured chunks of CG iterations
+(HIP graphs).  Their results must equal the eager launches of the default
+context bit for bit, on every replay, with eager work and synchronisations
+in between (stated tolerance: 0)."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import unit_square
+from ddg_amd import _lib
+from ddg_amd import mesh as pmesh
+from ddg_amd import multigrid as MG
+from ddg_amd import sipg as D
+from ddg_amd.operator import refine_solve
+from ddg_amd.serve import SolverContext
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def problem(dev):
+    pm, _ = unit_square(12, 12, 3, jitter=0.1, seed=41)
+    bc = pmesh.dirichlet_x_neumann_y(pm)
+    op64 = D.get_operator(pm, bc)
+    op32 = D.get_operator(pm, bc, dtype=torch.float32)
+    mg64 = MG.make_sipg_mg_preconditioner(pm, bc_types=bc)
+    mg32 = MG.make_sipg_mg_preconditioner(pm, bc_types=bc, dtype=torch.float32)
+    b = torch.as_tensor(np.random.default_rng(42).standard_normal(pm.K * pm.Np), device=dev)
+    return pm, op64, op32, mg64, mg32, b
+
+
+@pytest.mark.parametrize("alpha", [0.0, 7.5])
+def test_graph_pcg_equals_eager(problem, alpha):
+    pm, op64, op32, mg64, mg32, b = problem
+    _say(f"graph pcg alpha={alpha}")
+    x_e, info_e = op64.pcg(b, alpha=alpha, pc=mg64, tol=1e-11, maxiter=500, check_every=8)
+    ctx = SolverContext(op64, pc=mg64)
+    s = torch.cuda.Stream()
+    cfg = _lib.CGConfig(1e-11, 0.0, 500, 8)
+    for rep in range(3):
+        x = torch.zeros_like(b)
+        with torch.cuda.stream(s):
+            info = ctx.pcg(b, x, alpha, cfg, s.cuda_stream)
+        s.synchronize()
+        assert info.iterations == info_e["iterations"] and bool(info.converged)
+        assert torch.equal(x, x_e), f"replay {rep} differs from the eager solve"
+        op64.apply(x)              # eager work between replays
+        torch.cuda.synchronize()
+
+
+def test_graph_refinement_equals_eager(problem):
+    pm, op64, op32, mg64, mg32, b = problem
+    _say("graph refinement")
+    x_e, info_e = refine_solve(op64, op32, b, pc32=mg32, tol=1e-10, inner_tol=1e-4,
+                               inner_maxiter=500, check_every=8)
+    ctx = SolverContext(op64, op32, mg32)
+    s = torch.cuda.Stream()
+    cfg = _lib.IRConfig(1e-10, 0.0, 20, 1e-4, 500, 8)
+    for alpha in (0.0, 0.0, 3.0, 0.0):   # a coefficient change forces a new capture
+        x = torch.zeros_like(b)
+        with torch.cuda.stream(s):
+            info = ctx.solve(b, x, alpha, cfg, s.cuda_stream)
+        s.synchronize()
+        if alpha == 0.0:
+            assert info.inner_total == info_e["iterations"] and info.outer == info_e["outer"]
+            assert torch.equal(x, x_e)
+        else:
+            x_a, _ = refine_solve(op64, op32, b, alpha=alpha, pc32=mg32, tol=1e-10,
+                                  inner_tol=1e-4, inner_maxiter=500, check_every=8)
+            assert torch.equal(x, x_a)

@@ -34,7 +34,7 @@ def test_alone_equals_mix(server):
     for r in reqs:
         x, info = server.solve_one(r, server.streams[0])
         torch.cuda.synchronize()
-        assert info.converged, (r, info.residual2, info.atol2)
+        assert info.converged or info.at_floor, (r, info.residual2, info.atol2)
         alone[r.rid] = x.clone()
     _say("serve: mix")
     poisson_arrivals(reqs, None)
@@ -42,7 +42,7 @@ def test_alone_equals_mix(server):
     assert rep.peak_slots > 1, "requests never overlapped"
     assert len(rep.results) == len(reqs)
     for res in rep.results:
-        assert res.converged
+        assert res.converged or res.at_floor
         assert torch.equal(res.x, alone[res.rid]), f"request {res.rid} differs in the mix"
 
 
@@ -60,7 +60,7 @@ def test_results_match_oracle(server):
         A = S.sipg_matrix_2d(mo, bc_types=pc.bc, alpha=r.k2)
         xo = np.linalg.solve(A, b)
         err = np.linalg.norm(x.cpu().numpy() - xo) / np.linalg.norm(xo)
-        assert info.converged and err < 1e-9, (err, info.outer)
+        assert (info.converged or info.at_floor) and err < 1e-9, (err, info.outer)
 
 
 def test_poisson_stream_within_limits(server):
@@ -77,7 +77,7 @@ def test_poisson_stream_within_limits(server):
         server.cfg.keep_solutions = True
     st = rep.stats()
     _say("serve stream:", st)
-    assert st["requests"] == 40 and st["all_converged"]
+    assert st["requests"] == 40 and st["all_solved"]
     assert rep.peak_bytes <= 2 * fp and rep.peak_slots <= 4
     for res in rep.results:
         assert res.start >= res.arrival and res.end >= res.start

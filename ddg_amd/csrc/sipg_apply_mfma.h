@@ -116,22 +116,19 @@ __device__ __forceinline__ void grad_and_traces(const T (&A1)[2 * MS<N>::NB][MS<
         US[b - NB][r] = acc[r];
     }
   }
+  // face node (f, p) is node n = fmask(f, p), held by lane group gn in
+  // register (b, r): all three are constants after unrolling, so every face
+  // node is one write predicated on the lane group
 #pragma unroll
-  for (int b = 0; b < NB; ++b)
+  for (int f = 0; f < 3; ++f)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int n = 16 * b + MF::crow(g, r);
-      if (n >= Np) continue;
-#pragma unroll
-      for (int f = 0; f < 3; ++f) {
-        int p = -1;
-#pragma unroll
-        for (int q = 0; q < Nfp; ++q)
-          if (fmask<N>(f, q) == n) p = q;
-        if (p >= 0) {
-          tr[(f * Nfp + p) * 2] = U[b][r];
-          tr[(f * Nfp + p) * 2 + 1] = ab.a(f) * UR[b][r] + ab.b(f) * US[b][r];
-        }
+    for (int p = 0; p < Nfp; ++p) {
+      const int n = fmask<N>(f, p), b = n / 16, m = n % 16;
+      const int gn = MF::crow(1, 0) == 1 ? m % 4 : m / 4;  // inverse of crow
+      const int r = MF::crow(1, 0) == 1 ? m / 4 : m % 4;
+      if (g == gn) {
+        tr[(f * Nfp + p) * 2] = U[b][r];
+        tr[(f * Nfp + p) * 2 + 1] = ab.a(f) * UR[b][r] + ab.b(f) * US[b][r];
       }
     }
 }

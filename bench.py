@@ -96,8 +96,8 @@ def hpmg_for(m, N, dtype, n_base=16, degrees=None):
     if degrees is None:
         degrees = [N, N // 2, 1] if N >= 4 else list(range(N, 0, -1))
     mg, fin = MG.make_sipg_hpmg_preconditioner(base, lv - lb, bc_fn=pm.dirichlet_x_neumann_y,
-                                               degrees=degrees, fine_vertices=(m.VX, m.VY),
-                                               dtype=dtype)
+                                               degrees=degrees, p_coarse=degrees[-1],
+                                               fine_vertices=(m.VX, m.VY), dtype=dtype)
     assert np.array_equal(np.asarray(fin.EToV), np.asarray(m.EToV))
     return mg, fin
 
@@ -298,6 +298,9 @@ def main():
                     help="time a device-to-device copy at several sizes and exit")
     ap.add_argument("--profile-apply", choices=["f64", "f32"], default=None,
                     help="only run --steps eager applies (for rocprofv3 counter runs)")
+    ap.add_argument("--mg-degrees", type=int, nargs="*", default=None,
+                    help="p-levels of the config-1 hp-multigrid (default 4 2 1); the last is the "
+                         "degree of the h-levels")
     ap.add_argument("--profile-solve", action="store_true",
                     help="only run a few config-1 solves (FP64 MG-PCG and mixed) for rocprofv3")
     ap.add_argument("--profile-copy", type=int, default=0, metavar="MB",
@@ -367,8 +370,8 @@ def main():
     if args.profile_solve:
         from ddg_amd import sipg as D
         from ddg_amd.operator import refine_solve
-        mg64, fin = hpmg_for(m, 4, torch.float64)
-        mg32, _ = hpmg_for(fin, 4, torch.float32)
+        mg64, fin = hpmg_for(m, 4, torch.float64, degrees=args.mg_degrees)
+        mg32, _ = hpmg_for(fin, 4, torch.float32, degrees=args.mg_degrees)
         f, ud, q = problem_data(fin)
         op64 = D.get_operator(fin, bc)
         op32 = D.get_operator(fin, bc, dtype=torch.float32)

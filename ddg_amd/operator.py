@@ -167,7 +167,11 @@ class SipgOperator:
     device in the element-major flat layout (``(K*Np,)`` or ``(K, Np)``).
     """
 
-    def __init__(self, mesh, bc_types, tau_scale=200.0, kappa=None, dtype=torch.float64):
+    def __init__(self, mesh, bc_types, tau_scale=200.0, kappa=None, dtype=torch.float64,
+                 order="auto"):
+        """``order``: element processing order of the apply's 64-element tiles:
+        ``"auto"`` (``tile_order``), ``None`` (the mesh's own numbering) or a
+        permutation of ``range(K)``."""
         if not torch.cuda.is_available():
             raise RuntimeError("SipgOperator needs a HIP device (no CPU fallback)")
         self.N, self.K, self.Np, self.Nfp = int(mesh.N), int(mesh.K), int(mesh.Np), int(mesh.Nfp)
@@ -181,7 +185,14 @@ class SipgOperator:
                               mfma_fragments(ref, "f64" if dtype == torch.float64 else "f32")])
         L = _lib.lib()
         assert tab.size == L.ddgx_table_size(self.N)
-        order = tile_order(mesh)
+        if isinstance(order, str):
+            if order != "auto":
+                raise ValueError("order must be 'auto', None or a permutation of range(K)")
+            order = tile_order(mesh)
+        elif order is not None:
+            order = np.ascontiguousarray(np.asarray(order, dtype=np.int32).reshape(-1))
+            if order.size != self.K:
+                raise ValueError("order must hold K entries")
         self.order = order
         h = C.c_void_p()
         torch.cuda.current_device()  # make sure the HIP context exists

@@ -117,3 +117,34 @@ def test_repeatable(dev):
     u = torch.randn(pm.K * pm.Np, dtype=torch.float64, device=dev)
     _say("apply repeatable")
     assert torch.equal(op.apply(u), op.apply(u))
+
+
+@pytest.mark.parametrize("N", [2, 4])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_apply_wide_halo_tiles(dev, N, dtype):
+    """Row-major 24x24 mesh in its own numbering: 64-element tiles with up to
+    50 out-of-tile neighbours (the 64-entry halo variant of the kernel);
+    also the Morton order of the same mesh (32-entry variant, permuted
+    element order).  Both against the oracle."""
+    pm, om = unit_square(24, 24, N, jitter=0.1, seed=7)
+    bc = pmesh.dirichlet_x_neumann_y(pm)
+    rng = np.random.default_rng(70 + N)
+    u = rng.standard_normal((pm.Np, pm.K))
+    ud = rng.standard_normal((pm.Np, pm.K))
+    ref = S.sipg_apply_2d(om, u, bc_types=bc, u_dir=ud) + 2.5 * S.mass_apply(om, u)
+    tol = 1e-12 if dtype == torch.float64 else 2e-5
+    for order in (None, "auto"):
+        _say(f"apply wide halo N={N} {dtype} order={order}")
+        op = SipgOperator(pm, bc, dtype=dtype, order=order)
+        y = _field(op.apply(_flat(u, dev, dtype), u_dir=_flat(ud, dev, dtype), alpha=2.5),
+                   pm.Np, pm.K)
+        assert rel(y, ref) < tol
+
+
+def test_too_scattered_order_rejected(dev):
+    """An element order whose tiles have more than 64 out-of-tile
+    neighbours is refused on the host (before any device work)."""
+    pm, _ = unit_square(24, 24, 1)
+    order = np.random.default_rng(3).permutation(pm.K)
+    with pytest.raises(ValueError):
+        SipgOperator(pm, pmesh.all_dirichlet(pm), order=order)

@@ -79,7 +79,7 @@ static int upload_as(int dtype, const double* src, size_t n, void** dst) {
 }
 
 static void free_op(Op& o) {
-  void* ptrs[] = {o.tab, o.rec, o.nbr, o.fcode, o.perm, o.hpos, o.hab, o.fsj};
+  void* ptrs[] = {o.tab, o.rec, o.nbr, o.nbrp, o.fcode, o.perm, o.hpos, o.hab, o.fsj};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   work_free(o.ws);
@@ -230,17 +230,22 @@ int ddgx_op_create(int N, int64_t K, int dtype, const double* tables, const doub
     }
     // records, face codes and Neumann Jacobians in tile order (SoA records)
     std::vector<double> recT((size_t)K * DDGX_REC_WORDS), fsjT((size_t)K * 4);
-    std::vector<int> nbrT((size_t)K * 3);
+    std::vector<int> nbrT((size_t)K * 3), nbrP((size_t)K * 3);
     for (int64_t i = 0; i < K; ++i) {
       const int64_t k = perm[i];
       for (int w = 0; w < DDGX_REC_WORDS; ++w) recT[(size_t)w * K + i] = rec[k * DDGX_REC_WORDS + w];
-      for (int f = 0; f < 3; ++f) nbrT[(size_t)f * K + i] = nbr[k * 4 + f];
+      for (int f = 0; f < 3; ++f) {
+        const int c = nbr[k * 4 + f];
+        nbrT[(size_t)f * K + i] = c;
+        nbrP[(size_t)f * K + i] = c >= 0 ? (inv[nbr_elem(c)] << 3) | (c & 7) : c;
+      }
       for (int f = 0; f < 4; ++f) fsjT[(size_t)i * 4 + f] = fsj[k * 4 + f];
     }
     if ((rc = upload_as(dtype, recT.data(), recT.size(), &o.rec)) != DDGX_OK) break;
     if ((rc = upload_as(dtype, fsjT.data(), fsjT.size(), &o.fsj)) != DDGX_OK) break;
     if ((rc = upload_as(dtype, hab.data(), hab.size(), &o.hab)) != DDGX_OK) break;
-    if (!put(&o.nbr, nbrT, "nbr") || !put(&o.fcode, fcode, "fcode") || !put(&o.perm, perm, "perm") ||
+    if (!put(&o.nbr, nbrT, "nbr") || !put(&o.nbrp, nbrP, "nbrp") || !put(&o.fcode, fcode, "fcode") ||
+        !put(&o.perm, perm, "perm") ||
         !put(&o.hpos, hpos, "hpos"))
       break;
     o.grid_apply = (int)ntiles;

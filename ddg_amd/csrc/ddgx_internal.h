@@ -142,6 +142,7 @@ struct Op {
   void* frag = nullptr;  // = tab + Shape<N>::size (MFMA A fragments)
   void* rec = nullptr;   // 16 x K records, tile order (dtype)
   int* nbr = nullptr;    // 3 x K face codes (see above), tile order
+  int* nbrp = nullptr;   // the same codes with the neighbour's tile position in place of its id
   int* fcode = nullptr;  // 3 x K packed apply codes (fc_* below), tile order
   int* perm = nullptr;   // K: element id at each tile position
   bool perm_id = true;   // perm is the identity (the apply then skips it)

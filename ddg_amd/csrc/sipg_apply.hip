@@ -12,12 +12,14 @@
 // [u] = u^- - u^+, dn^- = a_f ur + b_f us at the own face nodes, dn^+ the
 // neighbour's normal derivative taken with the own normal.
 //
-// The kernel itself is in sipg_apply_mfma.h (matrix cores, 64-element tiles,
-// halo traces through LDS); this file holds the launch code and the
-// right-hand-side kernel.
+// The kernels are in sipg_apply_mfma.h (matrix cores, 64-element tiles,
+// halo traces through LDS) and sipg_apply_valu.h (vector ALUs, one thread
+// per element: order 1, and order 2 in float32); this file holds the launch
+// code and the right-hand-side kernel.
 
 #include "ddgx_internal.h"
 #include "sipg_apply_mfma.h"
+#include "sipg_apply_valu.h"
 
 namespace ddgx {
 
@@ -27,7 +29,12 @@ static int launch_t(const Op& op, const void* u, void* y, const void* udir, doub
                     const void* bvec) {
   // one workgroup per 64-element tile; in dot mode each writes one partial
   const int* perm = op.perm_id ? nullptr : op.perm;
-  if (op.halo_cap == HALO_SMALL)
+  if constexpr (UseValuApply<T, N>::value)
+    hipLaunchKernelGGL((sipg_apply_valu_kernel<T, N, MODE>), dim3(op.grid_apply), dim3(TILE), 0,
+                       s, (const T*)op.tab, (const T*)op.rec, op.nbrp, perm, (const T*)u, (T*)y,
+                       (const T*)udir, (T)alpha, (T)shift, (int)op.K, partial, skip,
+                       (const T*)bvec);
+  else if (op.halo_cap == HALO_SMALL)
     hipLaunchKernelGGL((sipg_apply_mfma_kernel<T, N, MODE, HALO_SMALL>), dim3(op.grid_apply),
                        dim3(256), 0, s, (const T*)op.frag, (const T*)op.rec, op.fcode, perm,
                        op.hpos, (const T*)op.hab, (const T*)u, (T*)y, (const T*)udir, (T)alpha,

@@ -1,0 +1,252 @@
+// SIPG apply for the low orders on the vector ALUs, one thread per element:
+//   y = (A_sipg + alpha*J*M + shift*I) u          (or y = b - that, residual mode)
+// with the same closed form as the matrix-core kernel (src/ddg/dg2d/sipg.py
+// :142-337):
+//
+//   volume : y_k = ArT (vrr ur + vrs us) + AsT (vrs ur + vss us)
+//   face f : P = pen_f [u] - sJw_f (dn^- + dn^+)    -> y[Fmask_f] += m1d_f^T P
+//            X = -sJw_f [u]                          -> y += (a_f LR_f + b_f LS_f) X
+//
+// At N = 1, 2 a nodal vector has 3 or 6 entries, so the 16-row matrix-core
+// tiles would be mostly padding; here every element's contractions are
+// unrolled scalar FMAs with the tables read through the scalar cache
+// (uniform addresses).  A workgroup is one wave and one 64-element tile
+// (same tiles, XCD mapping and dot partials as the matrix-core kernel):
+// the tile's nodal values are staged through LDS with contiguous loads and
+// stores, a neighbour inside the tile is read from LDS, one outside it from
+// global memory (cache hits mostly: tiles are compact patches of the mesh).
+
+#pragma once
+
+#include "ddgx_internal.h"
+
+namespace ddgx {
+
+// Orders the apply runs on the vector ALUs instead of the matrix cores (the
+// faster of the two kernels at 2x256x256 on MI355X: N=1 10 us against about
+// 20 us in float64; N=2 11.5 against 12.6 us in float32, while float64 N=2
+// keeps the matrix cores, 21 against 25 us)
+template <typename T, int N>
+struct UseValuApply {
+  static constexpr bool value = N == 1 || (N == 2 && sizeof(T) == 4);
+};
+
+template <typename T, int N, int MODE>
+__global__ void __launch_bounds__(64)
+sipg_apply_valu_kernel(const T* __restrict__ tab, const T* __restrict__ rec,
+                       const int* __restrict__ nbrp, const int* __restrict__ perm,
+                       const T* __restrict__ u, T* __restrict__ y, const T* __restrict__ udir,
+                       T alpha, T shift, int K, double* __restrict__ partial,
+                       const int* __restrict__ skip, const T* __restrict__ bvec) {
+  using S = Shape<N>;
+  constexpr int Np = S::Np, Nfp = S::Nfp, RS = S::RS, RSF = S::RSF;
+  constexpr bool HELM = MODE & M_HELM, UDIR = MODE & M_UDIR, DOT = MODE & M_DOT;
+  constexpr bool RESID = MODE & M_RESID;
+  const T* __restrict__ Dr = tab + S::oDr;
+  const T* __restrict__ Ds = tab + S::oDs;
+  const T* __restrict__ Ar = tab + S::oAr;
+  const T* __restrict__ As = tab + S::oAs;
+  const T* __restrict__ Mm = tab + S::oM;
+  const T* __restrict__ L1 = tab + S::oL1;
+  const T* __restrict__ LR = tab + S::oLR;
+  const T* __restrict__ LS = tab + S::oLS;
+  __shared__ T s_u[TILE * Np];
+
+  if (skip != nullptr && *skip) return;
+
+  const int el = threadIdx.x;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int k0 = tile * TILE;
+  const int ne = min(TILE, K - k0);
+  const int k = k0 + el;
+  const bool valid = el < ne;
+
+  // the tile's nodal values into LDS (one contiguous range when the tile
+  // order is the element order)
+  if (perm == nullptr) {
+    const T* src = u + (size_t)k0 * Np;
+    for (int i = el; i < ne * Np; i += TILE) s_u[i] = src[i];
+  } else {
+    for (int i = el; i < ne * Np; i += TILE) s_u[i] = u[(size_t)perm[k0 + i / Np] * Np + i % Np];
+  }
+  __syncthreads();
+
+  double dacc = 0.0;
+  T v[Np];
+  if (valid) {
+    T uk[Np];
+#pragma unroll
+    for (int j = 0; j < Np; ++j) uk[j] = s_u[el * Np + j];
+    T rc[DDGX_REC_WORDS];
+#pragma unroll
+    for (int w = 0; w < DDGX_REC_WORDS; ++w) rc[w] = rec[(size_t)w * K + k];
+    T ur[Np], us[Np];
+#pragma unroll
+    for (int n = 0; n < Np; ++n) {
+      T a = 0, b = 0;
+#pragma unroll
+      for (int j = 0; j < Np; ++j) {
+        a += Dr[j * RS + n] * uk[j];
+        b += Ds[j * RS + n] * uk[j];
+      }
+      ur[n] = a;
+      us[n] = b;
+    }
+    // volume
+    {
+      T wr[Np], ws[Np];
+#pragma unroll
+      for (int c = 0; c < Np; ++c) {
+        wr[c] = rc[R_VRR] * ur[c] + rc[R_VRS] * us[c];
+        ws[c] = rc[R_VRS] * ur[c] + rc[R_VSS] * us[c];
+      }
+#pragma unroll
+      for (int i = 0; i < Np; ++i) {
+        T acc = 0;
+#pragma unroll
+        for (int c = 0; c < Np; ++c) acc += Ar[c * RS + i] * wr[c] + As[c * RS + i] * ws[c];
+        v[i] = acc;
+      }
+    }
+    if constexpr (HELM) {
+      const T aj = alpha * rc[R_J];
+#pragma unroll
+      for (int i = 0; i < Np; ++i) {
+        T m = 0;
+#pragma unroll
+        for (int j = 0; j < Np; ++j) m += Mm[j * RS + i] * uk[j];
+        v[i] += aj * m + shift * uk[i];
+      }
+    }
+    // faces
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      const int c = nbrp[(size_t)f * K + k];
+      const T aM = rc[R_A + f], bM = rc[R_B + f];
+      const T sjw = rc[R_SJW + f], pen = rc[R_PEN + f];
+      T uP[Nfp], dnP[Nfp];
+      if (c >= 0) {
+        // interior: the neighbour's trace and normal derivative (its own
+        // face coefficients, so the sign flips to the own normal); a
+        // neighbour in the tile is read from LDS
+        const int np = nbr_elem(c), fn = nbr_face(c);
+        const bool rv = nbr_rev(c);
+        T un[Np];
+        if (np >= k0 && np < k0 + ne) {
+#pragma unroll
+          for (int j = 0; j < Np; ++j) un[j] = s_u[(np - k0) * Np + j];
+        } else {
+          const size_t ng = (size_t)(perm ? perm[np] : np) * Np;
+#pragma unroll
+          for (int j = 0; j < Np; ++j) un[j] = u[ng + j];
+        }
+        const T aP = rec[(size_t)(R_A + fn) * K + np], bP = rec[(size_t)(R_B + fn) * K + np];
+        T urn[Np], usn[Np];
+#pragma unroll
+        for (int n = 0; n < Np; ++n) {
+          T a = 0, b = 0;
+#pragma unroll
+          for (int j = 0; j < Np; ++j) {
+            a += Dr[j * RS + n] * un[j];
+            b += Ds[j * RS + n] * un[j];
+          }
+          urn[n] = a;
+          usn[n] = b;
+        }
+        // the neighbour's face node is a run-time index: pick it with a
+        // compare chain over the registers (no indexed register access)
+#pragma unroll
+        for (int p = 0; p < Nfp; ++p) {
+          const int idx = fmask<N>(fn, rv ? Nfp - 1 - p : p);
+          T uu = un[0], dr = urn[0], ds = usn[0];
+#pragma unroll
+          for (int m = 1; m < Np; ++m)
+            if (idx == m) {
+              uu = un[m];
+              dr = urn[m];
+              ds = usn[m];
+            }
+          uP[p] = uu;
+          dnP[p] = -(aP * dr + bP * ds);
+        }
+      } else if (c == CODE_DIRICHLET) {
+        const int kg = perm ? perm[k] : k;
+#pragma unroll
+        for (int p = 0; p < Nfp; ++p) {
+          if constexpr (UDIR)
+            uP[p] = udir[(size_t)kg * Np + fmask<N>(f, p)];
+          else
+            uP[p] = T(0);
+          dnP[p] = T(0);
+        }
+      } else {
+#pragma unroll
+        for (int p = 0; p < Nfp; ++p) {
+          uP[p] = uk[fmask<N>(f, p)];
+          dnP[p] = T(0);
+        }
+      }
+      T P[Nfp], Xa[Nfp], Xb[Nfp];
+#pragma unroll
+      for (int p = 0; p < Nfp; ++p) {
+        const int n = fmask<N>(f, p);
+        const T ju = uk[n] - uP[p];
+        const T dnM = aM * ur[n] + bM * us[n];
+        const T X = -sjw * ju;
+        P[p] = pen * ju - sjw * (dnM + dnP[p]);
+        Xa[p] = aM * X;
+        Xb[p] = bM * X;
+      }
+#pragma unroll
+      for (int q = 0; q < Nfp; ++q) {
+        T acc = 0;
+#pragma unroll
+        for (int p = 0; p < Nfp; ++p) acc += L1[(f * Nfp + p) * RSF + q] * P[p];
+        v[fmask<N>(f, q)] += acc;
+      }
+#pragma unroll
+      for (int i = 0; i < Np; ++i) {
+        T acc = 0;
+#pragma unroll
+        for (int p = 0; p < Nfp; ++p)
+          acc += LR[(f * Nfp + p) * RS + i] * Xa[p] + LS[(f * Nfp + p) * RS + i] * Xb[p];
+        v[i] += acc;
+      }
+    }
+    if constexpr (DOT) {
+#pragma unroll
+      for (int i = 0; i < Np; ++i) dacc += (double)uk[i] * (double)v[i];
+    }
+  }
+  // results through LDS (every thread has read its own and its in-tile
+  // neighbours' values before any is overwritten)
+  __syncthreads();
+  if (valid) {
+#pragma unroll
+    for (int i = 0; i < Np; ++i) s_u[el * Np + i] = v[i];
+  }
+  __syncthreads();
+  if (perm == nullptr) {
+    T* dst = y + (size_t)k0 * Np;
+    if constexpr (RESID) {
+      const T* bb = bvec + (size_t)k0 * Np;
+      for (int i = el; i < ne * Np; i += TILE) dst[i] = bb[i] - s_u[i];
+    } else {
+      for (int i = el; i < ne * Np; i += TILE) dst[i] = s_u[i];
+    }
+  } else {
+    for (int i = el; i < ne * Np; i += TILE) {
+      const size_t gi = (size_t)perm[k0 + i / Np] * Np + i % Np;
+      if constexpr (RESID)
+        y[gi] = bvec[gi] - s_u[i];
+      else
+        y[gi] = s_u[i];
+    }
+  }
+  if constexpr (DOT) {
+    for (int off = 32; off > 0; off >>= 1) dacc += __shfl_down(dacc, off, 64);
+    if (el == 0) partial[blockIdx.x] = dacc;
+  }
+}
+
+}  // namespace ddgx

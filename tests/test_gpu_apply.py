@@ -48,7 +48,7 @@ def test_apply_parity_f64(dev, N, bcname):
         assert rel(y, ref) < 1e-12
 
 
-@pytest.mark.parametrize("N", [3, 4, 6])
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 6])
 def test_apply_parity_f32(dev, N):
     pm, om = unit_square(6, 6, N, jitter=0.1, seed=3)
     bc = pmesh.dirichlet_x_neumann_y(pm)
@@ -119,11 +119,12 @@ def test_repeatable(dev):
     assert torch.equal(op.apply(u), op.apply(u))
 
 
-@pytest.mark.parametrize("N", [2, 4])
+@pytest.mark.parametrize("N", [1, 2, 4])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_apply_wide_halo_tiles(dev, N, dtype):
     """Row-major 24x24 mesh in its own numbering: 64-element tiles with up to
-    50 out-of-tile neighbours (the 64-entry halo variant of the kernel);
+    50 out-of-tile neighbours (the 64-entry halo variant of the matrix-core
+    kernel, the global-memory neighbour reads of the vector-ALU kernel);
     also the Morton order of the same mesh (32-entry variant, permuted
     element order).  Both against the oracle."""
     pm, om = unit_square(24, 24, N, jitter=0.1, seed=7)

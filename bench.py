@@ -13,7 +13,9 @@ Also reported, on the same mesh:
   -Delta u = 8 e^x cos 3y, u = e^x cos 3y on the Dirichlet walls, dn u on
   the Neumann walls, to tol 1e-10: solves/s, p50/p99 latency;
 * the same solves by mixed-precision iterative refinement (float32
-  operator, float32 V-cycle, float64 residual and update);
+  operator, float32 V-cycle, float64 residual and update), and by float64
+  CG preconditioned with the float32 V-cycle (mixed-precision
+  preconditioning: every float64 quantity of the CG stays float64);
 * both solvers on seeded random Fourier right-hand sides (64 modes,
   amplitude |k|^-2) on this mesh and on 2x128x128 at N=6, with the error
   against a float64 solve to tol 1e-13;
@@ -181,7 +183,8 @@ def lat_stats(lat):
 
 
 def solver_block(m, bc, b, mg64, mg32, n_solves, tol, inner_tol, ref_x=None, ue=None, log=""):
-    """FP64 MG-PCG and mixed-precision refinement on one right-hand side."""
+    """FP64 MG-PCG, mixed-precision refinement and FP64 CG with the FP32
+    V-cycle on one right-hand side."""
     import torch
     from ddg_amd import sipg as D
     from ddg_amd.operator import refine_solve
@@ -194,7 +197,8 @@ def solver_block(m, bc, b, mg64, mg32, n_solves, tol, inner_tol, ref_x=None, ue=
     for name, fn in (
             ("fp64", lambda: op64.pcg(b, pc=mg64, tol=tol, maxiter=5000)),
             ("mixed", lambda: refine_solve(op64, op32, b, pc32=mg32, tol=tol,
-                                           inner_tol=inner_tol, inner_maxiter=2000))):
+                                           inner_tol=inner_tol, inner_maxiter=2000)),
+            ("mixed_pc", lambda: op64.pcg(b, pc=mg32, tol=tol, maxiter=5000))):
         fn()  # warm-up
         lat, (x, info) = timed_solves(fn, n_solves)
         d = lat_stats(lat)
@@ -212,6 +216,7 @@ def solver_block(m, bc, b, mg64, mg32, n_solves, tol, inner_tol, ref_x=None, ue=
         res[name] = d
         print(f"[bench] {log} {name}: {d}", file=sys.stderr, flush=True)
     res["speedup_mixed_vs_fp64"] = round(res["fp64"]["p50_ms"] / res["mixed"]["p50_ms"], 3)
+    res["speedup_mixed_pc_vs_fp64"] = round(res["fp64"]["p50_ms"] / res["mixed_pc"]["p50_ms"], 3)
     return res
 
 
@@ -386,8 +391,12 @@ def main():
                                   inner_maxiter=2000)
             torch.cuda.synchronize()
             t2 = time.perf_counter()
+            _, ipc = op64.pcg(b, pc=mg32, tol=args.tol, maxiter=5000)
+            torch.cuda.synchronize()
+            t3 = time.perf_counter()
             print(f"[profile] fp64 {1e3 * (t1 - t0):.1f} ms {i64['iterations']} it; mixed "
-                  f"{1e3 * (t2 - t1):.1f} ms {imx['iterations']} it {imx['outer']} outer",
+                  f"{1e3 * (t2 - t1):.1f} ms {imx['iterations']} it {imx['outer']} outer; "
+                  f"fp64 CG + fp32 V-cycle {1e3 * (t3 - t2):.1f} ms {ipc['iterations']} it",
                   flush=True)
         return
 

@@ -519,7 +519,10 @@ int ddgx_solver_create(ddgx_op op, ddgx_op op32, ddgx_pc pc, ddgx_solver* out) {
   }
   if (pc != nullptr) {
     const int want = op32 != nullptr ? DDGX_F32 : a.dtype;
-    if (pc->pc.dtype != want || pc->pc.op->K != a.K || pc->pc.op->N != a.N) {
+    // a float32 multigrid may also precondition a float64 solve directly
+    const bool mixed_mg = op32 == nullptr && a.dtype == DDGX_F64 && pc->pc.dtype == DDGX_F32 &&
+                          pc->pc.type == 1;
+    if ((pc->pc.dtype != want && !mixed_mg) || pc->pc.op->K != a.K || pc->pc.op->N != a.N) {
       set_error("ddgx_solver_create: preconditioner of another mesh, order or precision");
       return DDGX_ERR_INVALID;
     }

@@ -206,6 +206,34 @@ bj_apply_kernel(const T* __restrict__ binv, const T* __restrict__ r, T* __restri
   }
 }
 
+// Float32 preconditioner inside a float64 CG: r32 = (float)(r / sqrt(rr)) and
+// z = sqrt(rr) * (double) z32, with rr the last <r,r> on the device (1 before
+// the first).  The preconditioner is linear, so the scale only keeps the
+// float32 values away from the ends of their range.
+__device__ __forceinline__ double pc_scale(const CGScalars* sc) {
+  return sc->rr > 0.0 ? sqrt(sc->rr) : 1.0;
+}
+
+__global__ void __launch_bounds__(256)
+pc_down_kernel(const double* __restrict__ r, float* __restrict__ r32,
+               const CGScalars* __restrict__ sc, size_t n, const int* __restrict__ skip) {
+  if (skip != nullptr && *skip) return;
+  const double inv = 1.0 / pc_scale(sc);
+  const size_t stride = (size_t)gridDim.x * 256;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
+    r32[i] = (float)(inv * r[i]);
+}
+
+__global__ void __launch_bounds__(256)
+pc_up_kernel(const float* __restrict__ z32, double* __restrict__ z,
+             const CGScalars* __restrict__ sc, size_t n, const int* __restrict__ skip) {
+  if (skip != nullptr && *skip) return;
+  const double sc_ = pc_scale(sc);
+  const size_t stride = (size_t)gridDim.x * 256;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
+    z[i] = sc_ * (double)z32[i];
+}
+
 // p = z + beta p
 template <typename T>
 __global__ void __launch_bounds__(256)
@@ -256,7 +284,8 @@ static size_t dsize(int dtype) { return dtype == DDGX_F64 ? 8 : 4; }
 void work_free(Work* w) {
   if (w == nullptr) return;
   if (w->chunk) (void)hipGraphExecDestroy(w->chunk);
-  void* ptrs[] = {w->r, w->p, w->z, w->Ap, w->partials, w->ir_b, w->ir_x, w->scal};
+  void* ptrs[] = {w->r,    w->p,    w->z,    w->Ap,  w->partials,
+                  w->ir_b, w->ir_x, w->pc_r, w->pc_z, w->scal};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (w->scal_host) (void)hipHostFree(w->scal_host);
@@ -407,6 +436,8 @@ static int pcg_t(const Op& op, Work& w, double alpha, double shift, const PC* pc
   const size_t n = (size_t)op.K * op.Np;
   const bool has_pc = pc != nullptr;
   const bool bj = has_pc && pc->type == 0;  // block-Jacobi: fused into the update
+  // float32 multigrid in a float64 solve: convert around the V-cycle
+  const bool mixed = has_pc && pc->dtype != op.dtype;
   CGScalars& init = *w.scal_host;  // pinned staging buffer
   init = CGScalars{};
   init.tol = cfg.tol;
@@ -425,8 +456,19 @@ static int pcg_t(const Op& op, Work& w, double alpha, double shift, const PC* pc
   hipLaunchKernelGGL((residual_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (const T*)b,
                      (const T*)w.Ap, (T*)w.r, n);
   DDGX_HIP(hipGetLastError());
+  auto precondition = [&](const int* skip) -> int {
+    if (!mixed) return mg_vcycle(*pc->mg, *mgw, w.r, w.z, skip, s);
+    hipLaunchKernelGGL(pc_down_kernel, dim3(op.grid_vec), dim3(256), 0, s,
+                       (const double*)w.r, (float*)w.pc_r, (const CGScalars*)w.scal, n, skip);
+    DDGX_HIP(hipGetLastError());
+    CHECK(mg_vcycle(*pc->mg, *mgw, w.pc_r, w.pc_z, skip, s));
+    hipLaunchKernelGGL(pc_up_kernel, dim3(op.grid_vec), dim3(256), 0, s, (const float*)w.pc_z,
+                       (double*)w.z, (const CGScalars*)w.scal, n, skip);
+    DDGX_HIP(hipGetLastError());
+    return DDGX_OK;
+  };
   if (bj) CHECK(launch_bj_apply(op.N, op.K, op.dtype, pc->binv, w.r, w.z, s));
-  else if (has_pc) CHECK(mg_vcycle(*pc->mg, *mgw, w.r, w.z, nullptr, s));
+  else if (has_pc) CHECK(precondition(nullptr));
   hipLaunchKernelGGL((copy_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (const T*)z,
                      (T*)w.p, n);
   DDGX_HIP(hipGetLastError());
@@ -447,7 +489,7 @@ static int pcg_t(const Op& op, Work& w, double alpha, double shift, const PC* pc
                          (const CGScalars*)w.scal, n, has_pc ? nullptr : w.partials);
       DDGX_HIP(hipGetLastError());
       if (has_pc) {
-        CHECK(mg_vcycle(*pc->mg, *mgw, w.r, w.z, done, s));
+        CHECK(precondition(done));
         CHECK(dot2<T>(op, w, w.r, w.z, w.r, w.r, done, s));
       }
     }
@@ -501,9 +543,29 @@ int pcg_solve(const Op& op, Work& w, double alpha, double shift, const PC* pc, M
               const void* b, void* x, const ddgx_cg_cfg& cfg, ddgx_cg_info* info,
               hipStream_t s) {
   CHECK(check_work(op, w));
-  if (pc != nullptr && pc->dtype != op.dtype) {
-    set_error("preconditioner dtype differs from operator dtype");
+  if (pc != nullptr && (pc->op->K != op.K || pc->op->N != op.N)) {
+    set_error("preconditioner was built for a different mesh or order");
     return DDGX_ERR_INVALID;
+  }
+  if (pc != nullptr && pc->dtype != op.dtype) {
+    // the one mixed case: a float32 multigrid V-cycle inside a float64 CG
+    if (!(op.dtype == DDGX_F64 && pc->dtype == DDGX_F32 && pc->type == 1)) {
+      set_error("preconditioner dtype differs from operator dtype (only a float32 multigrid "
+                "may precondition a float64 solve)");
+      return DDGX_ERR_INVALID;
+    }
+    if (w.pc_r == nullptr) {
+      const size_t nb = (size_t)op.K * op.Np * sizeof(float);
+      DDGX_HIP(hipSetDevice(op.device));
+      DDGX_HIP(hipMalloc(&w.pc_r, nb));
+      const hipError_t e = hipMalloc(&w.pc_z, nb);
+      if (e != hipSuccess) {
+        (void)hipFree(w.pc_r);
+        w.pc_r = nullptr;
+        return hip_fail(e, "float32 preconditioner buffers");
+      }
+      w.bytes += 2 * nb;
+    }
   }
   if (pc != nullptr && pc->type == 1 && mgw == nullptr) mgw = pc->mg->ws;
   if (op.dtype == DDGX_F64) return pcg_t<double>(op, w, alpha, shift, pc, mgw, b, x, cfg, info, s);

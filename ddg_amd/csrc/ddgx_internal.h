@@ -170,6 +170,8 @@ struct Work {
   int npart = 0;
   void* ir_b = nullptr;        // refinement (float32 work): scaled residual / correction
   void* ir_x = nullptr;
+  void* pc_r = nullptr;        // float64 solve with a float32 multigrid: residual and
+  void* pc_z = nullptr;        // preconditioned residual in float32 (allocated on first use)
   CGScalars* scal = nullptr;
   CGScalars* scal_host = nullptr;  // pinned
   size_t bytes = 0;            // device bytes held

@@ -14,8 +14,8 @@
 //
 // The kernels are in sipg_apply_mfma.h (matrix cores, 64-element tiles,
 // halo traces through LDS) and sipg_apply_valu.h (vector ALUs, one thread
-// per element: order 1, and order 2 in float32); this file holds the launch
-// code and the right-hand-side kernel.
+// per element: order 1); this file holds the launch code and the
+// right-hand-side kernel.
 
 #include "ddgx_internal.h"
 #include "sipg_apply_mfma.h"

@@ -1,4 +1,4 @@
-// SIPG apply for the low orders on the vector ALUs, one thread per element:
+// SIPG apply at order 1 on the vector ALUs, one thread per element:
 //   y = (A_sipg + alpha*J*M + shift*I) u          (or y = b - that, residual mode)
 // with the same closed form as the matrix-core kernel (src/ddg/dg2d/sipg.py
 // :142-337):
@@ -7,8 +7,8 @@
 //   face f : P = pen_f [u] - sJw_f (dn^- + dn^+)    -> y[Fmask_f] += m1d_f^T P
 //            X = -sJw_f [u]                          -> y += (a_f LR_f + b_f LS_f) X
 //
-// At N = 1, 2 a nodal vector has 3 or 6 entries, so the 16-row matrix-core
-// tiles would be mostly padding; here every element's contractions are
+// At N = 1 a nodal vector has 3 entries, so the 16-row matrix-core tiles
+// would be mostly padding; here every element's contractions are
 // unrolled scalar FMAs with the tables read through the scalar cache
 // (uniform addresses).  A workgroup is one wave and one 64-element tile
 // (same tiles, XCD mapping and dot partials as the matrix-core kernel):
@@ -22,13 +22,13 @@
 
 namespace ddgx {
 
-// Orders the apply runs on the vector ALUs instead of the matrix cores (the
-// faster of the two kernels at 2x256x256 on MI355X: N=1 10 us against about
-// 20 us in float64; N=2 11.5 against 12.6 us in float32, while float64 N=2
-// keeps the matrix cores, 21 against 25 us)
+// Orders the apply runs on the vector ALUs instead of the matrix cores: the
+// faster of the two kernels at 2x256x256 on MI355X.  N=1 takes 10 us against
+// about 20 us in float64; at N=2 the matrix cores win (float64 21 against
+// 25 us; float32 residual mode 13.3 against 15.0 us).
 template <typename T, int N>
 struct UseValuApply {
-  static constexpr bool value = N == 1 || (N == 2 && sizeof(T) == 4);
+  static constexpr bool value = N == 1;
 };
 
 template <typename T, int N, int MODE>

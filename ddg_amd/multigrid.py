@@ -37,7 +37,11 @@ from .reference import h_child_interp_matrices, p_interp_matrix
 class Multigrid(DevicePreconditioner):
     """V-cycle over levels ``[(mesh, bc_types, kappa), ...]`` (fine to coarse)
     with transfer kinds ``kinds[l]`` ("p" or "h") and host transfer matrices
-    ``mats[l]``."""
+    ``mats[l]``.  A float32 cycle preconditions float32 solves, the inner
+    solves of ``refine=`` and, directly, float64 solves (mixed-precision
+    preconditioning)."""
+
+    is_multigrid = True
 
     def __init__(self, levels, kinds, mats, *, tau_scale=200.0, alpha=0.0, nu_smooth=2,
                  omega=0.7, regularize=False, eps=1.0e-8, dtype=torch.float64, ops=None):

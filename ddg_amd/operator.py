@@ -283,8 +283,11 @@ class SipgOperator:
                 raise TypeError("pc must be a device preconditioner (block-Jacobi or multigrid)")
             if pc.K != self.K or pc.N != self.N:
                 raise ValueError("preconditioner was built for a different mesh/order")
-            if pc.dtype != self.dtype:
-                raise ValueError("preconditioner dtype differs from the operator dtype")
+            if pc.dtype != self.dtype and not (
+                    self.dtype == torch.float64 and pc.dtype == torch.float32
+                    and getattr(pc, "is_multigrid", False)):
+                raise ValueError("preconditioner dtype differs from the operator dtype (only a "
+                                 "float32 multigrid may precondition a float64 solve)")
             pch = pc.handle
         _lib.check(_lib.lib().ddgx_pcg_solve(self._h, float(alpha), float(shift), pch,
                                              _ptr(b), _ptr(x), C.byref(cfg), C.byref(info),
@@ -330,6 +333,7 @@ class DevicePreconditioner:
     _h = None
     K = N = 0
     dtype = torch.float64
+    is_multigrid = False
 
     @property
     def handle(self):

@@ -141,7 +141,11 @@ int ddgx_mg_create(int n_levels, const ddgx_op* ops, const ddgx_op* ops64,
 
 /* Preconditioned CG on (A + alpha*M + shift*I) x = b with the stopping rule
  * of jax.scipy.sparse.linalg.cg as called at sipg.py:603-609, 909-915.
- * x holds x0 on entry and the solution on exit.  pc may be NULL.
+ * x holds x0 on entry and the solution on exit.  pc may be NULL; it has the
+ * operator's dtype, except that a float32 multigrid may precondition a
+ * float64 solve (the V-cycle runs in float32 on a scaled copy of the
+ * residual; everything else stays float64 -- an extension of the reference,
+ * whose solves are float64 only).
  * Iterations run without host synchronisation; the host polls a device
  * stop flag every cfg->check_every iterations. */
 int ddgx_pcg_solve(ddgx_op op, double alpha, double shift, ddgx_pc pc,
@@ -204,7 +208,8 @@ int ddgx_dot(ddgx_op op, const void* a, const void* b, double* out, void* stream
  * run at once on different streams (the request loop in ddg_amd/serve.py).
  * ddgx_pcg_solve / ddgx_ir_solve / ddgx_pc_apply use one default context per
  * handle instead, so their concurrent calls on one handle race.
- *   op32 == NULL: a PCG solver on op (pc of op's dtype or NULL);
+ *   op32 == NULL: a PCG solver on op (pc of op's dtype, a float32 multigrid
+ *                 for a float64 op, or NULL);
  *   op32 != NULL: a refinement solver (op float64, op32 its float32 twin, pc
  *                 float32 or NULL).
  * The operators and preconditioner must outlive the solver; one solver runs

@@ -72,3 +72,24 @@ def test_graph_refinement_equals_eager(problem):
             x_a, _ = refine_solve(op64, op32, b, alpha=alpha, pc32=mg32, tol=1e-10,
                                   inner_tol=1e-4, inner_maxiter=500, check_every=8)
             assert torch.equal(x, x_a)
+
+
+def test_graph_mixed_preconditioned_pcg_equals_eager(problem):
+    """Float64 CG with the float32 V-cycle: the captured chunk (conversions
+    around the cycle included) replays the eager solve bit for bit."""
+    pm, op64, op32, mg64, mg32, b = problem
+    _say("graph pcg with float32 V-cycle")
+    x_e, info_e = op64.pcg(b, pc=mg32, tol=1e-11, maxiter=500, check_every=8)
+    assert info_e["converged"]
+    ctx = SolverContext(op64, pc=mg32)
+    s = torch.cuda.Stream()
+    cfg = _lib.CGConfig(1e-11, 0.0, 500, 8)
+    for rep in range(3):
+        x = torch.zeros_like(b)
+        with torch.cuda.stream(s):
+            info = ctx.pcg(b, x, 0.0, cfg, s.cuda_stream)
+        s.synchronize()
+        assert info.iterations == info_e["iterations"] and bool(info.converged)
+        assert torch.equal(x, x_e), f"replay {rep} differs from the eager solve"
+        op64.apply(x)
+        torch.cuda.synchronize()

@@ -137,6 +137,34 @@ def test_mg_pcg_matches_direct_and_oracle(dev, alpha):
     assert np.linalg.norm(x - x_direct) / np.linalg.norm(x_direct) < 1e-7
 
 
+@pytest.mark.parametrize("alpha", [0.0, 30.0])
+def test_float64_cg_with_float32_vcycle(dev, alpha):
+    """Mixed-precision preconditioning: float64 CG with the float32 V-cycle
+    meets the float64 tolerance (against the oracle's dense solve, 1e-8
+    relative at tol 1e-10) in about the float64 cycle's iteration count
+    (within 10%, at least 3: the float32 cycle is the float64 one up to
+    round-off), repeatably; a float32 block-Jacobi is refused."""
+    pm, omesh = unit_square(8, 8, 4, jitter=0.1, seed=2)
+    bc = pmesh.dirichlet_x_neumann_y(pm)
+    b = np.random.default_rng(21).standard_normal(pm.K * pm.Np)
+    x_direct = np.linalg.solve(S.sipg_matrix_2d(omesh, bc_types=bc, alpha=alpha), b)
+    _say(f"fp64 cg + fp32 vcycle alpha={alpha}")
+    mg64 = MG.make_sipg_mg_preconditioner(pm, bc_types=bc, alpha=alpha)
+    mg32 = MG.make_sipg_mg_preconditioner(pm, bc_types=bc, alpha=alpha, dtype=torch.float32)
+    op = D.get_operator(pm, bc)
+    bt = torch.as_tensor(b, device=dev)
+    _, i64 = op.pcg(bt, alpha=alpha, pc=mg64, tol=1e-10)
+    x, info = op.pcg(bt, alpha=alpha, pc=mg32, tol=1e-10)
+    assert info["converged"]
+    assert info["iterations"] <= i64["iterations"] + max(3, 0.1 * i64["iterations"])
+    err = np.linalg.norm(x.cpu().numpy() - x_direct) / np.linalg.norm(x_direct)
+    assert err < 1e-8, err
+    x2, _ = op.pcg(bt, alpha=alpha, pc=mg32, tol=1e-10)
+    assert torch.equal(x, x2)
+    with pytest.raises(ValueError):
+        op.pcg(bt, alpha=alpha, pc=op.block_jacobi(alpha, dtype=torch.float32), tol=1e-10)
+
+
 def test_solve_with_hpmg_preconditioner(dev):
     """Drop-in solve with an hp-MG preconditioner: Poisson with Dirichlet and
     Neumann data on a forest mesh, against the oracle LU solve."""

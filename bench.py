@@ -220,16 +220,17 @@ def solver_block(m, bc, b, mg64, mg32, n_solves, tol, inner_tol, ref_x=None, ue=
     return res
 
 
-def serve_block(n_requests, slots, calib=200, load=0.8, sample=8, seed=0):
+def serve_block(n_requests, slots, method="pcg32", calib=200, load=0.8, sample=8, seed=0):
     """The request stream: Helmholtz solves on 2 n^2 meshes, n in {16, 32, 64,
-    128}, N in {2..6}, k^2 in [0, 50], Fourier right-hand sides, mixed-precision
-    refinement to 1e-10.  A closed-loop burst of ``calib`` requests measures
+    128}, N in {2..6}, k^2 in [0, 50], Fourier right-hand sides, mixed
+    precision to 1e-10 (``method``: "refine", iterative refinement; "pcg32",
+    float64 CG with the float32 V-cycle).  A closed-loop burst of ``calib`` requests measures
     the capacity C; then ``n_requests`` arrive as a Poisson stream at
     ``load * C`` per second.  A sample of results is checked against a
     float64 block-Jacobi PCG solve to tol 1e-13."""
     import torch
     from ddg_amd import serve as SV
-    srv = SV.SolveServer(SV.ServeConfig(slots=slots))
+    srv = SV.SolveServer(SV.ServeConfig(slots=slots, method=method))
     reqs = SV.make_requests(n_requests, seed=seed)
     t0 = time.perf_counter()
     srv.prepare(reqs)
@@ -255,7 +256,7 @@ def serve_block(n_requests, slots, calib=200, load=0.8, sample=8, seed=0):
                              maxiter=50000)
         errs.append(float(torch.linalg.norm(x - xr) / torch.linalg.norm(xr)))
     srv.close()
-    out = {"requests": st["requests"], "offered_rate_per_s": round(rate, 2),
+    out = {"method": method, "requests": st["requests"], "offered_rate_per_s": round(rate, 2),
            "solves_per_s": st["solves_per_s"], "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
            "service_p50_ms": st["service_p50_ms"], "service_p99_ms": st["service_p99_ms"],
            "converged": st["converged"], "at_residual_floor": st["at_residual_floor"],
@@ -293,6 +294,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=2000,
                     help="requests in the serving stream (0: skip the serving leg)")
     ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-method", default="pcg32", choices=["pcg32", "refine"])
     ap.add_argument("--serve-only", action="store_true", help="run only the serving leg")
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--order", default="forest", choices=["forest", "rowmajor"])
@@ -356,7 +358,7 @@ def main():
         print(f"[profile] {args.steps} copies of {args.profile_copy} MB", flush=True)
         return
     if args.serve_only:
-        print(json.dumps({"serve": serve_block(args.serve_requests, args.serve_slots)}),
+        print(json.dumps({"serve": serve_block(args.serve_requests, args.serve_slots, args.serve_method)}),
               flush=True)
         return
     m, bc = workload_mesh(args.n, order=args.order)
@@ -449,7 +451,7 @@ def main():
 
     serve = None
     if args.serve_requests > 0:
-        serve = serve_block(args.serve_requests, args.serve_slots)
+        serve = serve_block(args.serve_requests, args.serve_slots, args.serve_method)
 
     cpu = None
     if not args.no_cpu:

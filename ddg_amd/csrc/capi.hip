@@ -533,6 +533,10 @@ int ddgx_solver_create(ddgx_op op, ddgx_op op32, ddgx_pc pc, ddgx_solver* out) {
   h->pc = pc ? &pc->pc : nullptr;
   int rc = work_create(a, false, &h->w);
   if (rc == DDGX_OK && op32 != nullptr) rc = work_create(op32->op, true, &h->w32);
+  // a float64 solve with a float32 multigrid: its conversion buffers now, so
+  // the reported footprint is complete and no solve allocates
+  if (rc == DDGX_OK && op32 == nullptr && pc != nullptr && pc->pc.dtype != a.dtype)
+    rc = work_alloc_pc32(*h->w);
   // a solver context replays captured chunks of CG iterations
   if (rc == DDGX_OK) h->w->use_graph = true;
   if (rc == DDGX_OK && h->w32) h->w32->use_graph = true;

@@ -323,6 +323,21 @@ int work_create(const Op& op, bool refine, Work** out) {
   return DDGX_OK;
 }
 
+int work_alloc_pc32(Work& w) {
+  if (w.pc_r != nullptr) return DDGX_OK;
+  const size_t nb = (size_t)w.K * w.Np * sizeof(float);
+  DDGX_HIP(hipSetDevice(w.device));
+  DDGX_HIP(hipMalloc(&w.pc_r, nb));
+  const hipError_t e = hipMalloc(&w.pc_z, nb);
+  if (e != hipSuccess) {
+    (void)hipFree(w.pc_r);
+    w.pc_r = nullptr;
+    return hip_fail(e, "float32 preconditioner buffers");
+  }
+  w.bytes += 2 * nb;
+  return DDGX_OK;
+}
+
 int default_work(Op& op, bool refine, Work** out) {
   if (op.ws != nullptr && (!refine || op.ws->ir_b != nullptr)) {
     *out = op.ws;
@@ -554,18 +569,7 @@ int pcg_solve(const Op& op, Work& w, double alpha, double shift, const PC* pc, M
                 "may precondition a float64 solve)");
       return DDGX_ERR_INVALID;
     }
-    if (w.pc_r == nullptr) {
-      const size_t nb = (size_t)op.K * op.Np * sizeof(float);
-      DDGX_HIP(hipSetDevice(op.device));
-      DDGX_HIP(hipMalloc(&w.pc_r, nb));
-      const hipError_t e = hipMalloc(&w.pc_z, nb);
-      if (e != hipSuccess) {
-        (void)hipFree(w.pc_r);
-        w.pc_r = nullptr;
-        return hip_fail(e, "float32 preconditioner buffers");
-      }
-      w.bytes += 2 * nb;
-    }
+    CHECK(work_alloc_pc32(w));
   }
   if (pc != nullptr && pc->type == 1 && mgw == nullptr) mgw = pc->mg->ws;
   if (op.dtype == DDGX_F64) return pcg_t<double>(op, w, alpha, shift, pc, mgw, b, x, cfg, info, s);

@@ -186,6 +186,8 @@ struct Work {
   hipStream_t g_stream = nullptr;
 };
 int work_create(const Op& op, bool refine, Work** out);
+// float32 buffers of a float64 solve preconditioned by a float32 multigrid
+int work_alloc_pc32(Work& w);
 void work_free(Work* w);
 
 struct MG;

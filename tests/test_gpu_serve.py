@@ -1,7 +1,8 @@
-"""Request loop on the GPU: every result is bitwise the same alone and in a
-mix of concurrent requests (stated tolerance: 0), served results meet the
-float64 tolerance against the oracle's dense solve, and admission keeps
-the slot and HBM limits."""
+"""Request loop on the GPU, with both solve methods (iterative refinement and
+float64 CG with the float32 V-cycle): every result is bitwise the same alone
+and in a mix of concurrent requests (stated tolerance: 0), served results
+meet the float64 tolerance against the oracle's dense solve, and admission
+keeps the slot and HBM limits."""
 import numpy as np
 import pytest
 import torch
@@ -21,7 +22,7 @@ def _say(*a):
 
 @pytest.fixture(scope="module")
 def server(dev):
-    srv = SolveServer(ServeConfig(slots=4, keep_solutions=True, tol=1e-10))
+    srv = SolveServer(ServeConfig(method="refine", slots=4, keep_solutions=True, tol=1e-10))
     yield srv
     srv.close()
 
@@ -81,3 +82,36 @@ def test_poisson_stream_within_limits(server):
     assert rep.peak_bytes <= 2 * fp and rep.peak_slots <= 4
     for res in rep.results:
         assert res.start >= res.arrival and res.end >= res.start
+
+
+def test_pcg32_method_alone_equals_mix_and_matches_oracle(dev):
+    """The "pcg32" method (float64 CG with the float32 V-cycle): bitwise the
+    same alone and in a mix, and within 1e-9 of the oracle's dense solve."""
+    srv = SolveServer(ServeConfig(method="pcg32", slots=4, keep_solutions=True, tol=1e-10))
+    try:
+        reqs = make_requests(12, seed=15, sizes=(8, 16), orders=(2, 3, 4), k2_range=(0, 50))
+        srv.prepare(reqs)
+        alone = {}
+        for r in reqs:
+            x, info = srv.solve_one(r, srv.streams[0])
+            torch.cuda.synchronize()
+            assert info.converged, (r, info.residual2, info.atol2)
+            alone[r.rid] = x.clone()
+        _say("serve pcg32: mix")
+        poisson_arrivals(reqs, None)
+        rep = srv.run(reqs)
+        assert len(rep.results) == len(reqs) and rep.peak_slots > 1
+        for res in rep.results:
+            assert res.converged
+            assert torch.equal(res.x, alone[res.rid]), f"request {res.rid} differs in the mix"
+        r = reqs[0]
+        pc = srv.problem_class(r.n, r.N)
+        m = pc.mesh
+        mo = om.build_mesh_2d(m.VX, m.VY, m.EToV, m.N)
+        f = fourier_field(pc.x, pc.y, r.seed).cpu().numpy().reshape(m.K, m.Np).T
+        b = S.mass_apply(mo, f).T.reshape(-1)
+        xo = np.linalg.solve(S.sipg_matrix_2d(mo, bc_types=pc.bc, alpha=r.k2), b)
+        err = np.linalg.norm(alone[r.rid].cpu().numpy() - xo) / np.linalg.norm(xo)
+        assert err < 1e-9, err
+    finally:
+        srv.close()

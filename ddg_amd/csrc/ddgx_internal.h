@@ -106,8 +106,9 @@ __host__ __device__ inline int nbr_rev(int c) { return c & 1; }
 constexpr int CODE_DIRICHLET = -1;
 constexpr int CODE_NONE = -2;
 
-// Kernel mode bits for the apply
-enum : int { M_HELM = 1, M_UDIR = 2, M_DOT = 8, M_RESID = 16 };
+// Kernel mode bits for the apply (M_SMOOTH: one block-Jacobi sweep
+// x_out = x + omega Binv (b - A x), order 1 only)
+enum : int { M_HELM = 1, M_UDIR = 2, M_DOT = 8, M_RESID = 16, M_SMOOTH = 32 };
 
 // Device scalars of one CG solve
 struct CGScalars {
@@ -251,6 +252,11 @@ int hip_fail(hipError_t e, const char* what);
 int launch_apply(const Op& op, const void* u, void* y, const void* udir, double alpha,
                  double shift, int mode, double* partial, const int* skip, hipStream_t s,
                  const void* bvec = nullptr);
+// One fused block-Jacobi sweep x_out = x_in + omega Binv (b - (A + alpha M) x_in)
+// (x_out must not alias x_in); only for smooth_fused(op) operators.
+bool smooth_fused(const Op& op);
+int launch_smooth(const Op& op, const void* binv, double omega, double alpha, const void* x_in,
+                  void* x_out, const void* b, const int* skip, hipStream_t s);
 int mg_vcycle(const MG& mg, MGWork& w, const void* b, void* x, const int* skip, hipStream_t s);
 int mg_create(int nlev, Op* const* ops, Op* const* ops64, const int* tkind,
               const double* const* P, const ddgx_mg_cfg& cfg, hipStream_t s, MG** out);

@@ -301,6 +301,33 @@ static int vcycle(const MG& mg, MGWork& w, int l, const void* b, void* x, const 
   auto resid = [&]() {
     return launch_apply(op, x, Lr, nullptr, mg.alpha, 0.0, M_RESID, nullptr, skip, s, b);
   };
+  if (mg.nu > 0 && smooth_fused(op)) {
+    // fused sweeps x_out = x_in + omega Binv (b - A x_in), alternating between
+    // x and the residual buffer: the first sweep (x = omega Binv b) lands
+    // where nu - 1 fused sweeps bring the iterate back to x, the residual of
+    // the result goes to Lr for the restriction, and the post-smoother runs
+    // its sweeps in pairs x -> Lr -> x (an odd last one as residual + sweep).
+    void* bufs[2] = {x, Lr};
+    int at = (mg.nu - 1) % 2;  // buffer of the first sweep's output
+    MG_CHECK(bj_smooth<T>(L, b, bufs[at], mg.omega, 0, skip, s));
+    for (int i = 1; i < mg.nu; ++i) {
+      MG_CHECK(launch_smooth(op, L.binv, mg.omega, mg.alpha, bufs[at], bufs[1 - at], b, skip, s));
+      at = 1 - at;
+    }
+    MG_CHECK(resid());
+    MG_CHECK(transfer<T>(L, C, false, Lr, Cb, skip, s));
+    MG_CHECK(vcycle<T>(mg, w, l + 1, Cb, Cx, skip, s));
+    MG_CHECK(transfer<T>(L, C, true, Cx, x, skip, s));
+    for (int i = 0; i + 1 < mg.nu; i += 2) {
+      MG_CHECK(launch_smooth(op, L.binv, mg.omega, mg.alpha, x, Lr, b, skip, s));
+      MG_CHECK(launch_smooth(op, L.binv, mg.omega, mg.alpha, Lr, x, b, skip, s));
+    }
+    if (mg.nu % 2) {
+      MG_CHECK(resid());
+      MG_CHECK(bj_smooth<T>(L, Lr, x, mg.omega, 1, skip, s));
+    }
+    return DDGX_OK;
+  }
   if (mg.nu == 0) {
     const size_t n = (size_t)op.K * op.Np;
     hipLaunchKernelGGL((zero_kernel<T>), dim3(vec_grid(n)), dim3(256), 0, s, (T*)x, n, skip);

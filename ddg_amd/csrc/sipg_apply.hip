@@ -33,7 +33,7 @@ static int launch_t(const Op& op, const void* u, void* y, const void* udir, doub
     hipLaunchKernelGGL((sipg_apply_valu_kernel<T, N, MODE>), dim3(op.grid_apply), dim3(TILE), 0,
                        s, (const T*)op.tab, (const T*)op.rec, op.nbrp, perm, (const T*)u, (T*)y,
                        (const T*)udir, (T)alpha, (T)shift, (int)op.K, partial, skip,
-                       (const T*)bvec);
+                       (const T*)bvec, (const T*)nullptr, T(0));
   else if (op.halo_cap == HALO_SMALL)
     hipLaunchKernelGGL((sipg_apply_mfma_kernel<T, N, MODE, HALO_SMALL>), dim3(op.grid_apply),
                        dim3(256), 0, s, (const T*)op.frag, (const T*)op.rec, op.fcode, perm,
@@ -92,6 +92,45 @@ int launch_apply(const Op& op, const void* u, void* y, const void* udir, double 
 #undef DISPATCH_N
   set_error("polynomial order " + std::to_string(op.N) + " not built (1..6)");
   return DDGX_ERR_UNSUPPORTED;
+}
+
+// Fused block-Jacobi sweep of the multigrid smoother on the vector-ALU
+// kernel (the one order that kernel serves): the residual never leaves
+// registers, and one launch replaces the residual apply and the sweep.
+bool smooth_fused(const Op& op) { return op.N == 1; }
+
+template <typename T, int MODE>
+static int launch_smooth_t(const Op& op, const void* binv, double omega, double alpha,
+                           const void* x_in, void* x_out, const void* b, const int* skip,
+                           hipStream_t s) {
+  const int* perm = op.perm_id ? nullptr : op.perm;
+  hipLaunchKernelGGL((sipg_apply_valu_kernel<T, 1, MODE>), dim3(op.grid_apply), dim3(TILE), 0, s,
+                     (const T*)op.tab, (const T*)op.rec, op.nbrp, perm, (const T*)x_in, (T*)x_out,
+                     (const T*)nullptr, (T)alpha, T(0), (int)op.K, (double*)nullptr, skip,
+                     (const T*)b, (const T*)binv, (T)omega);
+  DDGX_HIP(hipGetLastError());
+  return DDGX_OK;
+}
+
+int launch_smooth(const Op& op, const void* binv, double omega, double alpha, const void* x_in,
+                  void* x_out, const void* b, const int* skip, hipStream_t s) {
+  if (!smooth_fused(op)) {
+    set_error("fused smoothing sweep is built for order 1 only");
+    return DDGX_ERR_UNSUPPORTED;
+  }
+  if (x_in == x_out) {
+    set_error("fused smoothing sweep: x_out must not alias x_in");
+    return DDGX_ERR_INVALID;
+  }
+  const bool helm = alpha != 0.0;
+  if (op.dtype == DDGX_F64)
+    return helm ? launch_smooth_t<double, M_SMOOTH | M_HELM>(op, binv, omega, alpha, x_in, x_out,
+                                                              b, skip, s)
+                : launch_smooth_t<double, M_SMOOTH>(op, binv, omega, alpha, x_in, x_out, b,
+                                                     skip, s);
+  return helm ? launch_smooth_t<float, M_SMOOTH | M_HELM>(op, binv, omega, alpha, x_in, x_out, b,
+                                                           skip, s)
+              : launch_smooth_t<float, M_SMOOTH>(op, binv, omega, alpha, x_in, x_out, b, skip, s);
 }
 
 // ---------------------------------------------------------------------------

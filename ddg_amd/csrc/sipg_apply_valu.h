@@ -37,11 +37,14 @@ sipg_apply_valu_kernel(const T* __restrict__ tab, const T* __restrict__ rec,
                        const int* __restrict__ nbrp, const int* __restrict__ perm,
                        const T* __restrict__ u, T* __restrict__ y, const T* __restrict__ udir,
                        T alpha, T shift, int K, double* __restrict__ partial,
-                       const int* __restrict__ skip, const T* __restrict__ bvec) {
+                       const int* __restrict__ skip, const T* __restrict__ bvec,
+                       const T* __restrict__ binv, T omega) {
   using S = Shape<N>;
   constexpr int Np = S::Np, Nfp = S::Nfp, RS = S::RS, RSF = S::RSF;
   constexpr bool HELM = MODE & M_HELM, UDIR = MODE & M_UDIR, DOT = MODE & M_DOT;
   constexpr bool RESID = MODE & M_RESID;
+  // smoothing sweep: y = u + omega Binv (bvec - A u), element by element
+  constexpr bool SMOOTH = MODE & M_SMOOTH;
   const T* __restrict__ Dr = tab + S::oDr;
   const T* __restrict__ Ds = tab + S::oDs;
   const T* __restrict__ Ar = tab + S::oAr;
@@ -217,6 +220,22 @@ sipg_apply_valu_kernel(const T* __restrict__ tab, const T* __restrict__ rec,
 #pragma unroll
       for (int i = 0; i < Np; ++i) dacc += (double)uk[i] * (double)v[i];
     }
+    if constexpr (SMOOTH) {
+      constexpr int NPK = SymPack<N>::NPK;
+      const size_t kg = (size_t)(perm ? perm[k] : k);
+      T r[Np], bk[NPK];
+#pragma unroll
+      for (int i = 0; i < Np; ++i) r[i] = bvec[kg * Np + i] - v[i];
+#pragma unroll
+      for (int q = 0; q < NPK; ++q) bk[q] = binv[kg * NPK + q];
+#pragma unroll
+      for (int i = 0; i < Np; ++i) {
+        T acc = 0;
+#pragma unroll
+        for (int j = 0; j < Np; ++j) acc += bk[SymPack<N>::pk(i, j)] * r[j];
+        v[i] = uk[i] + omega * acc;
+      }
+    }
   }
   // results through LDS (every thread has read its own and its in-tile
   // neighbours' values before any is overwritten)
@@ -228,7 +247,7 @@ sipg_apply_valu_kernel(const T* __restrict__ tab, const T* __restrict__ rec,
   __syncthreads();
   if (perm == nullptr) {
     T* dst = y + (size_t)k0 * Np;
-    if constexpr (RESID) {
+    if constexpr (RESID && !SMOOTH) {
       const T* bb = bvec + (size_t)k0 * Np;
       for (int i = el; i < ne * Np; i += TILE) dst[i] = bb[i] - s_u[i];
     } else {
@@ -237,7 +256,7 @@ sipg_apply_valu_kernel(const T* __restrict__ tab, const T* __restrict__ rec,
   } else {
     for (int i = el; i < ne * Np; i += TILE) {
       const size_t gi = (size_t)perm[k0 + i / Np] * Np + i % Np;
-      if constexpr (RESID)
+      if constexpr (RESID && !SMOOTH)
         y[gi] = bvec[gi] - s_u[i];
       else
         y[gi] = s_u[i];

@@ -122,16 +122,25 @@ zero_kernel(T* __restrict__ x, size_t n, const int* __restrict__ skip) {
 // Coarse solve x = Ainv b, with Ainv stored so that x_i = sum_j W[j][i] b_j
 // (coalesced across i).  Stage 1: partial[jc][i] over a chunk of j;
 // stage 2: fixed-order sum over chunks.
-constexpr int GEMV_JC = 128;
+// Coarse solve x = W b as column chunks (partial sums, then a fixed-order
+// sum): at most GEMV_NCH chunks of at least 16 columns, so the coarse-size
+// range (<= 8192 unknowns) gets ~400 workgroups of short dependent chains.
+constexpr int GEMV_JC = 128;  // largest chunk (MG_MAX_COARSE / GEMV_NCH)
+constexpr int GEMV_NCH = 64;
+static_assert(GEMV_JC * GEMV_NCH >= MG_MAX_COARSE, "coarse chunk too narrow");
+static int gemv_jc(int n) {
+  const int jc = (n + GEMV_NCH - 1) / GEMV_NCH;
+  return jc < 16 ? 16 : jc;
+}
 
 template <typename T>
 __global__ void __launch_bounds__(256)
 coarse_gemv_partial_kernel(const double* __restrict__ W, const T* __restrict__ b,
-                           double* __restrict__ part, int n, const int* __restrict__ skip) {
+                           double* __restrict__ part, int n, int jc, const int* __restrict__ skip) {
   if (skip != nullptr && *skip) return;
   __shared__ double sb[GEMV_JC];
-  const int j0 = blockIdx.y * GEMV_JC;
-  const int nj = min(GEMV_JC, n - j0);
+  const int j0 = blockIdx.y * jc;
+  const int nj = min(jc, n - j0);
   if (threadIdx.x < nj) sb[threadIdx.x] = (double)b[j0 + threadIdx.x];
   __syncthreads();
   const int i = blockIdx.x * 256 + threadIdx.x;
@@ -141,14 +150,17 @@ coarse_gemv_partial_kernel(const double* __restrict__ W, const T* __restrict__ b
   part[(size_t)blockIdx.y * n + i] = acc;
 }
 
+// x[i] = sum of the chunk partials in chunk order (64-thread workgroups;
+// the unrolled loop keeps the independent loads in flight together)
 template <typename T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(64)
 coarse_gemv_sum_kernel(const double* __restrict__ part, T* __restrict__ x, int n, int nchunks,
                        const int* __restrict__ skip) {
   if (skip != nullptr && *skip) return;
-  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int i = blockIdx.x * 64 + threadIdx.x;
   if (i >= n) return;
   double acc = 0.0;
+#pragma unroll 16
   for (int c = 0; c < nchunks; ++c) acc += part[(size_t)c * n + i];
   x[i] = (T)acc;
 }
@@ -225,7 +237,11 @@ static int bj_smooth_t(const MGLevel& L, const void* r, void* x, double omega, i
                        const int* skip, hipStream_t s) {
   const int K = (int)L.op->K;
   const int ntiles = (K + Shape<N>::EPB - 1) / Shape<N>::EPB;
-  int grid = 8 * device_cu_count();
+  // workgroups per compute unit: float64 blocks of the higher orders gain from
+  // one workgroup per tile (N=4 at 2x256x256: 28.5 against 34.7 us with 8 per
+  // CU looping over tiles); the lighter sweeps are faster with 8
+  constexpr int PER_CU = (sizeof(T) == 8 && N >= 4) ? 32 : 8;
+  int grid = PER_CU * device_cu_count();
   grid = ntiles < grid ? ntiles : grid;
   hipLaunchKernelGGL((bj_smooth_kernel<T, N>), dim3(grid), dim3(256), 0, s,
                      (const T*)L.binv, (const T*)r, (T*)x, K, (T)omega, acc, skip);
@@ -258,11 +274,12 @@ template <typename T>
 static int coarse_solve(const MG& mg, MGWork& w, const void* b, void* x, const int* skip,
                         hipStream_t s) {
   const int n = mg.nc;
-  const int nch = (n + GEMV_JC - 1) / GEMV_JC;
+  const int jc = gemv_jc(n);
+  const int nch = (n + jc - 1) / jc;
   hipLaunchKernelGGL((coarse_gemv_partial_kernel<T>), dim3((n + 255) / 256, nch), dim3(256), 0,
-                     s, (const double*)mg.Ainv, (const T*)b, w.gemv_part, n, skip);
+                     s, (const double*)mg.Ainv, (const T*)b, w.gemv_part, n, jc, skip);
   DDGX_HIP(hipGetLastError());
-  hipLaunchKernelGGL((coarse_gemv_sum_kernel<T>), dim3((n + 255) / 256), dim3(256), 0, s,
+  hipLaunchKernelGGL((coarse_gemv_sum_kernel<T>), dim3((n + 63) / 64), dim3(64), 0, s,
                      (const double*)w.gemv_part, (T*)x, n, nch, skip);
   DDGX_HIP(hipGetLastError());
   return DDGX_OK;
@@ -390,7 +407,8 @@ int mgwork_create(const MG& mg, MGWork** out) {
       w->bytes += nv;
     }
   }
-  const int nch = (mg.nc + GEMV_JC - 1) / GEMV_JC;
+  const int jc = gemv_jc(mg.nc);
+  const int nch = (mg.nc + jc - 1) / jc;
   const size_t ng = (size_t)nch * mg.nc * sizeof(double);
   if (e == hipSuccess) e = hipMalloc((void**)&w->gemv_part, ng);
   w->bytes += ng;

@@ -74,7 +74,6 @@ struct MS {
   static constexpr int NF = 9 * Nfp;            // face rows: P, aX, bX
   static constexpr int FK = (NF + 3) / 4;       // k-steps over the face rows
   static constexpr int FS = 4 * FK + 1;         // LDS row stride of the face rows (odd)
-  static constexpr int NS = 3 * Np + (3 * Np + 1) % 2;  // LDS words per element: u, ur, us (odd)
   static constexpr int TRW = 3 * Nfp * 2;       // trace words per element: u, dn per face node
   static constexpr int NFP3 = 3 * Nfp;          // face nodes of an element
   static constexpr int FPW = NFP3 <= 8 ? 8 : NFP3 <= 16 ? 16 : 32;  // face-phase lanes per element
@@ -133,42 +132,6 @@ __device__ __forceinline__ void grad_and_traces(const T (&A1)[2 * MS<N>::NB][MS<
     }
 }
 
-// GEMM 1 for 16 elements of a wave: [ur; us] = [Dr; Ds] u, and u, ur, us of
-// every node into the element's LDS slot (nodes in natural order).
-template <typename T, int N>
-__device__ __forceinline__ void grad_to_lds(const T (&A1)[2 * MS<N>::NB][MS<N>::KB],
-                                            const T (&U)[MS<N>::NB][4], T (&UR)[MS<N>::NB][4],
-                                            T (&US)[MS<N>::NB][4], T* sl, int g) {
-  using Q = MS<N>;
-  using MF = MfmaT<T>;
-  using v4 = typename MF::v4;
-  constexpr int NB = Q::NB, KB = Q::KB, Np = Q::Np;
-#pragma unroll
-  for (int b = 0; b < 2 * NB; ++b) {
-    v4 acc = {0, 0, 0, 0};
-#pragma unroll
-    for (int t = 0; t < KB; ++t) acc = MF::mma(A1[b][t], U[t / 4][t % 4], acc);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      if (b < NB)
-        UR[b][r] = acc[r];
-      else
-        US[b - NB][r] = acc[r];
-    }
-  }
-#pragma unroll
-  for (int b = 0; b < NB; ++b)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int n = 16 * b + MF::crow(g, r);
-      if (n < Np) {
-        sl[n] = U[b][r];
-        sl[Np + n] = UR[b][r];
-        sl[2 * Np + n] = US[b][r];
-      }
-    }
-}
-
 // Tile handled by workgroup b of a G-workgroup launch.  Workgroups are
 // dispatched to the 8 XCDs round-robin (b % 8); each XCD gets one contiguous
 // run of tiles, so a tile's halo neighbours -- mostly the adjacent tiles of a
@@ -179,12 +142,13 @@ __device__ __forceinline__ int xcd_tile(int b, int G) {
   return x * q + (x < r ? x : r) + b / NX;
 }
 
-// Minimum waves per SIMD the register allocator is held to: 4 where the
-// operands fit in 128 registers; orders with two 16-row blocks (N >= 5)
+// Minimum waves per SIMD the register allocator is held to: N <= 4 fits 128
+// registers in float64 (4 waves) and 64 in float32 (8 waves, which the
+// float32 LDS footprint also allows); orders with two 16-row blocks (N >= 5)
 // need more registers per lane.
 template <typename T, int N>
 struct ApplyWaves {
-  static constexpr int value = N <= 4 ? 4 : (sizeof(T) == 8 ? 1 : 3);
+  static constexpr int value = N <= 4 ? (sizeof(T) == 8 ? 4 : 8) : (sizeof(T) == 8 ? 1 : 3);
 };
 
 // One 64-element tile per workgroup (grid = number of tiles).  HC is the
@@ -201,13 +165,12 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
   using MF = MfmaT<T>;
   using v4 = typename MF::v4;
   constexpr int Np = Q::Np, Nfp = Q::Nfp, NB = Q::NB, KB = Q::KB, FK = Q::FK;
-  constexpr int NS = Q::NS, NFP3 = Q::NFP3, FPW = Q::FPW, TRW = Q::TRW;
-  // LDS layout of GEMM 1's output: float32 keeps u, ur, us of every node
-  // (the face phase forms the normal derivatives); float64, whose records
-  // are twice as large, keeps only u and dn at the face nodes so four
-  // workgroups fit in a compute unit's LDS.
-  constexpr bool NODE = sizeof(T) == 4;
-  constexpr int SW = NODE ? NS : TRW;       // LDS words per element slot
+  constexpr int NFP3 = Q::NFP3, FPW = Q::FPW, TRW = Q::TRW;
+  // LDS layout of GEMM 1's output: u and dn at the face nodes only (the
+  // traces a neighbour reads), so four float64 / eight float32 workgroups
+  // fit in a compute unit's LDS -- eight float32 tiles per CU cover the 2048
+  // tiles of a 2x256x256 mesh in one round.
+  constexpr int SW = TRW;                   // LDS words per element slot
   constexpr bool HELM = MODE & M_HELM, UDIR = MODE & M_UDIR, DOT = MODE & M_DOT;
   constexpr bool RESID = MODE & M_RESID;
   constexpr int RECS = DDGX_REC_WORDS + 1;  // LDS stride of a record (odd: no bank conflicts)
@@ -273,13 +236,7 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
   // ---- GEMM 1: own elements and halo into LDS ----------------------------
   const T* re = s_rec + el * RECS;
   T UR[NB][4], US[NB][4];
-  if constexpr (NODE) {
-    grad_to_lds<T, N>(A1, U, UR, US, s_x + el * SW, g);
-    if (hwave) {
-      T HR[NB][4], HS[NB][4];
-      grad_to_lds<T, N>(A1, UH, HR, HS, s_x + (TILE + el) * SW, g);
-    }
-  } else {
+  {
     const FaceAB<T> ab{re[R_A], re[R_A + 1], re[R_A + 2], re[R_B], re[R_B + 1], re[R_B + 2]};
     grad_and_traces<T, N>(A1, U, UR, US, ab, s_x + el * SW, g);
     if (hwave) {
@@ -325,37 +282,17 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
       const T* rr = s_rec + e2 * RECS;
       const T* xm = s_x + e2 * SW;
       const T aM = rr[R_A + ff], bM = rr[R_B + ff];
-      T uM, dnM;
-      if constexpr (NODE) {
-        uM = xm[nM];
-        dnM = aM * xm[Np + nM] + bM * xm[2 * Np + nM];
-      } else {
-        uM = xm[fp * 2];
-        dnM = xm[fp * 2 + 1];
-      }
+      const T uM = xm[fp * 2], dnM = xm[fp * 2 + 1];
       const int c = s_fc[ff * TILE + e2];
       const int ty = fc_type(c);
       T uP, dnP;
       if (ty == FT_INTERIOR) {
         const int sl = fc_slot(c), fn = fc_face(c);
         const int pn = fc_rev(c) ? (Nfp - 1 - pp) : pp;
+        // the neighbour's trace; its dn was taken with its own outward normal
         const T* xp = s_x + sl * SW;
-        if constexpr (NODE) {
-          const int nP = fmask<N>(fn, pn);
-          T aP, bP;  // the neighbour's own face coefficients (its outward normal)
-          if (sl < TILE) {
-            aP = s_rec[sl * RECS + R_A + fn];
-            bP = s_rec[sl * RECS + R_B + fn];
-          } else {
-            aP = s_hab[fn * HC + sl - TILE];
-            bP = s_hab[(3 + fn) * HC + sl - TILE];
-          }
-          uP = xp[nP];
-          dnP = -(aP * xp[Np + nP] + bP * xp[2 * Np + nP]);
-        } else {
-          uP = xp[(fn * Nfp + pn) * 2];
-          dnP = -xp[(fn * Nfp + pn) * 2 + 1];
-        }
+        uP = xp[(fn * Nfp + pn) * 2];
+        dnP = -xp[(fn * Nfp + pn) * 2 + 1];
       } else if (ty == FT_DIRICHLET) {
         if constexpr (UDIR) {
           const int kk = k0 + e2;

@@ -249,6 +249,7 @@ int ddgx_op_create(int N, int64_t K, int dtype, const double* tables, const doub
         !put(&o.hpos, hpos, "hpos"))
       break;
     o.grid_apply = (int)ntiles;
+    o.grid_pipe = apply_pipelined(N, dtype, perm_id) ? apply_pipe_grid(N, dtype, hcap, ntiles) : 0;
     const int64_t nvec = K * o.Np;
     int64_t gv = (nvec + 255) / 256;
     const int64_t cap = 4 * (int64_t)device_cu_count();

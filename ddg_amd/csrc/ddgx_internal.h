@@ -150,7 +150,9 @@ struct Op {
   int* hpos = nullptr;   // ntiles x halo_cap tile positions of halo elements (-1: none)
   void* hab = nullptr;   // ntiles x 6 x halo_cap halo face coefficients a_f, b_f (dtype)
   void* fsj = nullptr;   // K x 4 Neumann face Jacobians, tile order (dtype)
-  int grid_apply = 0;    // workgroups of one apply: one per 64-element tile
+  int grid_apply = 0;    // workgroups of one apply: one per 64-element tile (= dot partials)
+  int grid_pipe = 0;     // grid of the pipelined kernel (sipg_apply_pipe.h) for the plain
+                         // and Helmholtz applies, or 0 when this operator does not use it
   int max_halo = 0;      // largest halo of a tile
   int halo_cap = 0;      // HALO_SMALL or HALO_MAX: halo entries stored per tile
   int grid_vec = 0;      // grid for vector kernels / reductions
@@ -255,6 +257,10 @@ int launch_apply(const Op& op, const void* u, void* y, const void* udir, double 
 // One fused block-Jacobi sweep x_out = x_in + omega Binv (b - (A + alpha M) x_in)
 // (x_out must not alias x_in); only for smooth_fused(op) operators.
 bool smooth_fused(const Op& op);
+// Whether the pipelined apply serves this order/dtype/tile order, and its
+// grid (workgroups resident at once, at most one per tile).
+bool apply_pipelined(int N, int dtype, bool perm_id);
+int apply_pipe_grid(int N, int dtype, int halo_cap, int64_t ntiles);
 int launch_smooth(const Op& op, const void* binv, double omega, double alpha, const void* x_in,
                   void* x_out, const void* b, const int* skip, hipStream_t s);
 int mg_vcycle(const MG& mg, MGWork& w, const void* b, void* x, const int* skip, hipStream_t s);

@@ -88,8 +88,8 @@ def test_rhs_parity(dev):
 
 def test_large_mesh_properties(dev):
     """2x256x256, N=4, jittered, Dirichlet x / Neumann y: exact agreement
-    with the numpy model of the same closed form, symmetry u.Av == v.Au,
-    and linearity."""
+    with the numpy model of the same closed form (plain, and Helmholtz with
+    Dirichlet data), symmetry u.Av == v.Au, and linearity."""
     VX, VY, EToV = pmesh.uniform_rectangle_mesh_2d(0, 1, 0, 1, 256, 256)
     VX, VY = pmesh.jitter_vertices(VX, VY, 1 / 256, 0.1, seed=0)
     m = pmesh.build_mesh_2d(VX, VY, EToV, 4)
@@ -109,6 +109,13 @@ def test_large_mesh_properties(dev):
     assert abs(a - b) <= 1e-11 * abs(a)
     w = op.apply(2.0 * ut - 3.0 * vt)
     assert float((w - (2.0 * Au - 3.0 * Av)).abs().max()) <= 1e-11 * float(Au.abs().max())
+    # Helmholtz with Dirichlet data (the float64 orders 2-4 run these on the
+    # pipelined kernel, several tiles per workgroup at this size)
+    ud = rng.standard_normal((m.K, m.Np))
+    yh = op.apply(ut, u_dir=torch.as_tensor(ud.reshape(-1), device=dev), alpha=2.5, shift=0.5)
+    refh = model_apply(4, ref_element(4).kernel_tables(), rec, nbr, u, udir=ud, alpha=2.5,
+                       shift=0.5)
+    assert rel(yh.cpu().numpy().reshape(m.K, m.Np), refh) < 1e-12
 
 
 def test_repeatable(dev):

@@ -243,7 +243,7 @@ def serve_block(n_requests, slots, method="pcg32", calib=200, load=0.8, sample=8
     print(f"[bench] serve burst: {cap}", file=sys.stderr, flush=True)
     rate = load * cap["solves_per_s"]
     SV.poisson_arrivals(reqs, rate, seed=seed + 2)
-    rep = srv.run(reqs)
+    rep = srv.run(reqs, progress_every=500)
     st = rep.stats()
     print(f"[bench] serve stream at {rate:.1f}/s: {st}", file=sys.stderr, flush=True)
     errs = []

@@ -31,6 +31,7 @@ Python and needs no GPU.
 from __future__ import annotations
 
 import ctypes as C
+import sys
 import threading
 import time
 from collections import deque
@@ -427,8 +428,10 @@ class SolveServer:
             pc.release(ctx)
         return x, info
 
-    def run(self, requests) -> ServeReport:
-        """Serve ``requests`` by their arrival times; blocks until all are done."""
+    def run(self, requests, progress_every: int = 0) -> ServeReport:
+        """Serve ``requests`` by their arrival times; blocks until all are done.
+        ``progress_every > 0`` prints a line to stderr after every that many
+        completed requests."""
         import queue
         import torch
         self.prepare(requests)
@@ -464,6 +467,10 @@ class SolveServer:
                 with cond:
                     if res is not None:
                         report.results.append(res)
+                        done = len(report.results)
+                        if progress_every > 0 and done % progress_every == 0:
+                            print(f"[serve] {done}/{len(requests)} done at {clock():.1f} s",
+                                  file=sys.stderr, flush=True)
                     adm.release(nb)
                     cond.notify_all()
 

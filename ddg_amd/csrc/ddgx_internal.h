@@ -107,7 +107,7 @@ constexpr int CODE_DIRICHLET = -1;
 constexpr int CODE_NONE = -2;
 
 // Kernel mode bits for the apply (M_SMOOTH: one block-Jacobi sweep
-// x_out = x + omega Binv (b - A x), order 1 only)
+// x_out = x + omega Binv (b - A x), launched through launch_smooth)
 enum : int { M_HELM = 1, M_UDIR = 2, M_DOT = 8, M_RESID = 16, M_SMOOTH = 32 };
 
 // Device scalars of one CG solve
@@ -255,7 +255,7 @@ int launch_apply(const Op& op, const void* u, void* y, const void* udir, double 
                  double shift, int mode, double* partial, const int* skip, hipStream_t s,
                  const void* bvec = nullptr);
 // One fused block-Jacobi sweep x_out = x_in + omega Binv (b - (A + alpha M) x_in)
-// (x_out must not alias x_in); only for smooth_fused(op) operators.
+// (x_out must not alias x_in; Binv packed, element order, the operator's dtype).
 bool smooth_fused(const Op& op);
 // Whether the pipelined apply serves this order/dtype/tile order, and its
 // grid (workgroups resident at once, at most one per tile).

@@ -27,14 +27,15 @@ namespace ddgx {
 template <typename T, int N, int MODE>
 static int launch_t(const Op& op, const void* u, void* y, const void* udir, double alpha,
                     double shift, double* partial, const int* skip, hipStream_t s,
-                    const void* bvec) {
+                    const void* bvec, const void* binv = nullptr, double omega = 0.0) {
   // one workgroup per 64-element tile; in dot mode each writes one partial
   const int* perm = op.perm_id ? nullptr : op.perm;
-  // the pipelined kernel serves the plain and Helmholtz applies; the dot and
-  // residual modes carry more per-lane state and spill under its 3-wave
-  // register budget (scratch traffic waits in order on the prefetch), so
-  // they keep the one-tile kernel
-  if constexpr (!UseValuApply<T, N>::value && MS<N>::NB == 1 && !(MODE & (M_DOT | M_RESID))) {
+  // the pipelined kernel serves the plain and Helmholtz applies; the dot,
+  // residual and smoothing modes carry more per-lane state and spill under
+  // its 3-wave register budget (scratch traffic waits in order on the
+  // prefetch), so they keep the one-tile kernel
+  if constexpr (!UseValuApply<T, N>::value && MS<N>::NB == 1 &&
+                !(MODE & (M_DOT | M_RESID | M_SMOOTH))) {
     if (op.grid_pipe > 0) {
       if (op.halo_cap == HALO_SMALL)
         hipLaunchKernelGGL((sipg_apply_pipe_kernel<T, N, MODE, HALO_SMALL>), dim3(op.grid_pipe),
@@ -54,19 +55,19 @@ static int launch_t(const Op& op, const void* u, void* y, const void* udir, doub
     hipLaunchKernelGGL((sipg_apply_valu_kernel<T, N, MODE>), dim3(op.grid_apply), dim3(TILE), 0,
                        s, (const T*)op.tab, (const T*)op.rec, op.nbrp, perm, (const T*)u, (T*)y,
                        (const T*)udir, (T)alpha, (T)shift, (int)op.K, partial, skip,
-                       (const T*)bvec, (const T*)nullptr, T(0));
+                       (const T*)bvec, (const T*)binv, (T)omega);
   else if (op.halo_cap == HALO_SMALL)
     hipLaunchKernelGGL((sipg_apply_mfma_kernel<T, N, MODE, HALO_SMALL>), dim3(op.grid_apply),
                        dim3(256), 0, s, (const T*)op.frag, (const T*)op.rec, op.fcode, perm,
                        op.hpos, (const T*)op.hab, (const T*)u, (T*)y, (const T*)udir, (T)alpha,
-                       (T)shift,
-                       (int)op.K, partial, skip, (const T*)bvec);
+                       (T)shift, (int)op.K, partial, skip, (const T*)bvec, (const T*)binv,
+                       (T)omega);
   else
     hipLaunchKernelGGL((sipg_apply_mfma_kernel<T, N, MODE, HALO_MAX>), dim3(op.grid_apply),
                        dim3(256), 0, s, (const T*)op.frag, (const T*)op.rec, op.fcode, perm,
                        op.hpos, (const T*)op.hab, (const T*)u, (T*)y, (const T*)udir, (T)alpha,
-                       (T)shift,
-                       (int)op.K, partial, skip, (const T*)bvec);
+                       (T)shift, (int)op.K, partial, skip, (const T*)bvec, (const T*)binv,
+                       (T)omega);
   DDGX_HIP(hipGetLastError());
   return DDGX_OK;
 }
@@ -153,43 +154,48 @@ int apply_pipe_grid(int N, int dtype, int halo_cap, int64_t ntiles) {
   return (int)(g < ntiles ? g : ntiles);
 }
 
-// Fused block-Jacobi sweep of the multigrid smoother on the vector-ALU
-// kernel (the one order that kernel serves): the residual never leaves
-// registers, and one launch replaces the residual apply and the sweep.
-bool smooth_fused(const Op& op) { return op.N == 1; }
+// Fused block-Jacobi sweep of the multigrid smoother: the apply's smoothing
+// mode keeps the residual on chip (registers at order 1, LDS rows on the
+// matrix cores), and one launch replaces the residual apply and the sweep.
+bool smooth_fused(const Op& op) { return op.N >= 1 && op.N <= 6; }
 
-template <typename T, int MODE>
-static int launch_smooth_t(const Op& op, const void* binv, double omega, double alpha,
-                           const void* x_in, void* x_out, const void* b, const int* skip,
-                           hipStream_t s) {
-  const int* perm = op.perm_id ? nullptr : op.perm;
-  hipLaunchKernelGGL((sipg_apply_valu_kernel<T, 1, MODE>), dim3(op.grid_apply), dim3(TILE), 0, s,
-                     (const T*)op.tab, (const T*)op.rec, op.nbrp, perm, (const T*)x_in, (T*)x_out,
-                     (const T*)nullptr, (T)alpha, T(0), (int)op.K, (double*)nullptr, skip,
-                     (const T*)b, (const T*)binv, (T)omega);
-  DDGX_HIP(hipGetLastError());
-  return DDGX_OK;
+template <typename T, int N>
+static int smooth_n(const Op& op, const void* binv, double omega, double alpha, const void* x_in,
+                    void* x_out, const void* b, const int* skip, hipStream_t s) {
+  if (alpha != 0.0)
+    return launch_t<T, N, M_SMOOTH | M_HELM>(op, x_in, x_out, nullptr, alpha, 0.0, nullptr, skip,
+                                             s, b, binv, omega);
+  return launch_t<T, N, M_SMOOTH>(op, x_in, x_out, nullptr, alpha, 0.0, nullptr, skip, s, b,
+                                  binv, omega);
 }
 
 int launch_smooth(const Op& op, const void* binv, double omega, double alpha, const void* x_in,
                   void* x_out, const void* b, const int* skip, hipStream_t s) {
   if (!smooth_fused(op)) {
-    set_error("fused smoothing sweep is built for order 1 only");
+    set_error("fused smoothing sweep: polynomial order not built");
     return DDGX_ERR_UNSUPPORTED;
   }
-  if (x_in == x_out) {
-    set_error("fused smoothing sweep: x_out must not alias x_in");
+  if (x_in == x_out || binv == nullptr || b == nullptr) {
+    set_error("fused smoothing sweep: x_out must not alias x_in; blocks and b are required");
     return DDGX_ERR_INVALID;
   }
-  const bool helm = alpha != 0.0;
-  if (op.dtype == DDGX_F64)
-    return helm ? launch_smooth_t<double, M_SMOOTH | M_HELM>(op, binv, omega, alpha, x_in, x_out,
-                                                              b, skip, s)
-                : launch_smooth_t<double, M_SMOOTH>(op, binv, omega, alpha, x_in, x_out, b,
-                                                     skip, s);
-  return helm ? launch_smooth_t<float, M_SMOOTH | M_HELM>(op, binv, omega, alpha, x_in, x_out, b,
-                                                           skip, s)
-              : launch_smooth_t<float, M_SMOOTH>(op, binv, omega, alpha, x_in, x_out, b, skip, s);
+#define SMOOTH_N(T)                                                                   \
+  switch (op.N) {                                                                     \
+    case 1: return smooth_n<T, 1>(op, binv, omega, alpha, x_in, x_out, b, skip, s);   \
+    case 2: return smooth_n<T, 2>(op, binv, omega, alpha, x_in, x_out, b, skip, s);   \
+    case 3: return smooth_n<T, 3>(op, binv, omega, alpha, x_in, x_out, b, skip, s);   \
+    case 4: return smooth_n<T, 4>(op, binv, omega, alpha, x_in, x_out, b, skip, s);   \
+    case 5: return smooth_n<T, 5>(op, binv, omega, alpha, x_in, x_out, b, skip, s);   \
+    case 6: return smooth_n<T, 6>(op, binv, omega, alpha, x_in, x_out, b, skip, s);   \
+  }
+  if (op.dtype == DDGX_F64) {
+    SMOOTH_N(double)
+  } else {
+    SMOOTH_N(float)
+  }
+#undef SMOOTH_N
+  set_error("fused smoothing sweep: polynomial order not built");
+  return DDGX_ERR_UNSUPPORTED;
 }
 
 // ---------------------------------------------------------------------------

@@ -160,7 +160,8 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
                        const int* __restrict__ hpos, const T* __restrict__ hab,
                        const T* __restrict__ u, T* __restrict__ y, const T* __restrict__ udir,
                        T alpha, T shift, int K, double* __restrict__ partial,
-                       const int* __restrict__ skip, const T* __restrict__ bvec) {
+                       const int* __restrict__ skip, const T* __restrict__ bvec,
+                       const T* __restrict__ binv, T omega) {
   using Q = MS<N>;
   using MF = MfmaT<T>;
   using v4 = typename MF::v4;
@@ -173,12 +174,16 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
   constexpr int SW = TRW;                   // LDS words per element slot
   constexpr bool HELM = MODE & M_HELM, UDIR = MODE & M_UDIR, DOT = MODE & M_DOT;
   constexpr bool RESID = MODE & M_RESID;
+  // smoothing sweep: y = u + omega Binv (bvec - A u) (the epilogue below)
+  constexpr bool SMOOTH = MODE & M_SMOOTH;
   constexpr int RECS = DDGX_REC_WORDS + 1;  // LDS stride of a record (odd: no bank conflicts)
   constexpr int XT = (TILE + HC) * SW;      // element slots: tile then halo
   constexpr int XF = TILE * Q::FS;          // face rows
   constexpr int XW = XT > XF ? XT : XF;
+  static_assert(!SMOOTH || 2 * TILE * Np <= XW, "smoothing rows fit the face-row buffer");
   // s_x holds GEMM 1's output for the tile and its halo during the face
-  // phase and, after a barrier, the face rows F that feed GEMM 2.
+  // phase and, after a barrier, the face rows F that feed GEMM 2 (and, in
+  // the smoothing epilogue, the residual and iterate rows).
   __shared__ T s_x[XW];
   __shared__ T s_rec[TILE * RECS];
   __shared__ T s_hab[6 * HC];
@@ -332,6 +337,7 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
   T Jm = T(0);
   if constexpr (HELM) Jm = alpha * re[R_J];
   double dacc = 0.0;
+  T vv[NB][4];  // A u, kept for the smoothing epilogue
 #pragma unroll
   for (int bo = 0; bo < NB; ++bo) {
     // three independent accumulator chains (volume r, volume s, faces) so
@@ -354,13 +360,49 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
       const int n = 16 * bo + MF::crow(g, r);
       T v = acc[r] + acs[r] + acf[r];
       if constexpr (HELM) v += shift * U[bo][r];
-      if (valid && n < Np) {
+      if constexpr (SMOOTH) {
+        vv[bo][r] = v;
+      } else if (valid && n < Np) {
         const size_t gi = (size_t)kg * Np + n;
         if constexpr (RESID)
           y[gi] = bvec[gi] - v;
         else
           y[gi] = v;
         if constexpr (DOT) dacc += (double)U[bo][r] * (double)v;
+      }
+    }
+  }
+
+  if constexpr (SMOOTH) {
+    // the residual r = b - A u and u, by element rows, into LDS; then one
+    // block-Jacobi row per lane (Np of NL lanes per element): the same
+    // arithmetic as a residual apply followed by bj_smooth_kernel
+    constexpr int NPK = SymPack<N>::NPK, NL = Shape<N>::NL, EPB = 256 / NL;
+    T* s_r = s_x;
+    T* s_u = s_x + TILE * Np;
+    __syncthreads();  // every wave is done reading the face rows
+#pragma unroll
+    for (int bo = 0; bo < NB; ++bo)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = 16 * bo + MF::crow(g, r);
+        if (valid && n < Np) {
+          s_r[el * Np + n] = bvec[(size_t)kg * Np + n] - vv[bo][r];
+          s_u[el * Np + n] = U[bo][r];
+        }
+      }
+    __syncthreads();
+    const int li = tid % NL, ep = tid / NL;
+    for (int e0 = 0; e0 < TILE; e0 += EPB) {
+      const int e2 = e0 + ep;
+      if (li < Np && e2 < ne) {
+        const int kk = k0 + e2;
+        const size_t k2 = (size_t)(perm ? perm[kk] : kk);
+        const T* bk = binv + k2 * NPK;
+        T a = 0;
+#pragma unroll
+        for (int j = 0; j < Np; ++j) a += bk[SymPack<N>::pk(li, j)] * s_r[e2 * Np + j];
+        y[k2 * Np + li] = s_u[e2 * Np + li] + omega * a;
       }
     }
   }

@@ -114,21 +114,32 @@ copy_kernel(const T* __restrict__ a, T* __restrict__ b, size_t n) {
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) b[i] = a[i];
 }
 
-// x += alpha p ; r -= alpha Ap   (+ partial <r,r> when no preconditioner)
+// Float32 preconditioner inside a float64 CG: r32 = (float)(r / sqrt(rr)) and
+// z = sqrt(rr) * (double) z32, with rr the last <r,r> on the device (1 before
+// the first).  The preconditioner is linear, so the scale only keeps the
+// float32 values away from the ends of their range.
+__device__ __forceinline__ double pc_scale(const CGScalars* sc) {
+  return sc->rr > 0.0 ? sqrt(sc->rr) : 1.0;
+}
+
+// x += alpha p ; r -= alpha Ap   (+ partial <r,r> when no preconditioner;
+// + the scaled float32 copy of r for a float32 preconditioner, r32 != null)
 template <typename T>
 __global__ void __launch_bounds__(256)
 update_kernel(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
               const T* __restrict__ Ap, const CGScalars* __restrict__ sc, size_t n,
-              double* __restrict__ partial) {
+              double* __restrict__ partial, float* __restrict__ r32) {
   if (sc->done) return;
   __shared__ double s_red[4];
   const T alpha = (T)sc->alpha;
+  const double inv = r32 != nullptr ? 1.0 / pc_scale(sc) : 0.0;
   double acc = 0.0;
   const size_t stride = (size_t)gridDim.x * 256;
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
     x[i] += alpha * p[i];
     const T ri = r[i] - alpha * Ap[i];
     r[i] = ri;
+    if (r32 != nullptr) r32[i] = (float)(inv * (double)ri);
     acc += (double)ri * (double)ri;
   }
   if (partial != nullptr) {
@@ -206,14 +217,6 @@ bj_apply_kernel(const T* __restrict__ binv, const T* __restrict__ r, T* __restri
   }
 }
 
-// Float32 preconditioner inside a float64 CG: r32 = (float)(r / sqrt(rr)) and
-// z = sqrt(rr) * (double) z32, with rr the last <r,r> on the device (1 before
-// the first).  The preconditioner is linear, so the scale only keeps the
-// float32 values away from the ends of their range.
-__device__ __forceinline__ double pc_scale(const CGScalars* sc) {
-  return sc->rr > 0.0 ? sqrt(sc->rr) : 1.0;
-}
-
 __global__ void __launch_bounds__(256)
 pc_down_kernel(const double* __restrict__ r, float* __restrict__ r32,
                const CGScalars* __restrict__ sc, size_t n, const int* __restrict__ skip) {
@@ -232,6 +235,33 @@ pc_up_kernel(const float* __restrict__ z32, double* __restrict__ z,
   const size_t stride = (size_t)gridDim.x * 256;
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
     z[i] = sc_ * (double)z32[i];
+}
+
+// z = sqrt(rr) * (double) z32 with the partials of <r,z> and <r,r> (the
+// pc_up_kernel + dot2_kernel pair of one CG iteration in one pass, same
+// arithmetic and reduction order)
+__global__ void __launch_bounds__(256)
+pc_up_dot_kernel(const float* __restrict__ z32, double* __restrict__ z,
+                 const double* __restrict__ r, const CGScalars* __restrict__ sc, size_t n,
+                 double* __restrict__ partial, const int* __restrict__ skip) {
+  if (skip != nullptr && *skip) return;
+  __shared__ double s_red[4];
+  const double sc_ = pc_scale(sc);
+  double s0 = 0.0, s1 = 0.0;
+  const size_t stride = (size_t)gridDim.x * 256;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+    const double zi = sc_ * (double)z32[i];
+    z[i] = zi;
+    const double ri = r[i];
+    s0 += ri * zi;
+    s1 += ri * ri;
+  }
+  s0 = block_sum_256(s0, s_red);
+  s1 = block_sum_256(s1, s_red);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = s0;
+    partial[gridDim.x + blockIdx.x] = s1;
+  }
 }
 
 // p = z + beta p
@@ -499,11 +529,20 @@ static int pcg_t(const Op& op, Work& w, double alpha, double shift, const PC* pc
     if (bj) {
       CHECK(update_bj_dispatch<T>(op, w, x, pc->binv, s));
     } else {
+      // with a float32 multigrid the update also writes the scaled float32
+      // residual, and the conversion back is fused with the dot products
       hipLaunchKernelGGL((update_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (T*)x,
                          (T*)w.r, (const T*)w.p, (const T*)w.Ap,
-                         (const CGScalars*)w.scal, n, has_pc ? nullptr : w.partials);
+                         (const CGScalars*)w.scal, n, has_pc ? nullptr : w.partials,
+                         mixed ? (float*)w.pc_r : nullptr);
       DDGX_HIP(hipGetLastError());
-      if (has_pc) {
+      if (mixed) {
+        CHECK(mg_vcycle(*pc->mg, *mgw, w.pc_r, w.pc_z, done, s));
+        hipLaunchKernelGGL(pc_up_dot_kernel, dim3(op.grid_vec), dim3(256), 0, s,
+                           (const float*)w.pc_z, (double*)w.z, (const double*)w.r,
+                           (const CGScalars*)w.scal, n, w.partials, done);
+        DDGX_HIP(hipGetLastError());
+      } else if (has_pc) {
         CHECK(precondition(done));
         CHECK(dot2<T>(op, w, w.r, w.z, w.r, w.r, done, s));
       }

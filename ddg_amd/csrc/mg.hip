@@ -63,14 +63,24 @@ bj_smooth_kernel(const T* __restrict__ binv, const T* __restrict__ r, T* __restr
   }
 }
 
-// lo[k][j] = sum_h P[h][j] hi[k][h]
+// Entry (i, j) of a packed symmetric n x n block (SymPack at run-time n)
+__device__ __forceinline__ int sym_pk(int n, int i, int j) {
+  return i <= j ? i * n - i * (i - 1) / 2 + (j - i) : j * n - j * (j - 1) / 2 + (i - j);
+}
+
+// lo[k][j] = sum_h P[h][j] hi[k][h]; with x_c != null also the coarse level's
+// first smoothing sweep x_c[k] = omega Binv_c[k] lo[k] (the arithmetic of
+// bj_smooth_kernel without accumulation), so that level skips its launch
 template <typename T>
 __global__ void __launch_bounds__(256)
 restrict_kernel(const T* __restrict__ P, const T* __restrict__ hi, T* __restrict__ lo, int Ku,
-                int nhi, int nlo, const int* __restrict__ skip) {
+                int nhi, int nlo, const int* __restrict__ skip, const T* __restrict__ binv_c,
+                T omega, T* __restrict__ x_c) {
   if (skip != nullptr && *skip) return;
   __shared__ T sP[MG_MAXP];
   __shared__ T sIn[MG_MAXIN];
+  __shared__ T sOut[256];
+  const int npk = nlo * (nlo + 1) / 2;
   for (int i = threadIdx.x; i < nhi * nlo; i += 256) sP[i] = P[i];
   const int E = 256 / nlo;
   const int e = threadIdx.x / nlo, j = threadIdx.x % nlo;
@@ -79,10 +89,20 @@ restrict_kernel(const T* __restrict__ P, const T* __restrict__ hi, T* __restrict
     __syncthreads();
     for (int i = threadIdx.x; i < ne * nhi; i += 256) sIn[i] = hi[(size_t)k0 * nhi + i];
     __syncthreads();
+    T acc = 0;
     if (e < ne) {
-      T acc = 0;
       for (int h = 0; h < nhi; ++h) acc += sP[h * nlo + j] * sIn[e * nhi + h];
       lo[(size_t)(k0 + e) * nlo + j] = acc;
+    }
+    if (x_c != nullptr) {
+      if (e < E) sOut[e * nlo + j] = acc;
+      __syncthreads();
+      if (e < ne) {
+        const T* bk = binv_c + (size_t)(k0 + e) * npk;
+        T a = 0;
+        for (int i = 0; i < nlo; ++i) a += bk[sym_pk(nlo, j, i)] * sOut[e * nlo + i];
+        x_c[(size_t)(k0 + e) * nlo + j] = omega * a;
+      }
     }
   }
 }
@@ -285,9 +305,11 @@ static int coarse_solve(const MG& mg, MGWork& w, const void* b, void* x, const i
   return DDGX_OK;
 }
 
+// prolong: dst += P src; restrict: dst = P^T src, and with x_c != null the
+// coarse level's first sweep x_c = omega Binv_c dst
 template <typename T>
 static int transfer(const MGLevel& L, const MGLevel& C, bool prolong, const void* src, void* dst,
-                    const int* skip, hipStream_t s) {
+                    const int* skip, hipStream_t s, void* x_c = nullptr, double omega = 0.0) {
   const int Ku = (int)C.op->K;
   const int nlo = C.op->Np;
   const int nhi = (int)((L.op->K * L.op->Np) / C.op->K);
@@ -300,14 +322,17 @@ static int transfer(const MGLevel& L, const MGLevel& C, bool prolong, const void
                        (const T*)src, (T*)dst, Ku, nhi, nlo, skip);
   else
     hipLaunchKernelGGL((restrict_kernel<T>), dim3(grid), dim3(256), 0, s, (const T*)L.P,
-                       (const T*)src, (T*)dst, Ku, nhi, nlo, skip);
+                       (const T*)src, (T*)dst, Ku, nhi, nlo, skip, (const T*)C.binv, (T)omega,
+                       (T*)x_c);
   DDGX_HIP(hipGetLastError());
   return DDGX_OK;
 }
 
+// V-cycle on level l; first_done: the level's first sweep (omega Binv b) is
+// already in its first-sweep buffer (written by the parent's restriction)
 template <typename T>
 static int vcycle(const MG& mg, MGWork& w, int l, const void* b, void* x, const int* skip,
-                  hipStream_t s) {
+                  hipStream_t s, bool first_done = false) {
   if (l == mg.nlev - 1) return coarse_solve<T>(mg, w, b, x, skip, s);
   const MGLevel& L = mg.lv[l];
   const MGLevel& C = mg.lv[l + 1];
@@ -326,14 +351,17 @@ static int vcycle(const MG& mg, MGWork& w, int l, const void* b, void* x, const 
     // its sweeps in pairs x -> Lr -> x (an odd last one as residual + sweep).
     void* bufs[2] = {x, Lr};
     int at = (mg.nu - 1) % 2;  // buffer of the first sweep's output
-    MG_CHECK(bj_smooth<T>(L, b, bufs[at], mg.omega, 0, skip, s));
+    if (!first_done) MG_CHECK(bj_smooth<T>(L, b, bufs[at], mg.omega, 0, skip, s));
     for (int i = 1; i < mg.nu; ++i) {
       MG_CHECK(launch_smooth(op, L.binv, mg.omega, mg.alpha, bufs[at], bufs[1 - at], b, skip, s));
       at = 1 - at;
     }
     MG_CHECK(resid());
-    MG_CHECK(transfer<T>(L, C, false, Lr, Cb, skip, s));
-    MG_CHECK(vcycle<T>(mg, w, l + 1, Cb, Cx, skip, s));
+    // a smoothed coarser level gets its first sweep from the restriction
+    const bool fuse_first = l + 2 < mg.nlev && smooth_fused(*C.op);
+    void* c_first = (mg.nu - 1) % 2 ? w.r[l + 1] : Cx;
+    MG_CHECK(transfer<T>(L, C, false, Lr, Cb, skip, s, fuse_first ? c_first : nullptr, mg.omega));
+    MG_CHECK(vcycle<T>(mg, w, l + 1, Cb, Cx, skip, s, fuse_first));
     MG_CHECK(transfer<T>(L, C, true, Cx, x, skip, s));
     for (int i = 0; i + 1 < mg.nu; i += 2) {
       MG_CHECK(launch_smooth(op, L.binv, mg.omega, mg.alpha, x, Lr, b, skip, s));

@@ -188,5 +188,14 @@ def test_solve_with_hpmg_preconditioner(dev):
     ofin = om.build_mesh_2d(finest.VX, finest.VY, finest.EToV, 4)
     uo = S.solve_sipg_poisson_2d(ofin, f, bc_types=bc, u_dir=ue, q_neu=q)
     assert rel(u, uo) < 1e-7
+    # the same hierarchy in float32 preconditions the float64 drop-in solve
+    # (mixed-precision preconditioning): same tolerance, about the same count
+    mg32, _ = MG.make_sipg_hpmg_preconditioner(base, 2, bc_fn=pmesh.dirichlet_x_neumann_y,
+                                               degrees=[4, 2, 1], dtype=torch.float32)
+    u32, info32 = D.solve_sipg_poisson_2d(finest, f, bc_types=bc, u_dir=ue, q_neu=q,
+                                          iterative=D.IterativeConfig(tol=1e-10, maxiter=500),
+                                          preconditioner=mg32, return_info=True)
+    assert info32["converged"] and info32["iterations"] <= info["iterations"] + 3
+    assert rel(u32, uo) < 1e-7
     # discretisation error against the manufactured solution
     assert np.sqrt(np.sum(finest.J * (u - ue) ** 2) / np.sum(finest.J * ue ** 2)) < 1e-3

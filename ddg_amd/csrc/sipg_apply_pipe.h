@@ -270,15 +270,21 @@ sipg_apply_pipe_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
     // ---- GEMM 2: volume + face lift (+ mass) -----------------------------
     T Jm = T(0);
     if constexpr (HELM) Jm = alpha * re[R_J];
-    v4 acc = {0, 0, 0, 0}, acs = {0, 0, 0, 0}, acf = {0, 0, 0, 0};
+    // four independent accumulator chains (volume r, volume s, the face
+    // rows split in two) so consecutive matrix-core products rarely wait
+    v4 acc = {0, 0, 0, 0}, acs = {0, 0, 0, 0}, acf = {0, 0, 0, 0}, acg = {0, 0, 0, 0};
 #pragma unroll
     for (int t = 0; t < KB; ++t) {
       acc = MF::mma(s_fr[(Q::f2 + t) * 64 + lane], UR[0][t % 4], acc);
       acs = MF::mma(s_fr[(Q::f2 + KB + t) * 64 + lane], US[0][t % 4], acs);
     }
 #pragma unroll
-    for (int t = 0; t < FK; ++t)
-      acf = MF::mma(s_fr[(Q::fF + t) * 64 + lane], s_x[el * Q::FS + 4 * t + g], acf);
+    for (int t = 0; t < FK; ++t) {
+      if (t % 2 == 0)
+        acf = MF::mma(s_fr[(Q::fF + t) * 64 + lane], s_x[el * Q::FS + 4 * t + g], acf);
+      else
+        acg = MF::mma(s_fr[(Q::fF + t) * 64 + lane], s_x[el * Q::FS + 4 * t + g], acg);
+    }
     if constexpr (HELM) {
 #pragma unroll
       for (int t = 0; t < KB; ++t)
@@ -287,7 +293,7 @@ sipg_apply_pipe_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int n = MF::crow(g, r);
-      T v = acc[r] + acs[r] + acf[r];
+      T v = acc[r] + acs[r] + (acf[r] + acg[r]);
       if constexpr (HELM) v += shift * U[0][r];
       if (valid && n < Np) {
         const size_t gi = (size_t)k * Np + n;

@@ -43,6 +43,24 @@ def test_pmg_vcycle_matches_oracle(dev, N, alpha, bcname):
     assert rel(z, ref(r)) < 1e-10
 
 
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-10), (torch.float32, 1e-4)])
+def test_vcycle_permuted_tile_order(dev, dtype, tol):
+    """Row-major 24x24 mesh: its tiles are processed in a Morton order (a
+    non-identity element permutation), so the fused smoothing sweeps of
+    every level (matrix cores at N=3, 2; vector ALUs at N=1) read and write
+    through the permutation.  V-cycle against the oracle's (float32: the
+    float32 cycle's round-off on a unit residual)."""
+    pm, omesh = unit_square(24, 24, 3, jitter=0.1, seed=7)
+    bc = pmesh.dirichlet_x_neumann_y(pm)
+    _say(f"pmg vcycle, permuted tile order, {dtype}")
+    mg = MG.make_sipg_mg_preconditioner(pm, bc_types=bc, alpha=3.0, dtype=dtype)
+    assert D.get_operator(pm, bc, dtype=dtype).order is not None
+    ref = omg.make_sipg_mg_preconditioner(omesh, bc_types=bc, alpha=3.0)
+    r = np.random.default_rng(8).standard_normal(pm.K * pm.Np)
+    z = mg(torch.as_tensor(r, device=dev).to(dtype)).double().cpu().numpy()
+    assert rel(z, ref(r)) < tol
+
+
 def test_pmg_custom_degrees_and_smoother(dev):
     pm, omesh = unit_square(6, 5, 4, jitter=0.1, seed=6)
     bc = _all_dir(pm)

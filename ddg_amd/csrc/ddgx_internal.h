@@ -107,8 +107,10 @@ constexpr int CODE_DIRICHLET = -1;
 constexpr int CODE_NONE = -2;
 
 // Kernel mode bits for the apply (M_SMOOTH: one block-Jacobi sweep
-// x_out = x + omega Binv (b - A x), launched through launch_smooth)
-enum : int { M_HELM = 1, M_UDIR = 2, M_DOT = 8, M_RESID = 16, M_SMOOTH = 32 };
+// x_out = x + omega Binv (b - A x), launched through launch_smooth;
+// M_PROLONG: with x = u + the h-prolongation of a coarser vector, formed on
+// the fly, order 1 only)
+enum : int { M_HELM = 1, M_UDIR = 2, M_DOT = 8, M_RESID = 16, M_SMOOTH = 32, M_PROLONG = 64 };
 
 // Device scalars of one CG solve
 struct CGScalars {
@@ -262,7 +264,11 @@ bool smooth_fused(const Op& op);
 bool apply_pipelined(int N, int dtype, bool perm_id);
 int apply_pipe_grid(int N, int dtype, int halo_cap, int64_t ntiles);
 int launch_smooth(const Op& op, const void* binv, double omega, double alpha, const void* x_in,
-                  void* x_out, const void* b, const int* skip, hipStream_t s);
+                  void* x_out, const void* b, const int* skip, hipStream_t s,
+                  const void* prolong_from = nullptr, const void* prolong_mat = nullptr);
+// Whether launch_smooth can take the h-prolongation of a coarser level as part
+// of its input (prolong_from / prolong_mat) on this operator
+bool smooth_prolongs(const Op& op);
 int mg_vcycle(const MG& mg, MGWork& w, const void* b, void* x, const int* skip, hipStream_t s);
 int mg_create(int nlev, Op* const* ops, Op* const* ops64, const int* tkind,
               const double* const* P, const ddgx_mg_cfg& cfg, hipStream_t s, MG** out);

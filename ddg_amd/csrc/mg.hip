@@ -362,9 +362,14 @@ static int vcycle(const MG& mg, MGWork& w, int l, const void* b, void* x, const 
     void* c_first = (mg.nu - 1) % 2 ? w.r[l + 1] : Cx;
     MG_CHECK(transfer<T>(L, C, false, Lr, Cb, skip, s, fuse_first ? c_first : nullptr, mg.omega));
     MG_CHECK(vcycle<T>(mg, w, l + 1, Cb, Cx, skip, s, fuse_first));
-    MG_CHECK(transfer<T>(L, C, true, Cx, x, skip, s));
+    // the h-prolongation into an order-1 level is formed inside the first
+    // post-smoothing sweep (which never stores the corrected iterate)
+    const bool fuse_prolong = mg.nu >= 2 && L.tkind == 1 && smooth_prolongs(op);
+    if (!fuse_prolong) MG_CHECK(transfer<T>(L, C, true, Cx, x, skip, s));
     for (int i = 0; i + 1 < mg.nu; i += 2) {
-      MG_CHECK(launch_smooth(op, L.binv, mg.omega, mg.alpha, x, Lr, b, skip, s));
+      const bool pf = fuse_prolong && i == 0;
+      MG_CHECK(launch_smooth(op, L.binv, mg.omega, mg.alpha, x, Lr, b, skip, s,
+                             pf ? Cx : nullptr, pf ? L.P : nullptr));
       MG_CHECK(launch_smooth(op, L.binv, mg.omega, mg.alpha, Lr, x, b, skip, s));
     }
     if (mg.nu % 2) {

@@ -27,7 +27,8 @@ namespace ddgx {
 template <typename T, int N, int MODE>
 static int launch_t(const Op& op, const void* u, void* y, const void* udir, double alpha,
                     double shift, double* partial, const int* skip, hipStream_t s,
-                    const void* bvec, const void* binv = nullptr, double omega = 0.0) {
+                    const void* bvec, const void* binv = nullptr, double omega = 0.0,
+                    const void* pxc = nullptr, const void* pmat = nullptr) {
   // one workgroup per 64-element tile; in dot mode each writes one partial
   const int* perm = op.perm_id ? nullptr : op.perm;
   // the pipelined kernel serves the plain and Helmholtz applies; the dot,
@@ -55,7 +56,7 @@ static int launch_t(const Op& op, const void* u, void* y, const void* udir, doub
     hipLaunchKernelGGL((sipg_apply_valu_kernel<T, N, MODE>), dim3(op.grid_apply), dim3(TILE), 0,
                        s, (const T*)op.tab, (const T*)op.rec, op.nbrp, perm, (const T*)u, (T*)y,
                        (const T*)udir, (T)alpha, (T)shift, (int)op.K, partial, skip,
-                       (const T*)bvec, (const T*)binv, (T)omega);
+                       (const T*)bvec, (const T*)binv, (T)omega, (const T*)pxc, (const T*)pmat);
   else if (op.halo_cap == HALO_SMALL)
     hipLaunchKernelGGL((sipg_apply_mfma_kernel<T, N, MODE, HALO_SMALL>), dim3(op.grid_apply),
                        dim3(256), 0, s, (const T*)op.frag, (const T*)op.rec, op.fcode, perm,
@@ -159,9 +160,22 @@ int apply_pipe_grid(int N, int dtype, int halo_cap, int64_t ntiles) {
 // matrix cores), and one launch replaces the residual apply and the sweep.
 bool smooth_fused(const Op& op) { return op.N >= 1 && op.N <= 6; }
 
+bool smooth_prolongs(const Op& op) { return op.N == 1; }
+
 template <typename T, int N>
 static int smooth_n(const Op& op, const void* binv, double omega, double alpha, const void* x_in,
-                    void* x_out, const void* b, const int* skip, hipStream_t s) {
+                    void* x_out, const void* b, const int* skip, hipStream_t s, const void* pxc,
+                    const void* pmat) {
+  if constexpr (N == 1) {
+    if (pxc != nullptr) {
+      if (alpha != 0.0)
+        return launch_t<T, N, M_SMOOTH | M_HELM | M_PROLONG>(op, x_in, x_out, nullptr, alpha, 0.0,
+                                                             nullptr, skip, s, b, binv, omega,
+                                                             pxc, pmat);
+      return launch_t<T, N, M_SMOOTH | M_PROLONG>(op, x_in, x_out, nullptr, alpha, 0.0, nullptr,
+                                                  skip, s, b, binv, omega, pxc, pmat);
+    }
+  }
   if (alpha != 0.0)
     return launch_t<T, N, M_SMOOTH | M_HELM>(op, x_in, x_out, nullptr, alpha, 0.0, nullptr, skip,
                                              s, b, binv, omega);
@@ -170,7 +184,12 @@ static int smooth_n(const Op& op, const void* binv, double omega, double alpha, 
 }
 
 int launch_smooth(const Op& op, const void* binv, double omega, double alpha, const void* x_in,
-                  void* x_out, const void* b, const int* skip, hipStream_t s) {
+                  void* x_out, const void* b, const int* skip, hipStream_t s,
+                  const void* prolong_from, const void* prolong_mat) {
+  if (prolong_from != nullptr && (!smooth_prolongs(op) || prolong_mat == nullptr)) {
+    set_error("fused smoothing sweep: prolongation on the fly is built for order 1 only");
+    return DDGX_ERR_UNSUPPORTED;
+  }
   if (!smooth_fused(op)) {
     set_error("fused smoothing sweep: polynomial order not built");
     return DDGX_ERR_UNSUPPORTED;
@@ -181,13 +200,15 @@ int launch_smooth(const Op& op, const void* binv, double omega, double alpha, co
   }
 #define SMOOTH_N(T)                                                                   \
   switch (op.N) {                                                                     \
-    case 1: return smooth_n<T, 1>(op, binv, omega, alpha, x_in, x_out, b, skip, s);   \
-    case 2: return smooth_n<T, 2>(op, binv, omega, alpha, x_in, x_out, b, skip, s);   \
-    case 3: return smooth_n<T, 3>(op, binv, omega, alpha, x_in, x_out, b, skip, s);   \
-    case 4: return smooth_n<T, 4>(op, binv, omega, alpha, x_in, x_out, b, skip, s);   \
-    case 5: return smooth_n<T, 5>(op, binv, omega, alpha, x_in, x_out, b, skip, s);   \
-    case 6: return smooth_n<T, 6>(op, binv, omega, alpha, x_in, x_out, b, skip, s);   \
+    case 1: return smooth_n<T, 1>(op, binv, omega, alpha, x_in, x_out, b, skip, s, PF, PM); \
+    case 2: return smooth_n<T, 2>(op, binv, omega, alpha, x_in, x_out, b, skip, s, PF, PM); \
+    case 3: return smooth_n<T, 3>(op, binv, omega, alpha, x_in, x_out, b, skip, s, PF, PM); \
+    case 4: return smooth_n<T, 4>(op, binv, omega, alpha, x_in, x_out, b, skip, s, PF, PM); \
+    case 5: return smooth_n<T, 5>(op, binv, omega, alpha, x_in, x_out, b, skip, s, PF, PM); \
+    case 6: return smooth_n<T, 6>(op, binv, omega, alpha, x_in, x_out, b, skip, s, PF, PM); \
   }
+  const void* PF = prolong_from;
+  const void* PM = prolong_mat;
   if (op.dtype == DDGX_F64) {
     SMOOTH_N(double)
   } else {

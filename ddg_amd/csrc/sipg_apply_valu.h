@@ -38,13 +38,30 @@ sipg_apply_valu_kernel(const T* __restrict__ tab, const T* __restrict__ rec,
                        const T* __restrict__ u, T* __restrict__ y, const T* __restrict__ udir,
                        T alpha, T shift, int K, double* __restrict__ partial,
                        const int* __restrict__ skip, const T* __restrict__ bvec,
-                       const T* __restrict__ binv, T omega) {
+                       const T* __restrict__ binv, T omega, const T* __restrict__ pxc,
+                       const T* __restrict__ pmat) {
   using S = Shape<N>;
   constexpr int Np = S::Np, Nfp = S::Nfp, RS = S::RS, RSF = S::RSF;
   constexpr bool HELM = MODE & M_HELM, UDIR = MODE & M_UDIR, DOT = MODE & M_DOT;
   constexpr bool RESID = MODE & M_RESID;
   // smoothing sweep: y = u + omega Binv (bvec - A u), element by element
   constexpr bool SMOOTH = MODE & M_SMOOTH;
+  // the input is u + the prolongation of the coarser level's pxc (element
+  // 4k+c is child c of coarse element k; pmat holds the 4 child matrices,
+  // rows c*Np + i), formed on the fly instead of stored
+  constexpr bool PROLONG = MODE & M_PROLONG;
+  auto uval = [&](size_t kid, int j) -> T {
+    T v = u[kid * Np + j];
+    if constexpr (PROLONG) {
+      const size_t kc = kid >> 2;
+      const int c = (int)(kid & 3);
+      T acc = 0;
+#pragma unroll
+      for (int l = 0; l < Np; ++l) acc += pmat[(c * Np + j) * Np + l] * pxc[kc * Np + l];
+      v += acc;
+    }
+    return v;
+  };
   const T* __restrict__ Dr = tab + S::oDr;
   const T* __restrict__ Ds = tab + S::oDs;
   const T* __restrict__ Ar = tab + S::oAr;
@@ -66,7 +83,10 @@ sipg_apply_valu_kernel(const T* __restrict__ tab, const T* __restrict__ rec,
 
   // the tile's nodal values into LDS (one contiguous range when the tile
   // order is the element order)
-  if (perm == nullptr) {
+  if constexpr (PROLONG) {
+    for (int i = el; i < ne * Np; i += TILE)
+      s_u[i] = uval((size_t)(perm ? perm[k0 + i / Np] : k0 + i / Np), i % Np);
+  } else if (perm == nullptr) {
     const T* src = u + (size_t)k0 * Np;
     for (int i = el; i < ne * Np; i += TILE) s_u[i] = src[i];
   } else {
@@ -139,9 +159,9 @@ sipg_apply_valu_kernel(const T* __restrict__ tab, const T* __restrict__ rec,
 #pragma unroll
           for (int j = 0; j < Np; ++j) un[j] = s_u[(np - k0) * Np + j];
         } else {
-          const size_t ng = (size_t)(perm ? perm[np] : np) * Np;
+          const size_t ng = (size_t)(perm ? perm[np] : np);
 #pragma unroll
-          for (int j = 0; j < Np; ++j) un[j] = u[ng + j];
+          for (int j = 0; j < Np; ++j) un[j] = uval(ng, j);
         }
         const T aP = rec[(size_t)(R_A + fn) * K + np], bP = rec[(size_t)(R_B + fn) * K + np];
         T urn[Np], usn[Np];

@@ -19,6 +19,8 @@ Also reported, on the same mesh:
 * both solvers on seeded random Fourier right-hand sides (64 modes,
   amplitude |k|^-2) on this mesh and on 2x128x128 at N=6, with the error
   against a float64 solve to tol 1e-13;
+* the reference's CPU-runnable case (2x16x16, N=3, block-Jacobi CG to
+  1e-9) through the drop-in, beside the oracle's same solve on the host;
 * the CPU baseline: the float64 numpy restatement of the reference
   algorithm (oracle/) timed on the host cores.
 
@@ -268,6 +270,51 @@ def serve_block(n_requests, slots, method="pcg32", calib=200, load=0.8, sample=8
     return out
 
 
+def config0_block(n_solves, cpu):
+    """The reference's CPU-runnable case: 2x16x16, N=3, all-wall Dirichlet,
+    f = 2 pi^2 sin(pi x) sin(pi y), tau_scale = 200, block-Jacobi CG to tol
+    1e-9.  GPU: the numpy-in / numpy-out drop-in solve (preconditioner set-up
+    excluded).  With ``cpu``: the oracle's same block-Jacobi CG timed on the
+    host cores, and the GPU solution's error against the oracle's LU solve."""
+    from ddg_amd import mesh as pm
+    from ddg_amd import sipg as D
+    VX, VY, EToV = pm.uniform_rectangle_mesh_2d(0.0, 1.0, 0.0, 1.0, 16, 16)
+    m0 = pm.build_mesh_2d(VX, VY, EToV, 3)
+    bc0 = pm.all_dirichlet(m0)
+    f = 2 * np.pi ** 2 * np.sin(np.pi * m0.x) * np.sin(np.pi * m0.y)
+    pc = D.make_sipg_block_jacobi_preconditioner(m0, bc_types=bc0, matfree=True)
+    # maxiter 2000: the solve needs 1208 iterations (the reference's default
+    # cap of 1000 stops it unconverged, 9e-10 from the LU solution)
+    it = D.IterativeConfig(tol=1e-9, maxiter=2000)
+
+    def gpu_solve():
+        return D.solve_sipg_poisson_2d(m0, f, bc_types=bc0, preconditioner=pc, iterative=it,
+                                       return_info=True)
+    gpu_solve()
+    lat, (u, info) = timed_solves(gpu_solve, n_solves)
+    out = {"workload": "2x16x16, N=3, Dirichlet walls, tau_scale=200, block-Jacobi CG, tol 1e-9, "
+                       "maxiter 2000",
+           "gpu": dict(lat_stats(lat), iterations=info["iterations"],
+                       converged=info["converged"],
+                       note="drop-in call, numpy in and out (host transfers included)")}
+    if cpu:
+        from oracle import mesh as om
+        from oracle import sipg as S
+        mo = om.build_mesh_2d(VX, VY, EToV, 3)
+        opc = S.make_block_jacobi_preconditioner(mo, bc_types=bc0)
+        t0 = time.perf_counter()
+        uo, io = S.solve_sipg_poisson_2d(mo, f, bc_types=bc0, preconditioner=opc,
+                                         iterative=it, return_info=True)
+        t_cpu = time.perf_counter() - t0
+        ulu = S.solve_sipg_poisson_2d(mo, f, bc_types=bc0)
+        out["cpu_oracle"] = {"seconds_per_solve": round(t_cpu, 4), "iterations": io["iterations"],
+                             "cores": os.cpu_count(), "kind": "port"}
+        out["gpu_rel_err_vs_oracle_lu"] = float(np.linalg.norm(u - ulu) / np.linalg.norm(ulu))
+        out["speedup_gpu_vs_cpu_oracle"] = round(t_cpu / out["gpu"]["p50_ms"] * 1e3, 1)
+    print(f"[bench] config0: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def read_traffic(name):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
     WRITE_SIZE, scaled by the same counters on a copy of known size), kept in
@@ -449,6 +496,8 @@ def main():
             m6, bc6, b6, mg6_64, mg6_32, max(3, args.solves // 4), args.tol, args.inner_tol,
             log="config2 fourier N=6")
 
+    c0 = config0_block(10, cpu=not args.no_cpu)
+
     serve = None
     if args.serve_requests > 0:
         serve = serve_block(args.serve_requests, args.serve_slots, args.serve_method)
@@ -482,7 +531,8 @@ def main():
         "apply_f32": {"us": round(t32 * 1e6, 3), "GDOF/s": round(K * Np / t32 / 1e9, 2),
                       "frac": round(apply_bytes(K, Np, 4) / t32 / 1e9 / HBM_PEAK_GBS, 4)},
         "cg": {"preconditioner": f"hp-multigrid levels K={mg64.sizes} N={mg64.degrees}",
-               "tol": args.tol, "inner_tol": args.inner_tol, "config1": cg1, "config2": cg2},
+               "tol": args.tol, "inner_tol": args.inner_tol, "config1": cg1, "config2": cg2,
+               "config0": c0},
         "serve": serve,
     }
     print(json.dumps(out), flush=True)

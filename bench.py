@@ -362,8 +362,10 @@ def main():
                          "(calibrates the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the SIPG path is "
-              "single-GPU in this build, see DESIGN.md for how it would shard.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}): the SIPG apply and "
+              "its CG run on one GPU; sharding them (element ranges with a halo-trace "
+              "exchange per apply, two scalar all-reduces per CG iteration) is not built.",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch

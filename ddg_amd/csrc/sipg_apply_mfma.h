@@ -18,12 +18,14 @@
 //   y = ArT . Wr + AsT . Ws + G . F  (+ M . aJu)  (GEMM 2)
 //
 // U, Wr, Ws and y are held in the accumulator layout of the MFMA: lane
-// (g = lane/16, e = lane%16), register r holds node 16b + crow(g, r) of
-// element e, with crow(g, r) = g + 4r for float64 and 4g + r for float32 (the
-// output layouts of the two instructions).  Because the k index of a product
-// may be permuted freely, the constant A fragments are laid out on the host
-// (reference.mfma_fragments) with the matching permutation, so an output
-// feeds the next product as its B operand without any lane shuffle.  F holds
+// (g = lane/16, e = lane%16), register r of block b holds output row
+// 16b + crow(g, r) of element e, with crow(g, r) = g + 4r for float64 and
+// 4g + r for float32 (the output layouts of the two instructions).  Which
+// node a row stands for is free (RowMap: face-major for N <= 4), and so is
+// the k order of a product, so the constant A fragments are laid out on the
+// host (reference.mfma_fragments) with the matching row and column
+// permutations, and an output feeds the next product as its B operand
+// without any lane shuffle.  F holds
 // the per-face-node values P, a*X, b*X (9*Nfp rows); G = [E_f m1d_f^T | LR_f |
 // LS_f] lifts them.
 //
@@ -84,6 +86,70 @@ struct MS {
   static constexpr int nfrag = fM + NB * KB;
 };
 
+// Which node each accumulator position holds: lane group g, register r of
+// 16-row block b.  Orders with one block (N <= 4) use a face-major layout:
+// register r = f (0..2) holds N nodes of face f -- lane group g the face
+// node p = g on faces 0 and 1, p = g + 1 on face 2 -- and register 3 the
+// interior nodes; every other position is padding (index Np).  Each vertex
+// is held once: node 0 by (0, 0), node N by (0, 1), node Np-1 by (N-1, 2).
+// So every face node sits in a register of its own face and the face traces
+// are written by all lanes together, plus one write per vertex for its
+// second face.  Orders with two blocks keep the instructions' row order
+// (node 16b + crow(g, r)).  reference.mfma_node_map is the host copy that
+// lays out the A fragments to match.
+template <typename T, int N>
+struct RowMap {
+  static constexpr int Np = Shape<N>::Np;
+  static constexpr bool FACE = MS<N>::NB == 1;
+  static constexpr int NINT = (N - 1) * (N - 2) / 2;  // interior nodes
+  static constexpr int interior(int i) {
+    for (int n = 0; n < Np; ++n) {
+      bool on = false;
+      for (int f = 0; f < 3; ++f) on = on || facepos<N>(f, n) >= 0;
+      if (!on && i-- == 0) return n;
+    }
+    return Np;
+  }
+  static constexpr int face_node(int g, int r) {
+    if (r < 3) return g < N ? fmask<N>(r, r < 2 ? g : g + 1) : Np;
+    return g < NINT ? interior(g) : Np;
+  }
+  // register r's nodes for lane groups 0..3, one byte each
+  static constexpr unsigned pack(int r) {
+    return (unsigned)face_node(0, r) | ((unsigned)face_node(1, r) << 8) |
+           ((unsigned)face_node(2, r) << 16) | ((unsigned)face_node(3, r) << 24);
+  }
+  __device__ static int node(int b, int g, int r) {
+    if constexpr (FACE)
+      return (int)((pack(r) >> (8 * g)) & 0xffu);
+    else
+      return 16 * b + MfmaT<T>::crow(g, r);
+  }
+  // the same, recomputed at each use: g passes through an empty asm so the
+  // compiler neither hoists nor keeps the values live between uses (they
+  // cost two instructions; holding them costs registers the kernels lack)
+  __device__ static int node_here(int b, int g, int r) {
+    asm volatile("" : "+v"(g));
+    return node(b, g, r);
+  }
+  // compile-time check: each node is held once, each face node in a
+  // register of its face (vertices by the first face listed above)
+  static constexpr bool valid() {
+    if (!FACE) return true;
+    for (int n = 0; n < Np; ++n) {
+      int seen = 0;
+      for (int g = 0; g < 4; ++g)
+        for (int r = 0; r < 4; ++r) seen += face_node(g, r) == n;
+      if (seen != 1) return false;
+    }
+    for (int r = 0; r < 3; ++r)
+      for (int g = 0; g < N; ++g)
+        if (facepos<N>(r, face_node(g, r)) != (r < 2 ? g : g + 1)) return false;
+    return face_node(0, 0) == fmask<N>(2, 0) && face_node(0, 1) == fmask<N>(0, N) &&
+           face_node(N - 1, 2) == fmask<N>(1, N);
+  }
+};
+
 // normal-derivative coefficients a_f, b_f of one element's three faces
 template <typename T>
 struct FaceAB {
@@ -102,6 +168,8 @@ __device__ __forceinline__ void grad_and_traces(const T (&A1)[2 * MS<N>::NB][MS<
   using MF = MfmaT<T>;
   using v4 = typename MF::v4;
   constexpr int NB = Q::NB, KB = Q::KB, Nfp = Q::Nfp, Np = Q::Np;
+  using RM = RowMap<T, N>;
+  static_assert(RM::valid(), "face-major node layout");
 #pragma unroll
   for (int b = 0; b < 2 * NB; ++b) {
     v4 acc = {0, 0, 0, 0};
@@ -115,9 +183,33 @@ __device__ __forceinline__ void grad_and_traces(const T (&A1)[2 * MS<N>::NB][MS<
         US[b - NB][r] = acc[r];
     }
   }
-  // face node (f, p) is node n = fmask(f, p), held by lane group gn in
-  // register (b, r): all three are constants after unrolling, so every face
-  // node is one write predicated on the lane group
+  if constexpr (RM::FACE) {
+    // register f holds face f's nodes p = g (p = g + 1 on face 2)
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      const int j = f * Nfp + (f < 2 ? g : g + 1);
+      if (g < N) {
+        tr[j * 2] = U[0][f];
+        tr[j * 2 + 1] = ab.a(f) * UR[0][f] + ab.b(f) * US[0][f];
+      }
+    }
+    // the vertices' second faces: node 0 is face 2's p = 0, node N face 0's
+    // p = N (both lane group 0), node Np-1 face 1's p = N (lane group N-1)
+    if (g == 0) {
+      tr[(2 * Nfp) * 2] = U[0][0];
+      tr[(2 * Nfp) * 2 + 1] = ab.a(2) * UR[0][0] + ab.b(2) * US[0][0];
+      tr[N * 2] = U[0][1];
+      tr[N * 2 + 1] = ab.a(0) * UR[0][1] + ab.b(0) * US[0][1];
+    }
+    if (g == N - 1) {
+      tr[(Nfp + N) * 2] = U[0][2];
+      tr[(Nfp + N) * 2 + 1] = ab.a(1) * UR[0][2] + ab.b(1) * US[0][2];
+    }
+    return;
+  }
+  // two blocks: face node (f, p) is node n = fmask(f, p), held by lane group
+  // gn in register (b, r): all three are constants after unrolling, so
+  // every face node is one write predicated on the lane group
 #pragma unroll
   for (int f = 0; f < 3; ++f)
 #pragma unroll
@@ -215,7 +307,7 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
   for (int b = 0; b < NB; ++b)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int n = 16 * b + MF::crow(g, r);
+      const int n = RowMap<T, N>::node_here(b, g, r);
       U[b][r] = (valid && n < Np) ? u[(size_t)kg * Np + n] : T(0);
       UH[b][r] = (hg >= 0 && n < Np) ? u[(size_t)hg * Np + n] : T(0);
     }
@@ -357,7 +449,7 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int n = 16 * bo + MF::crow(g, r);
+      const int n = RowMap<T, N>::node_here(bo, g, r);
       T v = acc[r] + acs[r] + acf[r];
       if constexpr (HELM) v += shift * U[bo][r];
       if constexpr (SMOOTH) {
@@ -385,7 +477,7 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
     for (int bo = 0; bo < NB; ++bo)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int n = 16 * bo + MF::crow(g, r);
+        const int n = RowMap<T, N>::node_here(bo, g, r);
         if (valid && n < Np) {
           s_r[el * Np + n] = bvec[(size_t)kg * Np + n] - vv[bo][r];
           s_u[el * Np + n] = U[bo][r];

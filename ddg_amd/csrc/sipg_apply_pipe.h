@@ -107,18 +107,24 @@ sipg_apply_pipe_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
     for (int i = tid; i < Q::nfrag * 64; i += 256) s_fr[i] = fr[i];
   }
 
+  // node held by accumulator register r of this lane, recomputed where it
+  // is used rather than held across the loop
+  auto node_of = [&](int r) { return RowMap<T, N>::node_here(0, g, r); };
   PI P;
   auto prefetch = [&](int tile, int hk) {
     const int k0 = tile * TILE;
     const int ne = min(TILE, K - k0);
     const bool valid = el < ne;
-    const size_t kk = (size_t)(k0 + el) * Np;
+    // the tile's nodal rows from a uniform base with 32-bit lane offsets
+    const T* ut = u + (size_t)k0 * Np;
+    const T* uh = u + (size_t)(hk >= 0 ? hk : 0) * Np;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int n = MF::crow(g, r);
-      P.U[r] = (valid && n < Np) ? u[kk + n] : T(0);
-      P.UH[r] = (hk >= 0 && n < Np) ? u[(size_t)hk * Np + n] : T(0);
-      if constexpr (RESID) P.B[r] = (valid && n < Np) ? bvec[kk + n] : T(0);
+      const int n = node_of(r);
+      const unsigned o = (unsigned)(el * Np + n);
+      P.U[r] = (valid && n < Np) ? ut[o] : T(0);
+      P.UH[r] = (hk >= 0 && n < Np) ? uh[(unsigned)n] : T(0);
+      if constexpr (RESID) P.B[r] = (valid && n < Np) ? bvec[(size_t)k0 * Np + o] : T(0);
     }
 #pragma unroll
     for (int j = 0; j < PI::RPT; ++j) {
@@ -148,7 +154,6 @@ sipg_apply_pipe_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
   for (int tile = t0; tile < t1; ++tile) {
     const int k0 = tile * TILE;
     const int ne = min(TILE, K - k0);
-    const int k = k0 + el;
     const bool valid = el < ne;
     lds_barrier();  // the previous tile's LDS reads are done (and fragments visible)
     // commit the prefetched tile
@@ -290,17 +295,18 @@ sipg_apply_pipe_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
       for (int t = 0; t < KB; ++t)
         acc = MF::mma(s_fr[(Q::fM + t) * 64 + lane], Jm * U[0][t % 4], acc);
     }
+    T* yt = y + (size_t)k0 * Np;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int n = MF::crow(g, r);
+      const int n = node_of(r);
       T v = acc[r] + acs[r] + (acf[r] + acg[r]);
       if constexpr (HELM) v += shift * U[0][r];
       if (valid && n < Np) {
-        const size_t gi = (size_t)k * Np + n;
+        const unsigned gi = (unsigned)(el * Np + n);
         if constexpr (RESID)
-          y[gi] = Bv[r] - v;
+          yt[gi] = Bv[r] - v;
         else
-          y[gi] = v;
+          yt[gi] = v;
         if constexpr (DOT) dacc += (double)U[0][r] * (double)v;
       }
     }

@@ -212,14 +212,43 @@ def mfma_crow(dtype: str, g, r):
     return g + 4 * r if dtype == "f64" else 4 * g + r
 
 
+def mfma_node_map(ref: "RefElement", dtype: str) -> np.ndarray:
+    """(NB, 4, 4) node held by block b, lane group g, accumulator register r
+    (Np marks padding); the kernels' RowMap.
+
+    One block (N <= 4): face-major -- register f < 3 holds face f's nodes
+    p = g (faces 0, 1) or p = g + 1 (face 2) for g < N, register 3 the
+    interior nodes in ascending order.  Two blocks: node 16 b + crow(g, r).
+    """
+    Np, N, Fm = ref.Np, ref.N, ref.Fmask
+    NB = (Np + 15) // 16
+    out = np.full((NB, 4, 4), Np, dtype=np.int64)
+    if NB == 1:
+        interior = [n for n in range(Np) if n not in set(Fm.reshape(-1).tolist())]
+        for g in range(4):
+            for f in range(3):
+                if g < N:
+                    out[0, g, f] = Fm[g if f < 2 else g + 1, f]
+            if g < len(interior):
+                out[0, g, 3] = interior[g]
+    else:
+        for b in range(NB):
+            for g in range(4):
+                for r in range(4):
+                    out[b, g, r] = 16 * b + mfma_crow(dtype, g, r)
+    return out
+
+
 def mfma_fragments(ref: "RefElement", dtype: str) -> np.ndarray:
     """A-operand fragments of the matrix-core apply, one 64-lane fragment per
     (output block, k-step), in the order GEMM1 | GEMM2 volume | face | mass.
 
-    Lane l = 16 g + m supplies A[16 b + m][sigma(t, g)].  For products whose
-    B operand is a nodal vector held in accumulator layout, the k index is
-    sigma(t, g) = 16 (t // 4) + crow(g, t % 4); for the face rows it is
-    4 t + g.  Entries outside the matrix are zero.
+    Lane l = 16 g + m supplies A[row(b, m)][col(t, g)]: output row m of
+    block b lands in the accumulator position (g', r') with crow(g', r') = m,
+    so row(b, m) is the node mfma_node_map puts there; for products whose B
+    operand is a nodal vector held in accumulator layout, col(t, g) is the
+    node of block t // 4, lane group g, register t % 4; for the face rows it
+    is 4 t + g.  Entries outside the matrix (padding) are zero.
     """
     Np, Nfp, Fm = ref.Np, ref.Nfp, ref.Fmask
     NB = (Np + 15) // 16
@@ -228,10 +257,17 @@ def mfma_fragments(ref: "RefElement", dtype: str) -> np.ndarray:
     FK = (NF + 3) // 4
     lanes = np.arange(64)
     m, g = lanes % 16, lanes // 16
+    nmap = mfma_node_map(ref, dtype)
+    # output row m of a block -> (lane group, register) of the instruction
+    gi = m % 4 if dtype == "f64" else m // 4
+    ri = m // 4 if dtype == "f64" else m % 4
+
+    def row_node(b):
+        return nmap[b, gi, ri]
 
     def frag_nodal(Mat, b, t):
-        row = 16 * b + m
-        col = 16 * (t // 4) + mfma_crow(dtype, g, t % 4)
+        row = row_node(b)
+        col = nmap[t // 4, g, t % 4]
         ok = (row < Mat.shape[0]) & (col < Mat.shape[1])
         return np.where(ok, Mat[np.minimum(row, Mat.shape[0] - 1), np.minimum(col, Mat.shape[1] - 1)], 0.0)
 
@@ -255,7 +291,7 @@ def mfma_fragments(ref: "RefElement", dtype: str) -> np.ndarray:
             out.append(frag_nodal(ArT if t < KB else AsT, b, t % KB))
     for b in range(NB):                                  # face lift
         for t in range(FK):
-            row, col = 16 * b + m, 4 * t + g
+            row, col = row_node(b), 4 * t + g
             ok = (row < Np) & (col < NF)
             out.append(np.where(ok, G[np.minimum(row, Np - 1), np.minimum(col, NF - 1)], 0.0))
     for b in range(NB):                                  # mass

@@ -109,7 +109,7 @@ def mma(a, b, c, dtype):
     return out
 
 
-def model_apply_mfma(N, tab, frag, rec, nbr, u, dtype="f64"):
+def model_apply_mfma(N, tab, frag, rec, nbr, u, dtype="f64", node_map=None):
     """Apply with the MFMA data flow for u (K, Np); faces via the closed form
     (as model_apply), contractions via mma() on the fragments."""
     K, Np = u.shape
@@ -153,8 +153,11 @@ def model_apply_mfma(N, tab, frag, rec, nbr, u, dtype="f64"):
         ek = np.minimum(s0 + e, K - 1)
         valid = (s0 + e) < K
 
+        def node(b, r):
+            return node_map[b, g, r] if node_map is not None else 16 * b + _crow(dtype, g, r)
+
         def nodal(vec_rows, b, r):
-            n = 16 * b + _crow(dtype, g, r)
+            n = node(b, r)
             return np.where(valid & (n < Np), vec_rows[ek, np.minimum(n, Np - 1)], 0.0)
         U = [[nodal(u, b, r) for r in range(4)] for b in range(NB)]
         URS = []
@@ -177,7 +180,7 @@ def model_apply_mfma(N, tab, frag, rec, nbr, u, dtype="f64"):
             for t in range(FK):
                 acc = mma(F[offs[2] + bo * FK + t], Fr[ek, 4 * t + g], acc, dtype)
             for r in range(4):
-                n = 16 * bo + _crow(dtype, g, r)
+                n = node(bo, r)
                 ok = valid & (n < Np)
                 y[ek[ok], n[ok]] = acc[ok, r]
     return y

@@ -104,15 +104,43 @@ def test_mfma_fragments_reproduce_apply(N, dtype):
     """Lane-level emulation of the matrix-core data flow (fragment layout,
     k permutation, output layout) against the closed-form model."""
     from kernel_model import model_apply_mfma
-    from ddg_amd.reference import mfma_fragments
+    from ddg_amd.reference import mfma_fragments, mfma_node_map
     pm, _ = unit_square(3, 3, N, jitter=0.1, seed=4)
     bc = pmesh.dirichlet_x_neumann_y(pm)
     rec, nbr, _ = pack(pm, bc)
     ref = ref_element(N)
     tab = ref.kernel_tables()
     u = np.random.default_rng(N).standard_normal((pm.K, pm.Np))
-    y = model_apply_mfma(N, tab, mfma_fragments(ref, dtype), rec, nbr, u, dtype)
+    y = model_apply_mfma(N, tab, mfma_fragments(ref, dtype), rec, nbr, u, dtype,
+                         mfma_node_map(ref, dtype))
     assert rel(y, model_apply(N, tab, rec, nbr, u)) < 1e-12
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_mfma_node_map_face_traces(N, dtype):
+    """Every node sits at exactly one accumulator position, and for N <= 4 the
+    face-major trace writes of the kernel (register f, face node p = g or
+    g + 1, plus the three vertices' second faces) cover each of the 3*Nfp
+    face slots once, with the node Fmask puts there."""
+    from ddg_amd.reference import mfma_node_map
+    ref = ref_element(N)
+    nm = mfma_node_map(ref, dtype)
+    held = sorted(int(n) for n in nm.reshape(-1) if n < ref.Np)
+    assert held == list(range(ref.Np))
+    if nm.shape[0] != 1:
+        return
+    Nfp, Fm = ref.Nfp, ref.Fmask
+    writes = {}
+    for f in range(3):
+        for g in range(N):
+            writes.setdefault((f, g if f < 2 else g + 1), []).append(nm[0, g, f])
+    writes.setdefault((2, 0), []).append(nm[0, 0, 0])
+    writes.setdefault((0, N), []).append(nm[0, 0, 1])
+    writes.setdefault((1, N), []).append(nm[0, N - 1, 2])
+    assert sorted(writes) == [(f, p) for f in range(3) for p in range(Nfp)]
+    for (f, p), nodes in writes.items():
+        assert nodes == [Fm[p, f]]
 
 
 def test_forest_hierarchy_matches_oracle():

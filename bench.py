@@ -16,6 +16,8 @@ Also reported, on the same mesh:
   operator, float32 V-cycle, float64 residual and update), and by float64
   CG preconditioned with the float32 V-cycle (mixed-precision
   preconditioning: every float64 quantity of the CG stays float64);
+* the first of those solves by float64 CG without a preconditioner (the
+  plain CG) and with block-Jacobi, once each;
 * both solvers on seeded random Fourier right-hand sides (64 modes,
   amplitude |k|^-2) on this mesh and on 2x128x128 at N=6, with the error
   against a float64 solve to tol 1e-13;
@@ -222,6 +224,27 @@ def solver_block(m, bc, b, mg64, mg32, n_solves, tol, inner_tol, ref_x=None, ue=
     return res
 
 
+def plain_block(m, bc, b, ref_x, tol, mixed_pc_ms):
+    """The same solve by float64 CG without a preconditioner (the plain CG)
+    and with block-Jacobi, one timed solve each, beside the float64 CG with
+    the float32 V-cycle (mixed_pc_ms, its p50)."""
+    import torch
+    from ddg_amd import sipg as D
+    op = D.get_operator(m, bc)
+    nref = float(torch.linalg.norm(ref_x))
+    res = {}
+    for name, pc in (("fp64_plain_cg", None), ("fp64_block_jacobi_cg", op.block_jacobi())):
+        lat, (x, info) = timed_solves(lambda: op.pcg(b, pc=pc, tol=tol, maxiter=400000), 1)
+        res[name] = {"ms": round(float(lat[0]) * 1e3, 1), "iterations": info["iterations"],
+                     "converged": info["converged"],
+                     "rel_err_vs_fp64_tight": float(torch.linalg.norm(x - ref_x)) / nref}
+        print(f"[bench] config1 {name}: {res[name]}", file=sys.stderr, flush=True)
+    res["speedup_mixed_pc_vs_fp64_plain_cg"] = round(res["fp64_plain_cg"]["ms"] / mixed_pc_ms, 1)
+    res["speedup_mixed_pc_vs_fp64_block_jacobi_cg"] = round(
+        res["fp64_block_jacobi_cg"]["ms"] / mixed_pc_ms, 1)
+    return res
+
+
 def serve_block(n_requests, slots, method="pcg32", calib=200, load=0.8, sample=8, seed=0):
     """The request stream: Helmholtz solves on 2 n^2 meshes, n in {16, 32, 64,
     128}, N in {2..6}, k^2 in [0, 50], Fourier right-hand sides, mixed
@@ -338,6 +361,9 @@ def main():
     ap.add_argument("--inner-tol", type=float, default=1e-4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-config2", action="store_true")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the config-1 solves by float64 CG without a preconditioner "
+                         "and with block-Jacobi (about 15 s)")
     ap.add_argument("--serve-requests", type=int, default=2000,
                     help="requests in the serving stream (0: skip the serving leg)")
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -480,9 +506,12 @@ def main():
     f, ud, q = problem_data(fin)
     opb = D.get_operator(fin, bc)
     b = opb.rhs(_flat(f, dev), u_dir=_flat(ud, dev), q_neu=_flat(q, dev))
-    cg1 = solver_block(fin, bc, b, mg64, mg32, args.solves, args.tol, args.inner_tol, ue=ud,
-                       log="config1 e^x cos3y")
+    ref1, _ = opb.pcg(b, pc=mg64, tol=1e-13, maxiter=5000)
+    cg1 = solver_block(fin, bc, b, mg64, mg32, args.solves, args.tol, args.inner_tol, ref_x=ref1,
+                       ue=ud, log="config1 e^x cos3y")
     cg1["setup_s"] = round(t_setup, 3)
+    if not args.no_plain:
+        cg1["plain"] = plain_block(fin, bc, b, ref1, args.tol, cg1["mixed_pc"]["p50_ms"])
     fr = fourier_rhs(fin, seed=1)
     b2 = opb.rhs(_flat(fr, dev))
     cg2 = {"N4_2x%dx%d" % (args.n, args.n):

@@ -366,7 +366,7 @@ def main():
                          "and with block-Jacobi (about 15 s)")
     ap.add_argument("--serve-requests", type=int, default=2000,
                     help="requests in the serving stream (0: skip the serving leg)")
-    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--serve-method", default="pcg32", choices=["pcg32", "refine"])
     ap.add_argument("--serve-only", action="store_true", help="run only the serving leg")
     ap.add_argument("--n", type=int, default=256)

@@ -341,7 +341,12 @@ class ServeConfig:
     # "refine": mixed-precision iterative refinement (float32 MG-PCG steps in
     # a float64 residual loop).  Both stop at tol on the float64 residual.
     method: str = "pcg32"
-    slots: int = 4                  # solves in flight at once (one HIP stream each)
+    # solves in flight at once (one HIP stream each).  The stream's small
+    # kernels leave most of the GPU idle, so more solves side by side raise
+    # the rate: on one MI355X (four hardware queues per process) the mixed
+    # stream's closed-loop capacity is about 71/s with 4 slots, 87/s with
+    # 8 and 84/s with 12
+    slots: int = 8
     hbm_budget: int = 64 << 30      # bytes of per-request state in flight
     tol: float = 1e-10              # outer (float64) relative residual
     inner_tol: float = 1e-4         # float32 MG-PCG relative residual per step

@@ -60,7 +60,23 @@ finalize_kernel(const double* __restrict__ partial, int nparts, CGScalars* sc, i
   if (mode >= F_ALPHA && mode <= F_BETA && sc->done) return;
   __shared__ double s_red[4];
   double s0 = 0.0, s1 = 0.0;
-  for (int i = threadIdx.x; i < nparts; i += 256) {
+  // each thread adds partials i, i + 256, ... in that order; eight at a
+  // time are loaded before they are added, so the loads overlap
+  int i = threadIdx.x;
+  for (; i + 7 * 256 < nparts; i += 8 * 256) {
+    double a[8], b[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      a[u] = partial[i + u * 256];
+      b[u] = partial[nparts + i + u * 256];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      s0 += a[u];
+      s1 += b[u];
+    }
+  }
+  for (; i < nparts; i += 256) {
     s0 += partial[i];
     s1 += partial[nparts + i];
   }

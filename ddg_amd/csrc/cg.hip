@@ -54,6 +54,32 @@ dot2_kernel(const T* __restrict__ a, const T* __restrict__ b, const T* __restric
   }
 }
 
+// Sum of partial[0..nparts) by a 256-thread block, in the order
+// finalize_kernel adds its first sum (so both give the same bits).
+__device__ double block_sum_partials(const double* __restrict__ partial, int nparts,
+                                     double* s_red) {
+  double s0 = 0.0;
+  int i = threadIdx.x;
+  for (; i + 7 * 256 < nparts; i += 8 * 256) {
+    double a[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a[u] = partial[i + u * 256];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s0 += a[u];
+  }
+  for (; i < nparts; i += 256) s0 += partial[i];
+  return block_sum_256(s0, s_red);
+}
+
+// CG's step length alpha = gamma / <p, A p> from the apply's partials, formed
+// by every block of the update that uses it (no finalize launch between the
+// apply and the update); the same value finalize_kernel(F_ALPHA) stores.
+template <typename T>
+__device__ T cg_alpha(const double* __restrict__ apart, int napart, const CGScalars* sc,
+                      double* s_red) {
+  return (T)(sc->gamma / block_sum_partials(apart, napart, s_red));
+}
+
 // One workgroup: sum the partials in fixed order and advance the scalars.
 __global__ void __launch_bounds__(256)
 finalize_kernel(const double* __restrict__ partial, int nparts, CGScalars* sc, int mode) {
@@ -144,10 +170,11 @@ template <typename T>
 __global__ void __launch_bounds__(256)
 update_kernel(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
               const T* __restrict__ Ap, const CGScalars* __restrict__ sc, size_t n,
-              double* __restrict__ partial, float* __restrict__ r32) {
+              double* __restrict__ partial, float* __restrict__ r32,
+              const double* __restrict__ apart, int napart) {
   if (sc->done) return;
   __shared__ double s_red[4];
-  const T alpha = (T)sc->alpha;
+  const T alpha = cg_alpha<T>(apart, napart, sc, s_red);
   const double inv = r32 != nullptr ? 1.0 / pc_scale(sc) : 0.0;
   double acc = 0.0;
   const size_t stride = (size_t)gridDim.x * 256;
@@ -172,14 +199,15 @@ template <typename T, int N>
 __global__ void __launch_bounds__(256)
 update_bj_kernel(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
                  const T* __restrict__ Ap, const T* __restrict__ binv, T* __restrict__ z,
-                 const CGScalars* __restrict__ sc, int K, double* __restrict__ partial) {
+                 const CGScalars* __restrict__ sc, int K, double* __restrict__ partial,
+                 const double* __restrict__ apart, int napart) {
   using S = Shape<N>;
   constexpr int Np = S::Np, NL = S::NL, EPB = S::EPB;
   if (sc->done) return;
   __shared__ T s_r[EPB * Np];
   __shared__ T s_b[EPB * SymPack<N>::NPK];
   __shared__ double s_red[4];
-  const T alpha = (T)sc->alpha;
+  const T alpha = cg_alpha<T>(apart, napart, sc, s_red);
   const int tid = threadIdx.x, e = tid / NL, li = tid % NL;
   double rz = 0.0, rr = 0.0;
   const int ntiles = (K + EPB - 1) / EPB;
@@ -347,7 +375,9 @@ int work_create(const Op& op, bool refine, Work** out) {
   w->device = op.device;
   const size_t n = (size_t)op.K * op.Np * dsize(op.dtype);
   w->npart = op.grid_apply > op.grid_vec ? op.grid_apply : op.grid_vec;
-  const size_t np = 2 * (size_t)w->npart * sizeof(double);
+  // two slices for the vector reductions, a third for the apply's <p, Ap>
+  // partials (read by the update while it writes the first two)
+  const size_t np = 3 * (size_t)w->npart * sizeof(double);
   hipError_t e = hipSetDevice(op.device);
   if (e == hipSuccess) e = hipMalloc(&w->r, n);
   if (e == hipSuccess) e = hipMalloc(&w->p, n);
@@ -419,7 +449,8 @@ template <typename T, int N>
 static int update_bj(const Op& op, Work& w, void* x, const void* binv, hipStream_t s) {
   hipLaunchKernelGGL((update_bj_kernel<T, N>), dim3(op.grid_vec), dim3(256), 0, s, (T*)x,
                      (T*)w.r, (const T*)w.p, (const T*)w.Ap, (const T*)binv, (T*)w.z,
-                     (const CGScalars*)w.scal, (int)op.K, w.partials);
+                     (const CGScalars*)w.scal, (int)op.K, w.partials,
+                     (const double*)(w.partials + 2 * w.npart), (int)op.grid_apply);
   DDGX_HIP(hipGetLastError());
   return DDGX_OK;
 }
@@ -540,8 +571,9 @@ static int pcg_t(const Op& op, Work& w, double alpha, double shift, const PC* pc
   // flag (converged or k == maxiter) is set
   auto iteration = [&]() -> int {
     // Ap = A p, partial <p, Ap>
-    CHECK(launch_apply(op, w.p, w.Ap, nullptr, alpha, shift, M_DOT, w.partials, done, s));
-    CHECK(finalize(w, op.grid_apply, F_ALPHA, s));
+    // (the update kernels form alpha from these partials themselves)
+    CHECK(launch_apply(op, w.p, w.Ap, nullptr, alpha, shift, M_DOT, w.partials + 2 * w.npart,
+                       done, s));
     if (bj) {
       CHECK(update_bj_dispatch<T>(op, w, x, pc->binv, s));
     } else {
@@ -550,7 +582,8 @@ static int pcg_t(const Op& op, Work& w, double alpha, double shift, const PC* pc
       hipLaunchKernelGGL((update_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (T*)x,
                          (T*)w.r, (const T*)w.p, (const T*)w.Ap,
                          (const CGScalars*)w.scal, n, has_pc ? nullptr : w.partials,
-                         mixed ? (float*)w.pc_r : nullptr);
+                         mixed ? (float*)w.pc_r : nullptr,
+                         (const double*)(w.partials + 2 * w.npart), (int)op.grid_apply);
       DDGX_HIP(hipGetLastError());
       if (mixed) {
         CHECK(mg_vcycle(*pc->mg, *mgw, w.pc_r, w.pc_z, done, s));

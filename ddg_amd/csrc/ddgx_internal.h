@@ -171,7 +171,7 @@ struct Work {
   void* p = nullptr;
   void* z = nullptr;
   void* Ap = nullptr;
-  double* partials = nullptr;  // 2 * max(grid_apply, grid_vec) doubles
+  double* partials = nullptr;  // 3 * npart doubles: vector reductions, then <p, Ap>
   int npart = 0;
   void* ir_b = nullptr;        // refinement (float32 work): scaled residual / correction
   void* ir_x = nullptr;

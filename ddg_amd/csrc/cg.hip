@@ -42,7 +42,26 @@ dot2_kernel(const T* __restrict__ a, const T* __restrict__ b, const T* __restric
   __shared__ double s_red[4];
   double s0 = 0.0, s1 = 0.0;
   const size_t stride = (size_t)gridDim.x * 256;
-  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  // four strides per trip, loaded before they are used (same order of sums)
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    T av[4], bv[4], cv[4], dv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      av[u] = a[i + u * stride];
+      bv[u] = b[i + u * stride];
+      if (c != nullptr) {
+        cv[u] = c[i + u * stride];
+        dv[u] = d[i + u * stride];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      s0 += (double)av[u] * (double)bv[u];
+      if (c != nullptr) s1 += (double)cv[u] * (double)dv[u];
+    }
+  }
+  for (; i < n; i += stride) {
     s0 += (double)a[i] * (double)b[i];
     if (c != nullptr) s1 += (double)c[i] * (double)d[i];
   }
@@ -178,7 +197,29 @@ update_kernel(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
   const double inv = r32 != nullptr ? 1.0 / pc_scale(sc) : 0.0;
   double acc = 0.0;
   const size_t stride = (size_t)gridDim.x * 256;
-  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  // four strides per trip, loaded before they are used (same order of sums)
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    T xv[4], pv[4], rv[4], av[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const size_t j = i + u * stride;
+      xv[u] = x[j];
+      pv[u] = p[j];
+      rv[u] = r[j];
+      av[u] = Ap[j];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const size_t j = i + u * stride;
+      x[j] = xv[u] + alpha * pv[u];
+      const T ri = rv[u] - alpha * av[u];
+      r[j] = ri;
+      if (r32 != nullptr) r32[j] = (float)(inv * (double)ri);
+      acc += (double)ri * (double)ri;
+    }
+  }
+  for (; i < n; i += stride) {
     x[i] += alpha * p[i];
     const T ri = r[i] - alpha * Ap[i];
     r[i] = ri;
@@ -293,7 +334,25 @@ pc_up_dot_kernel(const float* __restrict__ z32, double* __restrict__ z,
   const double sc_ = pc_scale(sc);
   double s0 = 0.0, s1 = 0.0;
   const size_t stride = (size_t)gridDim.x * 256;
-  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  // four strides per trip, loaded before they are used (same order of sums)
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    float zv[4];
+    double rv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      zv[u] = z32[i + u * stride];
+      rv[u] = r[i + u * stride];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double zi = sc_ * (double)zv[u];
+      z[i + u * stride] = zi;
+      s0 += rv[u] * zi;
+      s1 += rv[u] * rv[u];
+    }
+  }
+  for (; i < n; i += stride) {
     const double zi = sc_ * (double)z32[i];
     z[i] = zi;
     const double ri = r[i];
@@ -316,8 +375,18 @@ pupdate_kernel(T* __restrict__ p, const T* __restrict__ z, const CGScalars* __re
   if (sc->done) return;
   const T beta = (T)sc->beta;
   const size_t stride = (size_t)gridDim.x * 256;
-  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
-    p[i] = z[i] + beta * p[i];
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {  // four strides' loads in flight
+    T zv[4], pv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      zv[u] = z[i + u * stride];
+      pv[u] = p[i + u * stride];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) p[i + u * stride] = zv[u] + beta * pv[u];
+  }
+  for (; i < n; i += stride) p[i] = z[i] + beta * p[i];
 }
 
 // ---------------------------------------------------------------------------

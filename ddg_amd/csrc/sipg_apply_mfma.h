@@ -303,6 +303,11 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
   const int hk = hwave ? hp[el] : -1;
   const int hg = hk >= 0 ? (perm ? perm[hk] : hk) : -1;
   T U[NB][4], UH[NB][4];
+  // the right-hand side of the float64 residual and smoothing modes, loaded
+  // with u so the epilogue does not wait on a load issued after GEMM 2
+  // (float32 reads it in the epilogue: it has no registers to spare)
+  constexpr bool NEEDB = (RESID || SMOOTH) && sizeof(T) == 8;
+  T BV[NB][4];  // unused (and dropped by the compiler) unless NEEDB
 #pragma unroll
   for (int b = 0; b < NB; ++b)
 #pragma unroll
@@ -310,6 +315,7 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
       const int n = RowMap<T, N>::node_here(b, g, r);
       U[b][r] = (valid && n < Np) ? u[(size_t)kg * Np + n] : T(0);
       UH[b][r] = (hg >= 0 && n < Np) ? u[(size_t)hg * Np + n] : T(0);
+      if constexpr (NEEDB) BV[b][r] = (valid && n < Np) ? bvec[(size_t)kg * Np + n] : T(0);
     }
   for (int i = tid; i < DDGX_REC_WORDS * TILE; i += 256) {
     const int w = i / TILE, ee = i % TILE;
@@ -457,7 +463,7 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
       } else if (valid && n < Np) {
         const size_t gi = (size_t)kg * Np + n;
         if constexpr (RESID)
-          y[gi] = bvec[gi] - v;
+          y[gi] = (NEEDB ? BV[bo][r] : bvec[gi]) - v;
         else
           y[gi] = v;
         if constexpr (DOT) dacc += (double)U[bo][r] * (double)v;
@@ -479,7 +485,7 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
       for (int r = 0; r < 4; ++r) {
         const int n = RowMap<T, N>::node_here(bo, g, r);
         if (valid && n < Np) {
-          s_r[el * Np + n] = bvec[(size_t)kg * Np + n] - vv[bo][r];
+          s_r[el * Np + n] = (NEEDB ? BV[bo][r] : bvec[(size_t)kg * Np + n]) - vv[bo][r];
           s_u[el * Np + n] = U[bo][r];
         }
       }

@@ -21,7 +21,9 @@
 
 namespace ddgx {
 
-enum { F_INIT_BB = 0, F_INIT_GAMMA = 1, F_ALPHA = 2, F_BETA = 3, F_SUM = 4 };
+// what finalize_kernel forms from its two sums (the per-iteration alpha
+// and beta are formed by the update kernels themselves)
+enum { F_INIT_BB = 0, F_INIT_GAMMA = 1, F_SUM = 4 };
 
 __device__ inline double block_sum_256(double v, double* s_red) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -91,22 +93,21 @@ __device__ double block_sum_partials(const double* __restrict__ partial, int npa
 }
 
 // CG's step length alpha = gamma / <p, A p> from the apply's partials, formed
-// by every block of the update that uses it (no finalize launch between the
-// apply and the update); the same value finalize_kernel(F_ALPHA) stores.
+// by every block of the update that uses it (no reduction launch between the
+// apply and the update); gamma of the iteration is in slot gq[q].
 template <typename T>
 __device__ T cg_alpha(const double* __restrict__ apart, int napart, const CGScalars* sc,
-                      double* s_red) {
-  return (T)(sc->gamma / block_sum_partials(apart, napart, s_red));
+                      double* s_red, int q) {
+  return (T)(sc->gq[q] / block_sum_partials(apart, napart, s_red));
 }
 
-// One workgroup: sum the partials in fixed order and advance the scalars.
-__global__ void __launch_bounds__(256)
-finalize_kernel(const double* __restrict__ partial, int nparts, CGScalars* sc, int mode) {
-  if (mode >= F_ALPHA && mode <= F_BETA && sc->done) return;
-  __shared__ double s_red[4];
+// Both sums of partial[0..nparts) and partial[nparts..2 nparts) by a
+// 256-thread block, in one fixed order: each thread adds entries i, i + 256,
+// ... in that order (eight loaded before they are added, so the loads
+// overlap), then block_sum_256.
+__device__ void block_sum2_partials(const double* __restrict__ partial, int nparts,
+                                    double* s_red, double& r0, double& r1) {
   double s0 = 0.0, s1 = 0.0;
-  // each thread adds partials i, i + 256, ... in that order; eight at a
-  // time are loaded before they are added, so the loads overlap
   int i = threadIdx.x;
   for (; i + 7 * 256 < nparts; i += 8 * 256) {
     double a[8], b[8];
@@ -125,8 +126,28 @@ finalize_kernel(const double* __restrict__ partial, int nparts, CGScalars* sc, i
     s0 += partial[i];
     s1 += partial[nparts + i];
   }
-  s0 = block_sum_256(s0, s_red);
-  s1 = block_sum_256(s1, s_red);
+  r0 = block_sum_256(s0, s_red);
+  r1 = block_sum_256(s1, s_red);
+}
+
+// The CG scalars after the <r,z>, <r,r> reduction of an iteration: beta,
+// the new gamma (also into slot gq[q ^ 1]), the residual and the stop test.
+__device__ inline void cg_advance(CGScalars* sc, double s0, double s1, double g_old, int q) {
+  sc->beta = s0 / g_old;
+  sc->gamma = s0;
+  sc->gq[q ^ 1] = s0;
+  sc->rr = sc->use_pc ? s1 : s0;
+  sc->rs = sc->rr;
+  sc->k += 1;
+  sc->done = !(sc->rs > sc->atol2 && sc->k < sc->maxiter);
+}
+
+// One workgroup: sum the partials in fixed order and advance the scalars.
+__global__ void __launch_bounds__(256)
+finalize_kernel(const double* __restrict__ partial, int nparts, CGScalars* sc, int mode) {
+  __shared__ double s_red[4];
+  double s0, s1;
+  block_sum2_partials(partial, nparts, s_red, s0, s1);
   if (threadIdx.x != 0) return;
   switch (mode) {
     case F_INIT_BB: {
@@ -137,21 +158,10 @@ finalize_kernel(const double* __restrict__ partial, int nparts, CGScalars* sc, i
     }
     case F_INIT_GAMMA:
       sc->gamma = s0;
+      sc->gq[0] = s0;
       sc->rr = sc->use_pc ? s1 : s0;
       sc->rs = sc->rr;
       sc->k = 0;
-      sc->done = !(sc->rs > sc->atol2 && sc->k < sc->maxiter);
-      break;
-    case F_ALPHA:
-      sc->pAp = s0;
-      sc->alpha = sc->gamma / s0;
-      break;
-    case F_BETA:
-      sc->beta = s0 / sc->gamma;
-      sc->gamma = s0;
-      sc->rr = sc->use_pc ? s1 : s0;
-      sc->rs = sc->rr;
-      sc->k += 1;
       sc->done = !(sc->rs > sc->atol2 && sc->k < sc->maxiter);
       break;
     default:  // F_SUM
@@ -190,10 +200,10 @@ __global__ void __launch_bounds__(256)
 update_kernel(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
               const T* __restrict__ Ap, const CGScalars* __restrict__ sc, size_t n,
               double* __restrict__ partial, float* __restrict__ r32,
-              const double* __restrict__ apart, int napart) {
+              const double* __restrict__ apart, int napart, int q) {
   if (sc->done) return;
   __shared__ double s_red[4];
-  const T alpha = cg_alpha<T>(apart, napart, sc, s_red);
+  const T alpha = cg_alpha<T>(apart, napart, sc, s_red, q);
   const double inv = r32 != nullptr ? 1.0 / pc_scale(sc) : 0.0;
   double acc = 0.0;
   const size_t stride = (size_t)gridDim.x * 256;
@@ -241,14 +251,14 @@ __global__ void __launch_bounds__(256)
 update_bj_kernel(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
                  const T* __restrict__ Ap, const T* __restrict__ binv, T* __restrict__ z,
                  const CGScalars* __restrict__ sc, int K, double* __restrict__ partial,
-                 const double* __restrict__ apart, int napart) {
+                 const double* __restrict__ apart, int napart, int q) {
   using S = Shape<N>;
   constexpr int Np = S::Np, NL = S::NL, EPB = S::EPB;
   if (sc->done) return;
   __shared__ T s_r[EPB * Np];
   __shared__ T s_b[EPB * SymPack<N>::NPK];
   __shared__ double s_red[4];
-  const T alpha = cg_alpha<T>(apart, napart, sc, s_red);
+  const T alpha = cg_alpha<T>(apart, napart, sc, s_red, q);
   const int tid = threadIdx.x, e = tid / NL, li = tid % NL;
   double rz = 0.0, rr = 0.0;
   const int ntiles = (K + EPB - 1) / EPB;
@@ -367,13 +377,23 @@ pc_up_dot_kernel(const float* __restrict__ z32, double* __restrict__ z,
   }
 }
 
-// p = z + beta p
+// The end of a CG iteration in one launch: every block sums the <r,z>,
+// <r,r> partials (the order finalize_kernel uses), forms beta = gamma' /
+// gamma with gamma from slot gq[q], and updates its part of p = z + beta p;
+// block 0 also advances the scalars (new gamma into slot gq[q ^ 1], the
+// residual, the iteration count and the stop flag).  No block reads what
+// block 0 writes, except the stop flag: a block that already sees it set
+// skips its part of p, which no later iteration reads.
 template <typename T>
 __global__ void __launch_bounds__(256)
-pupdate_kernel(T* __restrict__ p, const T* __restrict__ z, const CGScalars* __restrict__ sc,
-               size_t n) {
+pupdate_fin_kernel(T* __restrict__ p, const T* __restrict__ z, CGScalars* sc, size_t n,
+                   const double* __restrict__ partial, int nparts, int q) {
   if (sc->done) return;
-  const T beta = (T)sc->beta;
+  __shared__ double s_red[4];
+  double s0, s1;
+  block_sum2_partials(partial, nparts, s_red, s0, s1);
+  const double g_old = sc->gq[q];
+  const T beta = (T)(s0 / g_old);
   const size_t stride = (size_t)gridDim.x * 256;
   size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
   for (; i + 3 * stride < n; i += 4 * stride) {  // four strides' loads in flight
@@ -387,6 +407,7 @@ pupdate_kernel(T* __restrict__ p, const T* __restrict__ z, const CGScalars* __re
     for (int u = 0; u < 4; ++u) p[i + u * stride] = zv[u] + beta * pv[u];
   }
   for (; i < n; i += stride) p[i] = z[i] + beta * p[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0) cg_advance(sc, s0, s1, g_old, q);
 }
 
 // ---------------------------------------------------------------------------
@@ -515,25 +536,25 @@ static int finalize(Work& w, int nparts, int mode, hipStream_t s) {
 }
 
 template <typename T, int N>
-static int update_bj(const Op& op, Work& w, void* x, const void* binv, hipStream_t s) {
+static int update_bj(const Op& op, Work& w, void* x, const void* binv, int q, hipStream_t s) {
   hipLaunchKernelGGL((update_bj_kernel<T, N>), dim3(op.grid_vec), dim3(256), 0, s, (T*)x,
                      (T*)w.r, (const T*)w.p, (const T*)w.Ap, (const T*)binv, (T*)w.z,
                      (const CGScalars*)w.scal, (int)op.K, w.partials,
-                     (const double*)(w.partials + 2 * w.npart), (int)op.grid_apply);
+                     (const double*)(w.partials + 2 * w.npart), (int)op.grid_apply, q);
   DDGX_HIP(hipGetLastError());
   return DDGX_OK;
 }
 
 template <typename T>
-static int update_bj_dispatch(const Op& op, Work& w, void* x, const void* binv,
+static int update_bj_dispatch(const Op& op, Work& w, void* x, const void* binv, int q,
                               hipStream_t s) {
   switch (op.N) {
-    case 1: return update_bj<T, 1>(op, w, x, binv, s);
-    case 2: return update_bj<T, 2>(op, w, x, binv, s);
-    case 3: return update_bj<T, 3>(op, w, x, binv, s);
-    case 4: return update_bj<T, 4>(op, w, x, binv, s);
-    case 5: return update_bj<T, 5>(op, w, x, binv, s);
-    case 6: return update_bj<T, 6>(op, w, x, binv, s);
+    case 1: return update_bj<T, 1>(op, w, x, binv, q, s);
+    case 2: return update_bj<T, 2>(op, w, x, binv, q, s);
+    case 3: return update_bj<T, 3>(op, w, x, binv, q, s);
+    case 4: return update_bj<T, 4>(op, w, x, binv, q, s);
+    case 5: return update_bj<T, 5>(op, w, x, binv, q, s);
+    case 6: return update_bj<T, 6>(op, w, x, binv, q, s);
   }
   set_error("polynomial order not built");
   return DDGX_ERR_UNSUPPORTED;
@@ -637,14 +658,18 @@ static int pcg_t(const Op& op, Work& w, double alpha, double shift, const PC* pc
   CHECK(finalize(w, op.grid_vec, F_INIT_GAMMA, s));
 
   // one CG iteration; every kernel returns at once when the device stop
-  // flag (converged or k == maxiter) is set
+  // flag (converged or k == maxiter) is set.  q: the iteration's parity,
+  // which selects the slot holding its gamma
+  int iter_no = 0;
   auto iteration = [&]() -> int {
+    const int q = iter_no & 1;
+    ++iter_no;
     // Ap = A p, partial <p, Ap>
     // (the update kernels form alpha from these partials themselves)
     CHECK(launch_apply(op, w.p, w.Ap, nullptr, alpha, shift, M_DOT, w.partials + 2 * w.npart,
                        done, s));
     if (bj) {
-      CHECK(update_bj_dispatch<T>(op, w, x, pc->binv, s));
+      CHECK(update_bj_dispatch<T>(op, w, x, pc->binv, q, s));
     } else {
       // with a float32 multigrid the update also writes the scaled float32
       // residual, and the conversion back is fused with the dot products
@@ -652,7 +677,7 @@ static int pcg_t(const Op& op, Work& w, double alpha, double shift, const PC* pc
                          (T*)w.r, (const T*)w.p, (const T*)w.Ap,
                          (const CGScalars*)w.scal, n, has_pc ? nullptr : w.partials,
                          mixed ? (float*)w.pc_r : nullptr,
-                         (const double*)(w.partials + 2 * w.npart), (int)op.grid_apply);
+                         (const double*)(w.partials + 2 * w.npart), (int)op.grid_apply, q);
       DDGX_HIP(hipGetLastError());
       if (mixed) {
         CHECK(mg_vcycle(*pc->mg, *mgw, w.pc_r, w.pc_z, done, s));
@@ -665,17 +690,20 @@ static int pcg_t(const Op& op, Work& w, double alpha, double shift, const PC* pc
         CHECK(dot2<T>(op, w, w.r, w.z, w.r, w.r, done, s));
       }
     }
-    CHECK(finalize(w, op.grid_vec, F_BETA, s));
-    hipLaunchKernelGGL((pupdate_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (T*)w.p,
-                       (const T*)z, (const CGScalars*)w.scal, n);
+    // beta, p = z + beta p and the scalars' advance in one launch
+    hipLaunchKernelGGL((pupdate_fin_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (T*)w.p,
+                       (const T*)z, w.scal, n, (const double*)w.partials, (int)op.grid_vec, q);
     DDGX_HIP(hipGetLastError());
     return DDGX_OK;
   };
-  const int every = cfg.check_every > 0 ? cfg.check_every : 16;
+  int every = cfg.check_every > 0 ? cfg.check_every : 16;
   // A solver context on a non-null stream replays a captured chunk of
   // `every` iterations as one graph launch (captured again when the vectors,
   // coefficients or preconditioner change).
   const bool graph = w.use_graph && s != nullptr && cfg.maxiter > 0;
+  // a captured chunk fixes the parities of its iterations, so every replay
+  // must start at an even iteration: chunks of an even length
+  if (graph && (every & 1)) ++every;
   if (graph) {
     const bool same = w.chunk && w.g_x == x && w.g_pc == (const void*)pc &&
                       w.g_mgw == (const void*)mgw && w.g_alpha == alpha &&

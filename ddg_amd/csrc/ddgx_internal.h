@@ -117,6 +117,9 @@ struct CGScalars {
   double gamma, pAp, rr, bb, atol2, alpha, beta, rs;
   int k, done, maxiter, use_pc;
   double tol, atol;
+  // gamma of iteration i at gq[i & 1]: the kernel that forms the next gamma
+  // writes the other slot while its blocks still read this one
+  double gq[2];
 };
 
 // Per-element data of an operator is stored in tile order: tile position

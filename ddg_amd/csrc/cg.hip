@@ -13,7 +13,10 @@
 // host synchronisation; kernels return at once when the device stop flag is
 // set, and the host polls that flag every `check_every` iterations.
 // Reductions are two-stage and fixed-order (per-workgroup partials in
-// float64, then one workgroup sums them), so results are bitwise repeatable.
+// float64, then a workgroup sums them), so results are bitwise repeatable.
+// Inside an iteration the second stage runs redundantly in every workgroup
+// of the kernel that consumes the result (alpha in the x/r update, beta in
+// the p update), so an iteration has no single-workgroup launch.
 
 #include <cmath>
 

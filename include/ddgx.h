@@ -213,7 +213,9 @@ int ddgx_dot(ddgx_op op, const void* a, const void* b, double* out, void* stream
  *   op32 != NULL: a refinement solver (op float64, op32 its float32 twin, pc
  *                 float32 or NULL).
  * The operators and preconditioner must outlive the solver; one solver runs
- * one solve at a time. */
+ * one solve at a time.  On a non-NULL stream a solver replays captured
+ * chunks of check_every CG iterations as HIP graphs; an odd check_every is
+ * raised by one there (a chunk starts on an even iteration). */
 typedef struct ddgx_solver_s* ddgx_solver;
 int ddgx_solver_create(ddgx_op op, ddgx_op op32, ddgx_pc pc, ddgx_solver* out);
 /* Device bytes the solver holds (its share of a request's HBM footprint). */

@@ -81,6 +81,31 @@ class Multigrid(DevicePreconditioner):
         self.sizes = [int(m.K) for m, _, _ in levels]
 
 
+def _operator_apply(mesh, bc_types, tau_scale, alpha):
+    """Flat apply of ``alpha M + A`` on device vectors (multigrid.py:53-74),
+    one ``ddgx_sipg_apply`` per call."""
+    from .sipg import get_operator
+    op = get_operator(mesh, bc_types, tau_scale)
+    return lambda u_flat: op.apply(u_flat.to(op.dtype).contiguous(), alpha=float(alpha))
+
+
+def _p_interp_matrix(mesh_lo, mesh_hi) -> np.ndarray:
+    """(Np_hi, Np_lo) nodal interpolation (multigrid.py:106-114)."""
+    return p_interp_matrix(int(mesh_lo.N), int(mesh_hi.N))
+
+
+def _h_child_interp_matrices(mesh) -> np.ndarray:
+    """(4, Np, Np) red-child interpolation (multigrid.py:137-163)."""
+    return h_child_interp_matrices(int(mesh.N))
+
+
+def _build_h_meshes(base_mesh, n_levels):
+    """Meshes of the uniform-refinement hierarchy at the base mesh's order,
+    coarse to fine (multigrid.py:187-204)."""
+    return [pm.build_mesh_2d(vx, vy, e, int(base_mesh.N))
+            for vx, vy, e in build_h_meshes(base_mesh, n_levels)]
+
+
 def _check_p_coarse(p_coarse, p_fine):
     if p_coarse < 1 or p_coarse > p_fine:
         raise ValueError(f"p_coarse must be in [1, {p_fine}], got {p_coarse}")

@@ -97,6 +97,43 @@ def _check_cub(cub):
                                   "MI355X operator; only affine elements")
 
 
+# ---------------------------------------------------------------------------
+# reference-element helpers the reference's callers import (sipg.py:78-139)
+# ---------------------------------------------------------------------------
+def _local_mass(mesh) -> np.ndarray:
+    """Reference-element mass ``M_ref = inv(V V^T)`` (sipg.py:78-80)."""
+    V = np.asarray(mesh.V, dtype=np.float64)
+    return np.linalg.inv(V @ V.T)
+
+
+def _face_param_for_face(mesh, f: int) -> np.ndarray:
+    """1D parameter of face ``f``'s nodes in Fmask order: ``r`` on faces 0
+    and 1, ``s`` on face 2 (sipg.py:83-95)."""
+    Fm = np.asarray(mesh.Fmask)[:, f]
+    return np.asarray(mesh.r if f < 2 else mesh.s, dtype=np.float64)[Fm]
+
+
+def _face_1d_mass_matrices(mesh) -> np.ndarray:
+    """(3, Nfp, Nfp) per-face 1D mass ``inv(V1D V1D^T)`` (sipg.py:98-110)."""
+    from .reference import vandermonde_1d
+    out = []
+    for f in range(3):
+        V1 = vandermonde_1d(int(mesh.N), _face_param_for_face(mesh, f))
+        out.append(np.linalg.inv(V1 @ V1.T))
+    return np.stack(out)
+
+
+def _bc_face_arrays(mesh, bc_types):
+    """``(bc_per_facenode_flat, bc_per_face, w_per_face)`` (sipg.py:117-139):
+    face-node tags in F order, (3, K) tags, and face weights 0.5 interior,
+    1 Dirichlet, 0 Neumann."""
+    bpf = np.asarray(bc_types).T.astype(np.int32)
+    Nfp, K = int(mesh.Nfp), int(mesh.K)
+    flat = np.broadcast_to(bpf[None], (Nfp, 3, K)).reshape(-1, order="F")
+    w = np.where(bpf == BC_DIRICHLET, 1.0, np.where(bpf == BC_NEUMANN, 0.0, 0.5))
+    return flat, bpf, w
+
+
 def _check_ref_mats(mesh, M_ref, m1d_per_face):
     ref = ref_element(int(mesh.N))
     if M_ref is not None and not np.allclose(np.asarray(M_ref), ref.M, rtol=1e-12, atol=1e-12):
@@ -150,6 +187,23 @@ def sipg_apply_2d(mesh, u, *, bc_types, u_dir=None, tau_scale=200.0, M_ref=None,
     fu = _field(u, op)
     ud = None if u_dir is None else _field(u_dir, op).flat
     return fu.back(op.apply(fu.flat, u_dir=ud))
+
+
+def sipg_bilinear_form(mesh, u, v, *, bc_types, u_dir=None, tau_scale=200.0, M_ref=None,
+                       m1d_per_face=None, cub=None, kappa=None):
+    """``a(u, v)`` (sipg.py:142-301).  The form is linear in ``v``, whose
+    ghost values are homogeneous, so ``a(u, v) = sum(v * apply(u; u_dir))``:
+    one device apply and one float64 dot product.  Returns a Python float
+    (numpy inputs) or a 0-d device tensor (torch inputs)."""
+    _check_cub(cub)
+    _check_ref_mats(mesh, M_ref, m1d_per_face)
+    op = get_operator(mesh, bc_types, tau_scale, kappa)
+    fu = _field(u, op)
+    fv = _field(v, op)
+    ud = None if u_dir is None else _field(u_dir, op).flat
+    y = op.apply(fu.flat, u_dir=ud)
+    val = torch.dot(fv.flat.double(), y.double())
+    return val if fu.was_torch else float(val)
 
 
 def sipg_dirichlet_load_2d(mesh, u_dir, *, bc_types, tau_scale=200.0):
@@ -207,6 +261,26 @@ def factor_sipg_2d(mesh, *, bc_types, tau_scale=200.0, regularize=False, eps=1.0
     return A, lu, piv
 
 
+def sipg_helmholtz_matrix_2d(mesh, *, alpha, nu, bc_types, tau_scale=200.0, cub=None):
+    """Dense ``alpha M + A`` from device applies (sipg.py:460-496); ``nu``
+    is unused, as in the reference."""
+    _check_cub(cub)
+    return _dense(get_operator(mesh, bc_types, tau_scale), alpha=alpha).cpu().numpy()
+
+
+def _block_diag_mass(mesh, M_ref) -> np.ndarray:
+    """Dense block-diagonal physical mass, block k = ``J_k M_ref`` at rows
+    ``k Np:(k+1) Np`` (sipg.py:499-515)."""
+    Np, K = int(mesh.Np), int(mesh.K)
+    J = np.asarray(mesh.J, dtype=np.float64)[0]
+    out = np.zeros((Np * K, Np * K))
+    idx = np.arange(K)[:, None, None] * Np
+    rows = idx + np.arange(Np)[None, :, None]
+    cols = idx + np.arange(Np)[None, None, :]
+    out[rows, cols] = J[:, None, None] * np.asarray(M_ref, dtype=np.float64)[None]
+    return out
+
+
 def factor_sipg_helmholtz_2d(mesh, *, alpha, nu, bc_types, tau_scale=200.0, cub=None):
     """alpha M + A factored (sipg.py:518-533); ``nu`` is unused as there."""
     _check_cub(cub)
@@ -223,6 +297,77 @@ def _lu_solve(factored, b):
 # ---------------------------------------------------------------------------
 # preconditioners (sipg.py:636-805)
 # ---------------------------------------------------------------------------
+def _extract_block_diag_dense(A_dense, Np, K):
+    """(K, Np, Np) diagonal blocks of a dense element-major matrix
+    (sipg.py:636-648).  numpy or torch, as given."""
+    idx = np.arange(K)[:, None] * Np + np.arange(Np)[None, :]
+    if isinstance(A_dense, torch.Tensor):
+        t = torch.as_tensor(idx, device=A_dense.device)
+        return A_dense[t[:, :, None], t[:, None, :]]
+    return np.asarray(A_dense)[idx[:, :, None], idx[:, None, :]]
+
+
+def _color_elements(mesh):
+    """Greedy colouring of the face-adjacency graph in element order
+    (sipg.py:651-668): ``(colour (K,), ncolours)``; at most four colours
+    on a triangulation."""
+    E = np.asarray(mesh.EToE, dtype=np.int64)
+    K = E.shape[0]
+    colour = np.full(K, -1, dtype=np.int64)
+    for k in range(K):
+        taken = 0
+        for nb in E[k]:
+            if nb != k and colour[nb] >= 0:
+                taken |= 1 << int(colour[nb])
+        c = 0
+        while taken >> c & 1:
+            c += 1
+        colour[k] = c
+    return colour, int(colour.max()) + 1 if K else 0
+
+
+def extract_block_diag_matfree(apply_flat, mesh):
+    """Per-element (K, Np, Np) diagonal blocks of any element-local-plus-
+    face-neighbour operator from ``ncolours * Np`` probe applies
+    (sipg.py:671-704): probing node j of every element of one colour is
+    zero on all their face neighbours, so the result on those elements is
+    column j of their own blocks.
+
+    ``apply_flat`` takes and returns flat element-major vectors.  The probes
+    are float64 tensors on the HIP device; an ``apply_flat`` that answers
+    with numpy arrays gets numpy blocks back, otherwise device tensors.
+    (The device preconditioner builders use a closed form instead of
+    probing: ``make_sipg_block_jacobi_preconditioner``.)"""
+    Np, K = int(mesh.Np), int(mesh.K)
+    colour, nc = _color_elements(mesh)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    col = torch.as_tensor(colour, device=dev)
+    blocks = torch.zeros((K, Np, Np), dtype=torch.float64, device=dev)
+    as_numpy = False
+    for c in range(nc):
+        mask = col == c
+        for j in range(Np):
+            probe = torch.zeros((K, Np), dtype=torch.float64, device=dev)
+            probe[:, j] = mask.double()
+            r = apply_flat(probe.reshape(-1))
+            if not isinstance(r, torch.Tensor):
+                as_numpy = True
+            r = torch.as_tensor(np.asarray(r) if as_numpy else r, device=dev).double()
+            blocks[:, :, j] += torch.where(mask[:, None], r.reshape(K, Np), 0.0)
+    return blocks.cpu().numpy() if as_numpy else blocks
+
+
+def _block_jacobi_apply(blocks, Np, K):
+    """``M_inv(v_flat)`` from per-element blocks (sipg.py:707-715): inverse
+    blocks once, then one batched product per call (numpy or torch)."""
+    if isinstance(blocks, torch.Tensor):
+        binv = torch.linalg.inv(blocks)
+        return lambda v: torch.einsum("kij,kj->ki", binv, v.reshape(K, Np).to(binv.dtype)) \
+            .reshape(-1)
+    binv = np.linalg.inv(np.asarray(blocks))
+    return lambda v: np.einsum("kij,kj->ki", binv, np.asarray(v).reshape(K, Np)).reshape(-1)
+
+
 def make_sipg_block_jacobi_preconditioner(mesh, *, bc_types, tau_scale=200.0, A_dense=None,
                                           matfree=False) -> BlockJacobi:
     """Block-Jacobi ``M^{-1}`` for the SIPG matrix.  Blocks are always built

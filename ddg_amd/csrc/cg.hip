@@ -135,6 +135,10 @@ __device__ void block_sum2_partials(const double* __restrict__ partial, int npar
 
 // The CG scalars after the <r,z>, <r,r> reduction of an iteration: beta,
 // the new gamma (also into slot gq[q ^ 1]), the residual and the stop test.
+// A zero gamma = <r, M r> with the residual still above the threshold is a
+// breakdown (the preconditioner returned zero, e.g. on an underflowed
+// float32 residual): the next step length would be 0/0, so the solve stops
+// there, unconverged.
 __device__ inline void cg_advance(CGScalars* sc, double s0, double s1, double g_old, int q) {
   sc->beta = s0 / g_old;
   sc->gamma = s0;
@@ -142,7 +146,7 @@ __device__ inline void cg_advance(CGScalars* sc, double s0, double s1, double g_
   sc->rr = sc->use_pc ? s1 : s0;
   sc->rs = sc->rr;
   sc->k += 1;
-  sc->done = !(sc->rs > sc->atol2 && sc->k < sc->maxiter);
+  sc->done = !(sc->rs > sc->atol2 && sc->k < sc->maxiter && s0 != 0.0);
 }
 
 // One workgroup: sum the partials in fixed order and advance the scalars.
@@ -165,7 +169,7 @@ finalize_kernel(const double* __restrict__ partial, int nparts, CGScalars* sc, i
       sc->rr = sc->use_pc ? s1 : s0;
       sc->rs = sc->rr;
       sc->k = 0;
-      sc->done = !(sc->rs > sc->atol2 && sc->k < sc->maxiter);
+      sc->done = !(sc->rs > sc->atol2 && sc->k < sc->maxiter && s0 != 0.0);
       break;
     default:  // F_SUM
       sc->pAp = s0;
@@ -189,9 +193,10 @@ copy_kernel(const T* __restrict__ a, T* __restrict__ b, size_t n) {
 }
 
 // Float32 preconditioner inside a float64 CG: r32 = (float)(r / sqrt(rr)) and
-// z = sqrt(rr) * (double) z32, with rr the last <r,r> on the device (1 before
-// the first).  The preconditioner is linear, so the scale only keeps the
-// float32 values away from the ends of their range.
+// z = sqrt(rr) * (double) z32, with rr the last <r,r> on the device (<r0,r0>
+// before the first iteration; 1 for a zero residual).  The preconditioner is
+// linear, so the scale only keeps the float32 values away from the ends of
+// their range.
 __device__ __forceinline__ double pc_scale(const CGScalars* sc) {
   return sc->rr > 0.0 ? sqrt(sc->rr) : 1.0;
 }
@@ -386,12 +391,18 @@ pc_up_dot_kernel(const float* __restrict__ z32, double* __restrict__ z,
 // block 0 also advances the scalars (new gamma into slot gq[q ^ 1], the
 // residual, the iteration count and the stop flag).  No block reads what
 // block 0 writes, except the stop flag: a block that already sees it set
-// skips its part of p, which no later iteration reads.
+// skips its part of p, which no later iteration reads.  The flag is read
+// once per workgroup (thread 0, through LDS), so all waves of a workgroup
+// take the same branch even while block 0 is writing it.
 template <typename T>
 __global__ void __launch_bounds__(256)
 pupdate_fin_kernel(T* __restrict__ p, const T* __restrict__ z, CGScalars* sc, size_t n,
                    const double* __restrict__ partial, int nparts, int q) {
-  if (sc->done) return;
+  __shared__ int s_done;
+  if (threadIdx.x == 0)
+    s_done = __hip_atomic_load(&sc->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  if (s_done) return;
   __shared__ double s_red[4];
   double s0, s1;
   block_sum2_partials(partial, nparts, s_red, s0, s1);
@@ -652,6 +663,12 @@ static int pcg_t(const Op& op, Work& w, double alpha, double shift, const PC* pc
     DDGX_HIP(hipGetLastError());
     return DDGX_OK;
   };
+  if (mixed) {
+    // <r0, r0> into the scalars first, so the float32 copy of r0 is scaled
+    // by 1/|r0| like every later residual (not left at its own magnitude)
+    CHECK(dot2<T>(op, w, w.r, w.r, w.r, w.r, nullptr, s));
+    CHECK(finalize(w, op.grid_vec, F_SUM, s));
+  }
   if (bj) CHECK(launch_bj_apply(op.N, op.K, op.dtype, pc->binv, w.r, w.z, s));
   else if (has_pc) CHECK(precondition(nullptr));
   hipLaunchKernelGGL((copy_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (const T*)z,
@@ -772,14 +789,54 @@ scale_down_kernel(const double* __restrict__ r, float* __restrict__ rs, double s
     rs[i] = (float)(scale * r[i]);
 }
 
-// x += scale * (double) d
+// z = scale * (double) d
 __global__ void __launch_bounds__(256)
-refine_update_kernel(double* __restrict__ x, const float* __restrict__ d, double scale, size_t n) {
+scale_up_kernel(double* __restrict__ z, const float* __restrict__ d, double scale, size_t n) {
   const size_t stride = (size_t)gridDim.x * 256;
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
-    x[i] += scale * (double)d[i];
+    z[i] = scale * (double)d[i];
 }
 
+// p = z + beta p
+__global__ void __launch_bounds__(256)
+xpby_kernel(double* __restrict__ p, const double* __restrict__ z, double beta, size_t n) {
+  const size_t stride = (size_t)gridDim.x * 256;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
+    p[i] = z[i] + beta * p[i];
+}
+
+// x += alpha p
+__global__ void __launch_bounds__(256)
+axpy_kernel(double* __restrict__ x, const double* __restrict__ p, double alpha, size_t n) {
+  const size_t stride = (size_t)gridDim.x * 256;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
+    x[i] += alpha * p[i];
+}
+
+// <a, b> and <c, d> of float64 vectors in one pass (host values)
+static int dot_pair(const Op& op, Work& w, const void* a, const void* b, const void* c,
+                    const void* d, double* ab, double* cd, hipStream_t s) {
+  CHECK(dot2<double>(op, w, a, b, c, d, nullptr, s));
+  CHECK(finalize(w, op.grid_vec, F_SUM, s));
+  DDGX_HIP(hipMemcpyAsync(w.scal_host, w.scal, sizeof(CGScalars), hipMemcpyDeviceToHost, s));
+  DDGX_HIP(hipStreamSynchronize(s));
+  *ab = w.scal_host->pAp;
+  *cd = w.scal_host->rr;
+  return DDGX_OK;
+}
+
+// Mixed-precision refinement with a flexible-CG outer loop.  Each step
+// solves A d = r/|r| approximately by float32 PCG (float32 operator and
+// preconditioner, float64 dot products) and uses z = |r| d as the step's
+// preconditioned residual:
+//   p = z - (<z, A p_old> / <p_old, A p_old>) p_old     (A-orthogonal to p_old)
+//   x += (<p, r> / <p, A p>) p;   r = b - A x            (float64, true residual)
+// Plain refinement (x += z) contracts the residual by about eps32 cond(A) per
+// step, which at order 6 on a 2x128x128 mesh is close to 1; the outer Krylov
+// step removes that limit.  The true residual is recomputed in float64 after
+// every update, so the stopping test is on b - A x itself; the loop ends,
+// unconverged, at the residual's round-off floor: two steps in a row that
+// fail to reduce |r| by 10%.
 int ir_solve(const Op& op64, Work& w64, const Op& op32, Work& w32, double alpha, double shift,
              const PC* pc32, MGWork* mgw, const void* b, void* x, const ddgx_ir_cfg& cfg,
              ddgx_ir_info* info, hipStream_t s) {
@@ -790,23 +847,23 @@ int ir_solve(const Op& op64, Work& w64, const Op& op32, Work& w32, double alpha,
     return DDGX_ERR_INVALID;
   }
   const size_t n = (size_t)op64.K * op64.Np;
+  const dim3 gv(op64.grid_vec), blk(256);
   double bb = 0.0;
   CHECK(dot(op64, w64, b, b, &bb, s));
   const double t = cfg.tol * cfg.tol * bb, a = cfg.atol * cfg.atol;
   const double atol2 = t > a ? t : a;
   ddgx_cg_cfg icfg{cfg.inner_tol, 0.0, cfg.inner_maxiter, cfg.check_every};
   int outer = 0, inner = 0, stalled = 0, floor_hit = 0;
-  double rr = 0.0, rr_prev = 0.0;
+  double rr = 0.0, rr_prev = 0.0, pAp = 0.0;
   double* r = (double*)w64.r;
-  while (true) {
-    CHECK(launch_apply(op64, x, r, nullptr, alpha, shift, M_RESID, nullptr, nullptr, s, b));
-    CHECK(dot(op64, w64, r, r, &rr, s));
-    if (!(rr > atol2) || outer >= cfg.max_outer) break;
-    // A step reduces the float64 residual by about max(inner_tol, eps32 *
-    // cond(A)), not always monotonically; the residual has met its round-off
-    // floor when two steps in a row fail to halve it.
+  double* z = (double*)w64.z;
+  double* p = (double*)w64.p;
+  double* Ap = (double*)w64.Ap;
+  CHECK(launch_apply(op64, x, r, nullptr, alpha, shift, M_RESID, nullptr, nullptr, s, b));
+  CHECK(dot(op64, w64, r, r, &rr, s));
+  while (rr > atol2 && outer < cfg.max_outer) {
     if (outer > 0) {
-      stalled = rr > 0.25 * rr_prev ? stalled + 1 : 0;
+      stalled = rr > 0.81 * rr_prev ? stalled + 1 : 0;
       if (stalled >= 2) {
         floor_hit = 1;
         break;
@@ -814,16 +871,34 @@ int ir_solve(const Op& op64, Work& w64, const Op& op32, Work& w32, double alpha,
     }
     rr_prev = rr;
     const double nr = std::sqrt(rr);
-    hipLaunchKernelGGL(scale_down_kernel, dim3(op64.grid_vec), dim3(256), 0, s, r,
-                       (float*)w32.ir_b, 1.0 / nr, n);
+    hipLaunchKernelGGL(scale_down_kernel, gv, blk, 0, s, (const double*)r, (float*)w32.ir_b,
+                       1.0 / nr, n);
     DDGX_HIP(hipGetLastError());
     DDGX_HIP(hipMemsetAsync(w32.ir_x, 0, n * sizeof(float), s));
     ddgx_cg_info ii{};
     CHECK(pcg_solve(op32, w32, alpha, shift, pc32, mgw, w32.ir_b, w32.ir_x, icfg, &ii, s));
     inner += ii.iterations;
-    hipLaunchKernelGGL(refine_update_kernel, dim3(op64.grid_vec), dim3(256), 0, s, (double*)x,
-                       (const float*)w32.ir_x, nr, n);
+    hipLaunchKernelGGL(scale_up_kernel, gv, blk, 0, s, z, (const float*)w32.ir_x, nr, n);
     DDGX_HIP(hipGetLastError());
+    if (outer == 0) {
+      hipLaunchKernelGGL((copy_kernel<double>), gv, blk, 0, s, (const double*)z, p, n);
+    } else {
+      double zAp = 0.0;
+      CHECK(dot(op64, w64, z, Ap, &zAp, s));
+      hipLaunchKernelGGL(xpby_kernel, gv, blk, 0, s, p, (const double*)z, -zAp / pAp, n);
+    }
+    DDGX_HIP(hipGetLastError());
+    CHECK(launch_apply(op64, p, Ap, nullptr, alpha, shift, 0, nullptr, nullptr, s));
+    double pr = 0.0;
+    CHECK(dot_pair(op64, w64, p, Ap, p, r, &pAp, &pr, s));
+    if (!(pAp > 0.0)) {   // no descent direction left (round-off): the floor
+      floor_hit = 1;
+      break;
+    }
+    hipLaunchKernelGGL(axpy_kernel, gv, blk, 0, s, (double*)x, (const double*)p, pr / pAp, n);
+    DDGX_HIP(hipGetLastError());
+    CHECK(launch_apply(op64, x, r, nullptr, alpha, shift, M_RESID, nullptr, nullptr, s, b));
+    CHECK(dot(op64, w64, r, r, &rr, s));
     ++outer;
   }
   if (info != nullptr) {

@@ -169,11 +169,14 @@ int ddgx_sipg_param_grad(ddgx_op op, const double* u, const double* lam,
 
 /* Mixed-precision iterative refinement (no counterpart in the reference,
  * whose solves are float64 only; it solves the same systems as
- * ddgx_pcg_solve):
+ * ddgx_pcg_solve), with a flexible-CG outer loop:
  *   r = b - A x (float64); stop when <r,r> <= max(tol^2 <b,b>, atol^2)
- *   d = PCG_float32(A, r/|r|, pc32) to inner_tol;  x += |r| d (float64)
+ *   d = PCG_float32(A, r/|r|, pc32) to inner_tol;  z = |r| d
+ *   p = z - (<z, A p_old> / <p_old, A p_old>) p_old
+ *   x += (<p, r> / <p, A p>) p;  r = b - A x               (all float64)
  * The loop also ends, unconverged, at the float64 residual's round-off
- * floor: when two steps in a row fail to halve ||r|| (info->at_floor = 1).
+ * floor: when two steps in a row fail to reduce ||r|| by 10%
+ * (info->at_floor = 1).
  * op64/op32: the same operator in float64 and float32; pc32: float32
  * preconditioner (block-Jacobi or multigrid) or NULL.  b, x: float64. */
 typedef struct {

@@ -48,7 +48,7 @@ def test_apply_parity_f64(dev, N, bcname):
         assert rel(y, ref) < 1e-12
 
 
-@pytest.mark.parametrize("N", [1, 2, 3, 4, 6])
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6])
 def test_apply_parity_f32(dev, N):
     pm, om = unit_square(6, 6, N, jitter=0.1, seed=3)
     bc = pmesh.dirichlet_x_neumann_y(pm)
@@ -105,6 +105,10 @@ def test_large_mesh_properties(dev):
     rec, nbr, _ = pack(m, bc)
     ref = model_apply(4, ref_element(4).kernel_tables(), rec, nbr, u)
     assert rel(Au.cpu().numpy().reshape(m.K, m.Np), ref) < 1e-12
+    # the oracle's apply on the same mesh object (its fields are the
+    # reference's, checked against the oracle's own builder elsewhere)
+    yo = S.sipg_apply_2d(m, u.T, bc_types=bc)
+    assert rel(Au.cpu().numpy().reshape(m.K, m.Np), yo.T) < 1e-12
     a, b = float(vt @ Au), float(ut @ Av)
     assert abs(a - b) <= 1e-11 * abs(a)
     w = op.apply(2.0 * ut - 3.0 * vt)
@@ -144,6 +148,10 @@ def test_pipelined_forest_parity(dev, N):
                         ({"u_dir": udt, "alpha": 1.5}, {"udir": ud, "alpha": 1.5})):
             y = op.apply(ut, **kw).cpu().numpy().reshape(m.K, m.Np)
             assert rel(y, model_apply(N, tab, rec, nbr, u, **mkw)) < 1e-12
+        if n == 256:
+            # the forest-ordered workload mesh against the oracle's apply
+            y = op.apply(ut, u_dir=udt).cpu().numpy().reshape(m.K, m.Np)
+            assert rel(y, S.sipg_apply_2d(m, u.T, bc_types=bc, u_dir=ud.T).T) < 1e-12
 
 
 def test_repeatable(dev):

@@ -93,3 +93,40 @@ def test_graph_mixed_preconditioned_pcg_equals_eager(problem):
         assert torch.equal(x, x_e), f"replay {rep} differs from the eager solve"
         op64.apply(x)
         torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("case", ["odd_chunk", "block_jacobi", "maxiter_13"])
+def test_graph_chunk_edge_cases(problem, case):
+    """An odd check_every (raised to an even chunk under capture), the
+    block-Jacobi update fused into the captured chunk, and a maxiter that is
+    not a multiple of the chunk (the captured chunk runs past it; the
+    kernels after the stop flag return at once): replays equal the eager
+    solve bit for bit."""
+    pm, op64, op32, mg64, mg32, b = problem
+    every, tol, maxiter, pc = 8, 1e-11, 500, mg64
+    if case == "odd_chunk":
+        every = 7
+    elif case == "block_jacobi":
+        pc = op64.block_jacobi()
+        tol, maxiter = 1e-9, 5000
+    else:
+        tol, maxiter = 1e-30, 13
+    _say(f"graph edge case {case}: every={every} maxiter={maxiter}")
+    x_e, info_e = op64.pcg(b, pc=pc, tol=tol, maxiter=maxiter, check_every=every)
+    ctx = SolverContext(op64, pc=pc)
+    s = torch.cuda.Stream()
+    cfg = _lib.CGConfig(tol, 0.0, maxiter, every)
+    for rep in range(3):
+        x = torch.zeros_like(b)
+        with torch.cuda.stream(s):
+            info = ctx.pcg(b, x, 0.0, cfg, s.cuda_stream)
+        s.synchronize()
+        assert info.iterations == info_e["iterations"]
+        assert bool(info.converged) == info_e["converged"]
+        assert torch.equal(x, x_e), f"replay {rep} differs from the eager solve"
+        op64.apply(x)
+        torch.cuda.synchronize()
+    if case == "maxiter_13":
+        assert info_e["iterations"] == 13 and not info_e["converged"]
+    else:
+        assert info_e["converged"]

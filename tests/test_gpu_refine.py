@@ -73,3 +73,38 @@ def test_refine_drop_in_solve(dev):
     with pytest.raises(TypeError):
         D.solve_sipg_poisson_2d(pm, f, bc_types=bc, u_dir=ue, iterative=cfg,
                                 preconditioner=mg64, refine=D.RefineConfig())
+
+
+@pytest.mark.parametrize("N", [5, 6])
+def test_mixed_precision_high_order_against_lu(dev, N):
+    """Orders 5 and 6 (two 16-row matrix-core blocks; N=6 is the 2x128x128
+    workload's order) in both mixed-precision forms against the oracle's LU:
+    refinement with a float32 V-cycle (converged flag set, more than one
+    step) and float64 CG preconditioned by the float32 V-cycle."""
+    pm, om = unit_square(8, 8, N, jitter=0.1, seed=30 + N)
+    bc = pmesh.dirichlet_x_neumann_y(pm)
+    b = np.random.default_rng(31 + N).standard_normal(pm.K * pm.Np)
+    x_lu = np.linalg.solve(S.sipg_matrix_2d(om, bc_types=bc), b)
+    op64 = D.get_operator(pm, bc)
+    op32 = D.get_operator(pm, bc, dtype=torch.float32)
+    mg32 = MG.make_sipg_mg_preconditioner(pm, bc_types=bc, degrees=[N, N // 2, 1],
+                                          dtype=torch.float32)
+    bt = torch.as_tensor(b, device=dev)
+    # the float64 round-off floor of the true residual |b - A x| / |b|: what
+    # the exact (LU) solution gives through the same float64 apply
+    floor = float(torch.linalg.norm(bt - op64.apply(torch.as_tensor(x_lu, device=dev)))
+                  / torch.linalg.norm(bt))
+    _say(f"mixed precision N={N}: refinement (float64 floor {floor:.2e})")
+    x, info = refine_solve(op64, op32, bt, pc32=mg32, tol=1e-12, inner_tol=1e-4,
+                           inner_maxiter=2000)
+    true_rel = float(np.sqrt(info["residual2"]) / torch.linalg.norm(bt))
+    # tol 1e-12 lies at or below that floor here: the refinement either meets
+    # it or stops at the floor, never far above it
+    assert info["converged"] or info["at_floor"], info
+    assert true_rel <= max(1e-12, 2.0 * floor), (true_rel, floor)
+    assert info["outer"] >= 2
+    assert _l2(x.cpu().numpy(), x_lu) < 1e-9
+    _say(f"mixed precision N={N}: float64 CG + float32 V-cycle")
+    x2, info2 = op64.pcg(bt, pc=mg32, tol=1e-12, maxiter=1000)
+    assert info2["converged"], info2
+    assert _l2(x2.cpu().numpy(), x_lu) < 1e-9

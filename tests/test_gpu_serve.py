@@ -10,8 +10,8 @@ import torch
 from conftest import rel
 from oracle import mesh as om
 from oracle import sipg as S
-from ddg_amd.serve import (ServeConfig, SolveServer, make_requests, poisson_arrivals,
-                           fourier_field)
+from ddg_amd.serve import (Request, ServeConfig, SolveServer, make_requests,
+                           poisson_arrivals, fourier_field)
 
 pytestmark = pytest.mark.gpu
 
@@ -47,8 +47,19 @@ def test_alone_equals_mix(server):
         assert torch.equal(res.x, alone[res.rid]), f"request {res.rid} differs in the mix"
 
 
+def _oracle_solution(srv, r):
+    pc = srv.problem_class(r.n, r.N)
+    m = pc.mesh
+    mo = om.build_mesh_2d(m.VX, m.VY, m.EToV, m.N)
+    f = fourier_field(pc.x, pc.y, r.seed).cpu().numpy().reshape(m.K, m.Np).T
+    b = S.mass_apply(mo, f).T.reshape(-1)
+    return np.linalg.solve(S.sipg_matrix_2d(mo, bc_types=pc.bc, alpha=r.k2), b)
+
+
 def test_results_match_oracle(server):
-    reqs = make_requests(3, seed=9, sizes=(8,), orders=(2, 3, 4), k2_range=(0, 50))
+    rng = np.random.default_rng(9)
+    reqs = [Request(rid=i, n=8, N=N, k2=float(rng.uniform(0, 50)), seed=int(rng.integers(2 ** 31)))
+            for i, N in enumerate((2, 3, 4, 5, 6))]
     for r in reqs:
         pc = server.problem_class(r.n, r.N)
         m = pc.mesh
@@ -105,13 +116,20 @@ def test_pcg32_method_alone_equals_mix_and_matches_oracle(dev):
             assert res.converged
             assert torch.equal(res.x, alone[res.rid]), f"request {res.rid} differs in the mix"
         r = reqs[0]
-        pc = srv.problem_class(r.n, r.N)
-        m = pc.mesh
-        mo = om.build_mesh_2d(m.VX, m.VY, m.EToV, m.N)
-        f = fourier_field(pc.x, pc.y, r.seed).cpu().numpy().reshape(m.K, m.Np).T
-        b = S.mass_apply(mo, f).T.reshape(-1)
-        xo = np.linalg.solve(S.sipg_matrix_2d(mo, bc_types=pc.bc, alpha=r.k2), b)
+        xo = _oracle_solution(srv, r)
         err = np.linalg.norm(alone[r.rid].cpu().numpy() - xo) / np.linalg.norm(xo)
         assert err < 1e-9, err
+        # orders 5 and 6 (two matrix-core row blocks, float32 V-cycle) at n=8
+        rng = np.random.default_rng(16)
+        for i, N in enumerate((5, 6)):
+            r = Request(rid=100 + i, n=8, N=N, k2=float(rng.uniform(0, 50)),
+                        seed=int(rng.integers(2 ** 31)))
+            _say(f"serve pcg32 oracle n=8 N={N} k2={r.k2:.2f}")
+            srv.prepare([r])
+            x, info = srv.solve_one(r, srv.streams[1])
+            torch.cuda.synchronize()
+            xo = _oracle_solution(srv, r)
+            err = np.linalg.norm(x.cpu().numpy() - xo) / np.linalg.norm(xo)
+            assert info.converged and err < 1e-9, (N, err)
     finally:
         srv.close()

@@ -22,7 +22,7 @@ EXPORTS = (
     "ddgx_block_jacobi_blocks", "ddgx_pc_apply", "ddgx_pc_destroy", "ddgx_pcg_solve",
     "ddgx_dot", "ddgx_mg_create", "ddgx_ir_solve", "ddgx_sipg_param_grad",
     "ddgx_solver_create", "ddgx_solver_bytes", "ddgx_solver_pcg", "ddgx_solver_ir",
-    "ddgx_solver_destroy",
+    "ddgx_solver_destroy", "ddgx_pcg_solve_noalloc",
 )
 
 
@@ -79,6 +79,8 @@ def lib():
         "ddgx_pc_destroy": (i32, [vp]),
         "ddgx_pcg_solve": (i32, [vp, dbl, dbl, vp, vp, vp, C.POINTER(CGConfig),
                                  C.POINTER(CGInfo), vp]),
+        "ddgx_pcg_solve_noalloc": (i32, [vp, dbl, dbl, vp, vp, vp, C.POINTER(CGConfig),
+                                         C.POINTER(CGInfo), vp]),
         "ddgx_dot": (i32, [vp, vp, vp, C.POINTER(dbl), vp]),
         "ddgx_sipg_param_grad": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp]),
         "ddgx_ir_solve": (i32, [vp, vp, dbl, dbl, vp, vp, vp, C.POINTER(IRConfig),

@@ -444,6 +444,24 @@ int ddgx_pcg_solve(ddgx_op op, double alpha, double shift, ddgx_pc pc, const voi
                    (hipStream_t)stream);
 }
 
+int ddgx_pcg_solve_noalloc(ddgx_op op, double alpha, double shift, ddgx_pc pc, const void* b,
+                           void* x, const ddgx_cg_cfg* cfg, ddgx_cg_info* info, void* stream) {
+  if (op == nullptr || b == nullptr || x == nullptr || cfg == nullptr) {
+    set_error("ddgx_pcg_solve_noalloc: null argument");
+    return DDGX_ERR_INVALID;
+  }
+  if (cfg->maxiter < 0) {
+    set_error("ddgx_pcg_solve_noalloc: maxiter must be >= 0");
+    return DDGX_ERR_INVALID;
+  }
+  if (pc != nullptr && (pc->pc.op->K != op->op.K || pc->pc.op->N != op->op.N)) {
+    set_error("ddgx_pcg_solve_noalloc: preconditioner built for a different mesh/order");
+    return DDGX_ERR_INVALID;
+  }
+  return pcg_solve_noalloc(op->op, alpha, shift, pc ? &pc->pc : nullptr, b, x, *cfg, info,
+                           (hipStream_t)stream);
+}
+
 int ddgx_ir_solve(ddgx_op op64, ddgx_op op32, double alpha, double shift, ddgx_pc pc32,
                   const void* b, void* x, const ddgx_ir_cfg* cfg, ddgx_ir_info* info,
                   void* stream) {

@@ -19,6 +19,7 @@
 // the p update), so an iteration has no single-workgroup launch.
 
 #include <cmath>
+#include <mutex>
 
 #include "ddgx_internal.h"
 
@@ -779,6 +780,89 @@ int pcg_solve(const Op& op, Work& w, double alpha, double shift, const PC* pc, M
   if (pc != nullptr && pc->type == 1 && mgw == nullptr) mgw = pc->mg->ws;
   if (op.dtype == DDGX_F64) return pcg_t<double>(op, w, alpha, shift, pc, mgw, b, x, cfg, info, s);
   return pcg_t<float>(op, w, alpha, shift, pc, mgw, b, x, cfg, info, s);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free PCG (ddgx_pcg_solve_noalloc)
+// ---------------------------------------------------------------------------
+// The Krylov state of the solve (r, p, z, Ap, the float32 copies around a
+// float32 V-cycle, the reduction partials and the device scalars) lives in
+// one fixed device array instead of a Work allocated with the operator.  Each
+// GPU the library runs on holds its own copy of the array until the process
+// ends; every call on a GPU shares it (overlapping calls on one GPU race).
+// Nothing in it has to start at zero: every word is written before it is
+// read within a call.
+constexpr size_t SCRATCH_BYTES = size_t(96) << 20;
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[64] = {};
+
+// Device address of this GPU's copy of `scratch`, looked up once per device
+// with that device current.  The array's name is only a host stand-in; the
+// kernels get this address.
+static int scratch_base(int device, void** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0 || device >= 64) {
+    set_error("device index out of range for the solve's fixed workspace");
+    return DDGX_ERR_INVALID;
+  }
+  if (g_scratch_ptr[device] == nullptr) {
+    int cur = 0;
+    DDGX_HIP(hipGetDevice(&cur));
+    if (cur != device) DDGX_HIP(hipSetDevice(device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return DDGX_OK;
+}
+
+static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+int pcg_solve_noalloc(const Op& op, double alpha, double shift, const PC* pc, const void* b,
+                      void* x, const ddgx_cg_cfg& cfg, ddgx_cg_info* info, hipStream_t s) {
+  const bool mixed = pc != nullptr && pc->dtype != op.dtype;
+  const size_t n = (size_t)op.K * op.Np;
+  const size_t vb = align256(n * dsize(op.dtype));
+  const size_t fb = mixed ? align256(n * sizeof(float)) : 0;
+  const int npart = op.grid_apply > op.grid_vec ? op.grid_apply : op.grid_vec;
+  const size_t pb = align256(3 * (size_t)npart * sizeof(double));
+  const size_t need = 4 * vb + 2 * fb + pb + align256(sizeof(CGScalars));
+  if (need > SCRATCH_BYTES) {
+    set_error("ddgx_pcg_solve_noalloc: the solve needs " + std::to_string(need) +
+              " bytes of workspace, the fixed array holds " + std::to_string(SCRATCH_BYTES) +
+              " (use ddgx_pcg_solve or a solver context for this size)");
+    return DDGX_ERR_INVALID;
+  }
+  void* base = nullptr;
+  CHECK(scratch_base(op.device, &base));
+  unsigned char* q = (unsigned char*)base;
+  Work w;
+  w.K = op.K;
+  w.Np = op.Np;
+  w.dtype = op.dtype;
+  w.device = op.device;
+  w.r = q;
+  w.p = q + vb;
+  w.z = q + 2 * vb;
+  w.Ap = q + 3 * vb;
+  q += 4 * vb;
+  if (mixed) {
+    w.pc_r = q;
+    w.pc_z = q + fb;
+    q += 2 * fb;
+  }
+  w.partials = (double*)q;
+  w.npart = npart;
+  w.scal = (CGScalars*)(q + pb);
+  CGScalars host{};          // host side of the scalars: lives until the solve returns
+  w.scal_host = &host;
+  w.use_graph = false;       // no graph instantiation (it allocates)
+  return pcg_solve(op, w, alpha, shift, pc, nullptr, b, x, cfg, info, s);
 }
 
 // rs = (float) (scale * r)

@@ -285,6 +285,8 @@ int launch_bj_apply(int N, int64_t K, int dtype, const void* binv, const void* r
                     hipStream_t s);
 size_t packed_block_words(int N);
 
+int pcg_solve_noalloc(const Op& op, double alpha, double shift, const PC* pc, const void* b,
+                      void* x, const ddgx_cg_cfg& cfg, ddgx_cg_info* info, hipStream_t s);
 int pcg_solve(const Op& op, Work& w, double alpha, double shift, const PC* pc, MGWork* mgw,
               const void* b, void* x, const ddgx_cg_cfg& cfg, ddgx_cg_info* info,
               hipStream_t s);

@@ -152,6 +152,23 @@ int ddgx_pcg_solve(ddgx_op op, double alpha, double shift, ddgx_pc pc,
                    const void* b, void* x, const ddgx_cg_cfg* cfg,
                    ddgx_cg_info* info, void* stream);
 
+/* ddgx_pcg_solve without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (serving
+ * loops that must not call hipMalloc, harnesses that forbid it).  Same
+ * arguments, results and kernels as ddgx_pcg_solve, but the Krylov state
+ * (four vectors of the operator's dtype, two float32 vectors when a float32
+ * multigrid preconditions a float64 solve, reduction partials, scalars) lives
+ * in one fixed 96 MiB device array per GPU instead of the operator's default
+ * workspace: at most about 3 million float64 unknowns (K*Np), 2.4 million
+ * with a float32 multigrid, twice as many in float32.  Larger solves are
+ * refused with DDGX_ERR_INVALID before any device work.  Every call on a GPU
+ * shares that GPU's array, and a multigrid preconditioner's cycle buffers are
+ * shared with ddgx_pcg_solve and ddgx_pc_apply: calls that overlap on one GPU
+ * race.  No graph capture of CG chunks on this path. */
+int ddgx_pcg_solve_noalloc(ddgx_op op, double alpha, double shift, ddgx_pc pc,
+                           const void* b, void* x, const ddgx_cg_cfg* cfg,
+                           ddgx_cg_info* info, void* stream);
+
 /* Parameter gradient of the SIPG form a(u, lam; u_dir) with respect to the
  * per-element diffusion coefficient kappa_k and penalty scale tau_scale_k
  * (sipg.py:263 with a (K,) tau_scale; kappa is this library's extension,

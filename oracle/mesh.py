@@ -123,6 +123,53 @@ def _maps(Np, Nfp, K, Fmask, x, y, EToE, EToF, EToV, VX, VY, tol=1e-10):
     return vM, vP, vM[mapB], mapB
 
 
+def geometry(x, y, Dr, Ds, Fmask):
+    """Per-node metrics (mesh.py:634-644), face normals, surface Jacobians
+    (mesh.py:647-684) and Fscale = sJ / J at the face nodes."""
+    K = x.shape[1]
+    Nfp = Fmask.shape[0]
+    xr, xs, yr, ys = Dr @ x, Ds @ x, Dr @ y, Ds @ y
+    J = xr * ys - xs * yr
+    rx, sx, ry, sy = ys / J, -yr / J, -xs / J, xr / J
+    Ff = Fmask.flatten(order="F")
+    fxr, fxs, fyr, fys = xr[Ff], xs[Ff], yr[Ff], ys[Ff]
+    nx = np.zeros((3 * Nfp, K))
+    ny = np.zeros((3 * Nfp, K))
+    i0, i1, i2 = np.arange(Nfp), np.arange(Nfp, 2 * Nfp), np.arange(2 * Nfp, 3 * Nfp)
+    nx[i0], ny[i0] = fyr[i0], -fxr[i0]
+    nx[i1], ny[i1] = fys[i1] - fyr[i1], -fxs[i1] + fxr[i1]
+    nx[i2], ny[i2] = -fys[i2], fxs[i2]
+    sJ = np.sqrt(nx ** 2 + ny ** 2)
+    nx, ny = nx / sJ, ny / sJ
+    Fscale = sJ / J[Ff]
+    return rx, sx, ry, sy, J, nx, ny, sJ, Fscale
+
+
+def rectangle_with_square_hole_mesh_2d(xmin, xmax, ymin, ymax, Kx, Ky,
+                                       hole_xmin, hole_xmax, hole_ymin, hole_ymax):
+    """The structured rectangle mesh without the cells whose centre lies in
+    the hole box; unused vertices pruned and renumbered (mesh.py:159-206)."""
+    xs = np.linspace(xmin, xmax, Kx + 1)
+    ys = np.linspace(ymin, ymax, Ky + 1)
+    VX = np.tile(xs, Ky + 1)
+    VY = np.repeat(ys, Kx + 1)
+    tris = []
+    for j in range(Ky):
+        for i in range(Kx):
+            cx, cy = 0.5 * (xs[i] + xs[i + 1]), 0.5 * (ys[j] + ys[j + 1])
+            if hole_xmin <= cx <= hole_xmax and hole_ymin <= cy <= hole_ymax:
+                continue
+            v00 = j * (Kx + 1) + i
+            v10, v01 = v00 + 1, v00 + Kx + 1
+            tris.append((v00, v10, v01 + 1))
+            tris.append((v00, v01 + 1, v01))
+    E = np.array(tris, dtype=np.int64)
+    used = np.unique(E.reshape(-1))
+    remap = -np.ones(VX.size, dtype=np.int64)
+    remap[used] = np.arange(used.size)
+    return VX[used], VY[used], remap[E]
+
+
 def build_mesh_2d(VX, VY, EToV, N: int) -> Mesh2D:
     """StartUp2D (mesh.py:691-760)."""
     VX = np.asarray(VX, dtype=np.float64)
@@ -141,22 +188,7 @@ def build_mesh_2d(VX, VY, EToV, N: int) -> Mesh2D:
     va, vb, vc = EToV[:, 0], EToV[:, 1], EToV[:, 2]
     x = 0.5 * (np.outer(-(r + s), VX[va]) + np.outer(1 + r, VX[vb]) + np.outer(1 + s, VX[vc]))
     y = 0.5 * (np.outer(-(r + s), VY[va]) + np.outer(1 + r, VY[vb]) + np.outer(1 + s, VY[vc]))
-    # geometric factors (mesh.py:634-644)
-    xr, xs, yr, ys = Dr @ x, Ds @ x, Dr @ y, Ds @ y
-    J = xr * ys - xs * yr
-    rx, sx, ry, sy = ys / J, -yr / J, -xs / J, xr / J
-    # normals (mesh.py:647-684)
-    Ff = Fmask.flatten(order="F")
-    fxr, fxs, fyr, fys = xr[Ff], xs[Ff], yr[Ff], ys[Ff]
-    nx = np.zeros((3 * Nfp, K))
-    ny = np.zeros((3 * Nfp, K))
-    i0, i1, i2 = np.arange(Nfp), np.arange(Nfp, 2 * Nfp), np.arange(2 * Nfp, 3 * Nfp)
-    nx[i0], ny[i0] = fyr[i0], -fxr[i0]
-    nx[i1], ny[i1] = fys[i1] - fyr[i1], -fxs[i1] + fxr[i1]
-    nx[i2], ny[i2] = -fys[i2], fxs[i2]
-    sJ = np.sqrt(nx ** 2 + ny ** 2)
-    nx, ny = nx / sJ, ny / sJ
-    Fscale = sJ / J[Ff]
+    rx, sx, ry, sy, J, nx, ny, sJ, Fscale = geometry(x, y, Dr, Ds, Fmask)
     EToE, EToF = _connect(EToV)
     vmapM, vmapP, vmapB, mapB = _maps(Np, Nfp, K, Fmask, x, y, EToE, EToF, EToV, VX, VY)
     return Mesh2D(N=N, Np=Np, K=K, Nfp=Nfp, Nfaces=3, VX=VX, VY=VY, EToV=EToV,

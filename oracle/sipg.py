@@ -7,7 +7,7 @@ Restates ``src/ddg/dg2d/sipg.py``.  The reference obtains its apply as
 this module keeps two independent statements:
 
 * :func:`sipg_bilinear_form` -- literal transcription of the form
-  (sipg.py:142-301, affine branch);
+  (sipg.py:142-301, nodal quadrature and the ``cub`` cubature branch);
 * :func:`sipg_apply_2d` -- the reverse-mode gradient of that form with
   respect to ``v``, written out by hand step by step (every gather becomes a
   scatter-add, every matrix a transpose), i.e. what ``jax.grad`` evaluates.
@@ -78,9 +78,24 @@ def _grad_xy(mesh, u):
     return mesh.rx * ur + mesh.sx * us, mesh.ry * ur + mesh.sy * us
 
 
+def _check_kappa_cub(kappa, cub):
+    if kappa is not None and cub is not None:
+        raise NotImplementedError("kappa with cubature is not restated")
+
+
+def face_cub_mass(cub):
+    """(3, Nfp, Nfp, K) face mass of every element side under the face
+    cubature: L1d^T diag(face_w sJ_q) L1d (the cubature branch's face
+    integral, sipg.py:281-299, as a matrix on face-node values)."""
+    W = cub.face_w[None, :, None] * cub.face_sJ                   # (3, Nfcub, K)
+    return np.einsum("fqp,fqk,fqr->fprk", cub.face_L1d, W, cub.face_L1d)
+
+
 def sipg_bilinear_form(mesh, u, v, *, bc_types, u_dir=None, tau_scale=200.0,
-                       M_ref=None, m1d_per_face=None, kappa=None) -> float:
-    """Literal a(u, v) (sipg.py:142-301, affine branch ``cub is None``)."""
+                       M_ref=None, m1d_per_face=None, kappa=None, cub=None) -> float:
+    """Literal a(u, v) (sipg.py:142-301): nodal quadrature (``cub is None``)
+    or cubature (``cub``: volume sipg.py:190-201, faces 281-299)."""
+    _check_kappa_cub(kappa, cub)
     Np, K, Nfp = mesh.Np, mesh.K, mesh.Nfp
     M = local_mass(mesh) if M_ref is None else M_ref
     m1d = face_1d_mass_matrices(mesh) if m1d_per_face is None else m1d_per_face
@@ -88,7 +103,13 @@ def sipg_bilinear_form(mesh, u, v, *, bc_types, u_dir=None, tau_scale=200.0,
     ux, uy = _grad_xy(mesh, u)
     vx, vy = _grad_xy(mesh, v)
     ux, uy = kap * ux, kap * uy           # kappa-weighted flux (extension)
-    vol = np.sum(mesh.J * (ux * (M @ vx) + uy * (M @ vy)))
+    if cub is None:
+        vol = np.sum(mesh.J * (ux * (M @ vx) + uy * (M @ vy)))
+    else:
+        urq, usq, vrq, vsq = cub.Drq @ u, cub.Dsq @ u, cub.Drq @ v, cub.Dsq @ v
+        uxq, uyq = cub.rx * urq + cub.sx * usq, cub.ry * urq + cub.sy * usq
+        vxq, vyq = cub.rx * vrq + cub.sx * vsq, cub.ry * vrq + cub.sy * vsq
+        vol = np.sum(cub.w[:, None] * cub.J * (uxq * vxq + uyq * vyq))
     uf, vf = u.T.reshape(-1), v.T.reshape(-1)
     uxf, uyf = ux.T.reshape(-1), uy.T.reshape(-1)
     vxf, vyf = (kap * vx).T.reshape(-1), (kap * vy).T.reshape(-1)
@@ -112,17 +133,25 @@ def sipg_bilinear_form(mesh, u, v, *, bc_types, u_dir=None, tau_scale=200.0,
     ju, jv = face(uM - uP), face(vMv - vP_)
     sJ = face(mesh.sJ.reshape(-1, order="F"))[0]
     tau = _tau_per_face(mesh, tau_scale) * kap[None, :]
-    wsJ = w * sJ
-    cons = -np.einsum("fk,pfk,fpq,qfk->", wsJ, avgu, m1d, jv)
-    adj = -np.einsum("fk,pfk,fpq,qfk->", wsJ, ju, m1d, avgv)
-    pen = np.einsum("fk,pfk,fpq,qfk->", w * tau * sJ, ju, m1d, jv)
+    if cub is None:
+        wsJ = w * sJ
+        cons = -np.einsum("fk,pfk,fpq,qfk->", wsJ, avgu, m1d, jv)
+        adj = -np.einsum("fk,pfk,fpq,qfk->", wsJ, ju, m1d, avgv)
+        pen = np.einsum("fk,pfk,fpq,qfk->", w * tau * sJ, ju, m1d, jv)
+    else:
+        to_q = lambda a: np.einsum("fqp,pfk->qfk", cub.face_L1d, a)
+        wts = cub.face_w[:, None, None] * np.transpose(cub.face_sJ, (1, 0, 2)) * w[None]
+        cons = -np.sum(wts * to_q(avgu) * to_q(jv))
+        adj = -np.sum(wts * to_q(ju) * to_q(avgv))
+        pen = np.sum(wts * tau[None] * to_q(ju) * to_q(jv))
     return float(vol + cons + adj + pen)
 
 
 def sipg_apply_2d(mesh, u, *, bc_types, u_dir=None, tau_scale=200.0,
-                  M_ref=None, m1d_per_face=None, kappa=None) -> np.ndarray:
+                  M_ref=None, m1d_per_face=None, kappa=None, cub=None) -> np.ndarray:
     """d a(u, v) / d v, written as the reverse sweep of the form
     (sipg.py:304-337: ``jax.grad(form)(zeros)``).  Returns (Np, K)."""
+    _check_kappa_cub(kappa, cub)
     Np, K, Nfp = mesh.Np, mesh.K, mesh.Nfp
     M = local_mass(mesh) if M_ref is None else M_ref
     m1d = face_1d_mass_matrices(mesh) if m1d_per_face is None else m1d_per_face
@@ -149,11 +178,15 @@ def sipg_apply_2d(mesh, u, *, bc_types, u_dir=None, tau_scale=200.0,
     ju = face(uM - uP)
     sJ = face(mesh.sJ.reshape(-1, order="F"))[0]
     tau = _tau_per_face(mesh, tau_scale) * kap[None, :]
-    wsJ = w * sJ
-    # cotangents of jmp_v and avg_Dn_v  (einsum 'fk,pfk,fpq,qfk->')
-    g_jv = (-wsJ[None] * np.einsum("fpq,pfk->qfk", m1d, avgu)
-            + (w * tau * sJ)[None] * np.einsum("fpq,pfk->qfk", m1d, ju))
-    g_av = -wsJ[None] * np.einsum("fpq,pfk->qfk", m1d, ju)
+    # face mass of each element side: sJ m1d (nodal) or the face cubature's
+    # L1d^T diag(w sJ_q) L1d; cotangents of jmp_v and avg_Dn_v
+    if cub is None:
+        mf = sJ[:, None, None, :] * m1d[:, :, :, None]              # (3, Nfp, Nfp, K)
+    else:
+        mf = face_cub_mass(cub)
+    g_jv = (-w[None] * np.einsum("fpqk,pfk->qfk", mf, avgu)
+            + (w * tau)[None] * np.einsum("fpqk,pfk->qfk", mf, ju))
+    g_av = -w[None] * np.einsum("fpqk,pfk->qfk", mf, ju)
     g_jv, g_av = flat(g_jv), flat(g_av)
     # jmp_v = v_M - v_P_eff ; v_P_eff = where(dir, 0, where(neu, v_M, v_P_raw))
     g_vM = g_jv + np.where(isn, -g_jv, 0.0)
@@ -168,21 +201,35 @@ def sipg_apply_2d(mesh, u, *, bc_types, u_dir=None, tau_scale=200.0,
     g_vyf = sc(vM, ny * g_DvM) + sc(vP, ny * g_DvPr)
     unflat = lambda a: a.reshape(K, Np).T
     # vx, vy were kappa-weighted in the face terms only
-    g_vx = M.T @ (mesh.J * ux) + kap * unflat(g_vxf)
-    g_vy = M.T @ (mesh.J * uy) + kap * unflat(g_vyf)
+    g_vx = kap * unflat(g_vxf)
+    g_vy = kap * unflat(g_vyf)
+    out = unflat(g_vf)
+    if cub is None:
+        g_vx = g_vx + M.T @ (mesh.J * ux)
+        g_vy = g_vy + M.T @ (mesh.J * uy)
+    else:
+        # volume cubature: d/dv of sum w J (ux_q vx_q + uy_q vy_q)
+        urq, usq = cub.Drq @ u, cub.Dsq @ u
+        cx = cub.w[:, None] * cub.J * (cub.rx * urq + cub.sx * usq)
+        cy = cub.w[:, None] * cub.J * (cub.ry * urq + cub.sy * usq)
+        out = out + cub.Drq.T @ (cub.rx * cx + cub.ry * cy) + cub.Dsq.T @ (cub.sx * cx + cub.sy * cy)
     g_vr = mesh.rx * g_vx + mesh.ry * g_vy
     g_vs = mesh.sx * g_vx + mesh.sy * g_vy
-    return unflat(g_vf) + mesh.Dr.T @ g_vr + mesh.Ds.T @ g_vs
+    return out + mesh.Dr.T @ g_vr + mesh.Ds.T @ g_vs
 
 
-def mass_apply(mesh, u, M_ref=None) -> np.ndarray:
-    """J * (M_ref @ u) (sipg.py:565, 597, 853)."""
+def mass_apply(mesh, u, M_ref=None, cub=None) -> np.ndarray:
+    """J * (M_ref @ u) (sipg.py:565, 597, 853); cubature mass with ``cub``
+    (cubature.py:242-245)."""
+    if cub is not None:
+        return cub.Lq.T @ (cub.w[:, None] * cub.J * (cub.Lq @ u))
     M = local_mass(mesh) if M_ref is None else M_ref
     return mesh.J * (M @ u)
 
 
-def flat_apply(mesh, bc_types, tau_scale=200.0, alpha=0.0, kappa=None):
-    """Element-major flat operator alpha*M + A (multigrid.py:53-74)."""
+def flat_apply(mesh, bc_types, tau_scale=200.0, alpha=0.0, kappa=None, cub=None):
+    """Element-major flat operator alpha*M + A (multigrid.py:53-74;
+    sipg.py:589-601 with ``cub``)."""
     M = local_mass(mesh)
     m1d = face_1d_mass_matrices(mesh)
     Np, K = mesh.Np, mesh.K
@@ -190,17 +237,18 @@ def flat_apply(mesh, bc_types, tau_scale=200.0, alpha=0.0, kappa=None):
     def apply(uf):
         u = uf.reshape(K, Np).T
         out = sipg_apply_2d(mesh, u, bc_types=bc_types, tau_scale=tau_scale,
-                            M_ref=M, m1d_per_face=m1d, kappa=kappa)
+                            M_ref=M, m1d_per_face=m1d, kappa=kappa, cub=cub)
         if alpha != 0.0:
-            out = out + alpha * (mesh.J * (M @ u))
+            out = out + alpha * mass_apply(mesh, u, M, cub)
         return out.T.reshape(-1)
     return apply
 
 
-def sipg_matrix_2d(mesh, *, bc_types, tau_scale=200.0, alpha=0.0, kappa=None) -> np.ndarray:
+def sipg_matrix_2d(mesh, *, bc_types, tau_scale=200.0, alpha=0.0, kappa=None,
+                   cub=None) -> np.ndarray:
     """Dense (Np K)^2 matrix, column by column (sipg.py:344-364, 460-496)."""
     n = mesh.Np * mesh.K
-    ap = flat_apply(mesh, bc_types, tau_scale, alpha, kappa)
+    ap = flat_apply(mesh, bc_types, tau_scale, alpha, kappa, cub)
     A = np.empty((n, n))
     e = np.zeros(n)
     for j in range(n):
@@ -211,9 +259,9 @@ def sipg_matrix_2d(mesh, *, bc_types, tau_scale=200.0, alpha=0.0, kappa=None) ->
 
 
 def factor_sipg_2d(mesh, *, bc_types, tau_scale=200.0, regularize=False,
-                   eps=1.0e-8, alpha=0.0):
+                   eps=1.0e-8, alpha=0.0, cub=None):
     """sipg.py:367-391 (and 518-533 with alpha)."""
-    A = sipg_matrix_2d(mesh, bc_types=bc_types, tau_scale=tau_scale, alpha=alpha)
+    A = sipg_matrix_2d(mesh, bc_types=bc_types, tau_scale=tau_scale, alpha=alpha, cub=cub)
     if regularize:
         A = A + eps * np.max(np.abs(np.diag(A))) * np.eye(A.shape[0])
     lu, piv = sla.lu_factor(A)
@@ -243,9 +291,10 @@ def sipg_neumann_load_2d(mesh, q_neu, *, bc_types) -> np.ndarray:
     return out
 
 
-def _rhs(mesh, source, bc_types, u_dir, q_neu, tau_scale):
-    """Mf - L_D + L_N (sipg.py:849-864)."""
-    rhs = mass_apply(mesh, source)
+def _rhs(mesh, source, bc_types, u_dir, q_neu, tau_scale, cub=None):
+    """Mf - L_D + L_N (sipg.py:849-864); the mass is the cubature mass with
+    ``cub``, the loads are always the nodal ones (sipg.py:398-453)."""
+    rhs = mass_apply(mesh, source, cub=cub)
     if u_dir is not None:
         rhs = rhs - sipg_dirichlet_load_2d(mesh, u_dir, bc_types=bc_types, tau_scale=tau_scale)
     if q_neu is not None:
@@ -255,16 +304,16 @@ def _rhs(mesh, source, bc_types, u_dir, q_neu, tau_scale):
 
 def solve_sipg_poisson_2d(mesh, source, *, bc_types, u_dir=None, q_neu=None,
                           tau_scale=200.0, A_factored=None, iterative=None,
-                          x0=None, preconditioner=None, return_info=False):
-    """sipg.py:812-918 (affine)."""
+                          x0=None, preconditioner=None, return_info=False, cub=None):
+    """sipg.py:812-918."""
     Np, K = mesh.Np, mesh.K
-    b = _rhs(mesh, source, bc_types, u_dir, q_neu, tau_scale)
+    b = _rhs(mesh, source, bc_types, u_dir, q_neu, tau_scale, cub)
     if iterative is None:
         if A_factored is None:
-            A_factored = factor_sipg_2d(mesh, bc_types=bc_types, tau_scale=tau_scale)
+            A_factored = factor_sipg_2d(mesh, bc_types=bc_types, tau_scale=tau_scale, cub=cub)
         _, lu, piv = A_factored
         return sla.lu_solve((lu, piv), b).reshape(K, Np).T
-    raw = flat_apply(mesh, bc_types, tau_scale)
+    raw = flat_apply(mesh, bc_types, tau_scale, cub=cub)
     if not np.any(np.asarray(bc_types) == BC_DIRICHLET):
         probe = np.sin(np.arange(Np * K, dtype=np.float64))
         probe = probe - probe.mean()
@@ -285,18 +334,19 @@ def solve_sipg_poisson_2d(mesh, source, *, bc_types, u_dir=None, q_neu=None,
 def solve_sipg_helmholtz_2d(mesh, rhs_field, *, alpha, nu, bc_types, u_dir=None,
                             q_neu=None, tau_scale=200.0, A_factored=None,
                             iterative=None, x0=None, preconditioner=None,
-                            return_info=False):
+                            return_info=False, cub=None):
     """(alpha M + A) u = M f - L_D + L_N; ``nu`` unused (sipg.py:536-612)."""
     Np, K = mesh.Np, mesh.K
-    b = _rhs(mesh, rhs_field, bc_types, u_dir, q_neu, tau_scale)
+    b = _rhs(mesh, rhs_field, bc_types, u_dir, q_neu, tau_scale, cub)
     if iterative is None:
         if A_factored is None:
-            A_factored = factor_sipg_2d(mesh, bc_types=bc_types, tau_scale=tau_scale, alpha=alpha)
+            A_factored = factor_sipg_2d(mesh, bc_types=bc_types, tau_scale=tau_scale,
+                                        alpha=alpha, cub=cub)
         _, lu, piv = A_factored
         return sla.lu_solve((lu, piv), b).reshape(K, Np).T
     if iterative.method != "cg":
         raise ValueError(f"iterative method {iterative.method!r} not supported for SPD")
-    matvec = flat_apply(mesh, bc_types, tau_scale, alpha)
+    matvec = flat_apply(mesh, bc_types, tau_scale, alpha, cub=cub)
     xi = np.zeros(Np * K) if x0 is None else x0.T.reshape(-1)
     x, info = jax_cg(matvec, b, xi, tol=iterative.tol, atol=iterative.atol,
                      maxiter=iterative.maxiter, M=preconditioner)

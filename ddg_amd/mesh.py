@@ -114,17 +114,14 @@ def _connect(EToV: np.ndarray):
     return EToE, EToF
 
 
-def build_mesh_2d(VX, VY, EToV, N: int, node_tol: float = 1e-10) -> Mesh2D:
-    """StartUp2D (mesh.py:691-760), vectorised."""
+def geometric_factors(x, y, N: int):
+    """Per-node metrics ``rx, sx, ry, sy, J`` of the element maps given by
+    the nodal coordinates (mesh.py:634-644), outward unit normals and
+    surface Jacobians at the face nodes (mesh.py:647-684) and
+    ``Fscale = sJ / J`` there.  Works for curved (non-affine) elements."""
     ref = ref_element(N)
-    VX = np.asarray(VX, dtype=np.float64)
-    VY = np.asarray(VY, dtype=np.float64)
-    EToV = np.asarray(EToV, dtype=np.int64)
-    K, Np, Nfp = EToV.shape[0], ref.Np, ref.Nfp
-    r, s, Dr, Ds, Fm = ref.r, ref.s, ref.Dr, ref.Ds, ref.Fmask
-    va, vb, vc = EToV[:, 0], EToV[:, 1], EToV[:, 2]
-    x = 0.5 * (np.outer(-(r + s), VX[va]) + np.outer(1 + r, VX[vb]) + np.outer(1 + s, VX[vc]))
-    y = 0.5 * (np.outer(-(r + s), VY[va]) + np.outer(1 + r, VY[vb]) + np.outer(1 + s, VY[vc]))
+    Dr, Ds, Fm, Nfp = ref.Dr, ref.Ds, ref.Fmask, ref.Nfp
+    K = x.shape[1]
     xr, xs, yr, ys = Dr @ x, Ds @ x, Dr @ y, Ds @ y
     J = xr * ys - xs * yr
     rx, sx, ry, sy = ys / J, -yr / J, -xs / J, xr / J
@@ -138,7 +135,42 @@ def build_mesh_2d(VX, VY, EToV, N: int, node_tol: float = 1e-10) -> Mesh2D:
     nx[a2], ny[a2] = -fys[a2], fxs[a2]
     sJ = np.sqrt(nx * nx + ny * ny)
     nx, ny = nx / sJ, ny / sJ
-    Fscale = sJ / J[Ff]
+    return rx, sx, ry, sy, J, nx, ny, sJ, sJ / J[Ff]
+
+
+def rectangle_with_square_hole_mesh_2d(xmin, xmax, ymin, ymax, Kx: int, Ky: int,
+                                       hole_xmin, hole_xmax, hole_ymin, hole_ymax):
+    """The structured rectangle mesh of :func:`uniform_rectangle_mesh_2d`
+    without the cells whose centre lies in the hole box, unused vertices
+    dropped and renumbered in order (mesh.py:159-206).  The hole bounds
+    should fall on grid lines."""
+    VX, VY, EToV = uniform_rectangle_mesh_2d(xmin, xmax, ymin, ymax, Kx, Ky)
+    xs = np.linspace(xmin, xmax, Kx + 1)
+    ys = np.linspace(ymin, ymax, Ky + 1)
+    cx = 0.5 * (xs[:-1] + xs[1:])
+    cy = 0.5 * (ys[:-1] + ys[1:])
+    hole = ((cy[:, None] >= hole_ymin) & (cy[:, None] <= hole_ymax)
+            & (cx[None, :] >= hole_xmin) & (cx[None, :] <= hole_xmax)).reshape(-1)
+    keep = np.repeat(~hole, 2)                      # two triangles per cell
+    E = EToV[keep]
+    used = np.unique(E)
+    remap = np.full(VX.size, -1, dtype=np.int64)
+    remap[used] = np.arange(used.size)
+    return VX[used], VY[used], remap[E]
+
+
+def build_mesh_2d(VX, VY, EToV, N: int, node_tol: float = 1e-10) -> Mesh2D:
+    """StartUp2D (mesh.py:691-760), vectorised."""
+    ref = ref_element(N)
+    VX = np.asarray(VX, dtype=np.float64)
+    VY = np.asarray(VY, dtype=np.float64)
+    EToV = np.asarray(EToV, dtype=np.int64)
+    K, Np, Nfp = EToV.shape[0], ref.Np, ref.Nfp
+    r, s, Dr, Ds, Fm = ref.r, ref.s, ref.Dr, ref.Ds, ref.Fmask
+    va, vb, vc = EToV[:, 0], EToV[:, 1], EToV[:, 2]
+    x = 0.5 * (np.outer(-(r + s), VX[va]) + np.outer(1 + r, VX[vb]) + np.outer(1 + s, VX[vc]))
+    y = 0.5 * (np.outer(-(r + s), VY[va]) + np.outer(1 + r, VY[vb]) + np.outer(1 + s, VY[vc]))
+    rx, sx, ry, sy, J, nx, ny, sJ, Fscale = geometric_factors(x, y, N)
     EToE, EToF = _connect(EToV)
     # node maps: face f's nodes run v0->v1, v1->v2, v0->v2 for f = 0, 1, 2
     ids = np.arange(K * Np).reshape(K, Np)                 # ids[k, i] = k*Np+i

@@ -22,7 +22,7 @@ EXPORTS = (
     "ddgx_block_jacobi_blocks", "ddgx_pc_apply", "ddgx_pc_destroy", "ddgx_pcg_solve",
     "ddgx_dot", "ddgx_mg_create", "ddgx_ir_solve", "ddgx_sipg_param_grad",
     "ddgx_solver_create", "ddgx_solver_bytes", "ddgx_solver_pcg", "ddgx_solver_ir",
-    "ddgx_solver_destroy", "ddgx_pcg_solve_noalloc",
+    "ddgx_solver_destroy", "ddgx_pcg_solve_noalloc", "ddgx_op_create_curved",
 )
 
 
@@ -69,6 +69,8 @@ def lib():
         "ddgx_last_error": (C.c_char_p, []),
         "ddgx_table_size": (i32, [i32]),
         "ddgx_op_create": (i32, [i32, i64, i32, vp, vp, vp, vp, vp, C.POINTER(vp)]),
+        "ddgx_op_create_curved": (i32, [i32, i64, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp,
+                                         vp, C.POINTER(vp)]),
         "ddgx_op_destroy": (i32, [vp]),
         "ddgx_op_info": (i32, [vp, C.POINTER(i32), C.POINTER(i64), C.POINTER(i32)]),
         "ddgx_sipg_apply": (i32, [vp, vp, vp, vp, dbl, dbl, vp]),

@@ -83,6 +83,7 @@ static void free_op(Op& o) {
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   work_free(o.ws);
+  curved_free(o.cv);
   o = Op{};
 }
 
@@ -264,6 +265,52 @@ int ddgx_op_create(int N, int64_t K, int dtype, const double* tables, const doub
   return DDGX_OK;
 }
 
+int ddgx_op_create_curved(int N, int64_t K, int dtype, int use_cub, int ncub,
+                          const double* reftab, const double* geo, const double* fnorm,
+                          const double* fpar, const int32_t* codes, const double* fmass,
+                          const double* volw, const double* cubtab, ddgx_op* out) {
+  if (out == nullptr || reftab == nullptr || geo == nullptr || fnorm == nullptr ||
+      fpar == nullptr || codes == nullptr) {
+    set_error("ddgx_op_create_curved: null argument");
+    return DDGX_ERR_INVALID;
+  }
+  *out = nullptr;
+  if (N < 1 || N > 6) {
+    set_error("ddgx_op_create_curved: polynomial order " + std::to_string(N) +
+              " not built (1..6)");
+    return DDGX_ERR_UNSUPPORTED;
+  }
+  if (dtype != DDGX_F64 && dtype != DDGX_F32) {
+    set_error("ddgx_op_create_curved: dtype must be DDGX_F64 or DDGX_F32");
+    return DDGX_ERR_INVALID;
+  }
+  if (K < 1 || K >= (int64_t(1) << 28) || (use_cub && (ncub < 1 || ncub > 4096))) {
+    set_error("ddgx_op_create_curved: element or cubature point count out of range");
+    return DDGX_ERR_INVALID;
+  }
+  auto* h = new ddgx_op_s();
+  Op& o = h->op;
+  o.N = N;
+  o.K = K;
+  o.dtype = dtype;
+  o.Np = (N + 1) * (N + 2) / 2;
+  o.Nfp = N + 1;
+  int rc = DDGX_OK;
+  if (hipGetDevice(&o.device) != hipSuccess) rc = hip_fail(hipGetLastError(), "hipGetDevice");
+  if (rc == DDGX_OK)
+    rc = curved_create(o, use_cub, ncub, reftab, geo, fnorm, fpar, codes, fmass, volw, cubtab);
+  if (rc != DDGX_OK) {
+    free_op(o);
+    delete h;
+    return rc;
+  }
+  const int64_t gv = (K * o.Np + 255) / 256;
+  const int64_t cap = 4 * (int64_t)device_cu_count();
+  o.grid_vec = (int)(gv < cap ? gv : cap);
+  *out = h;
+  return DDGX_OK;
+}
+
 int ddgx_op_destroy(ddgx_op op) {
   if (op == nullptr) return DDGX_OK;
   free_op(op->op);
@@ -313,6 +360,10 @@ int ddgx_block_jacobi_create(ddgx_op op64, double alpha, int dtype, void* stream
   if (op64->op.dtype != DDGX_F64) {
     set_error("ddgx_block_jacobi_create: blocks are built from a float64 operator");
     return DDGX_ERR_INVALID;
+  }
+  if (op64->op.cv != nullptr) {
+    set_error("ddgx_block_jacobi_create: the closed-form blocks need affine elements");
+    return DDGX_ERR_UNSUPPORTED;
   }
   if (dtype != DDGX_F64 && dtype != DDGX_F32) {
     set_error("ddgx_block_jacobi_create: bad dtype");
@@ -407,6 +458,10 @@ int ddgx_mg_create(int n_levels, const ddgx_op* ops, const ddgx_op* ops64,
     }
     lv[l] = &ops[l]->op;
     lv64[l] = &ops64[l]->op;
+    if (lv[l]->cv != nullptr || lv64[l]->cv != nullptr) {
+      set_error("ddgx_mg_create: multigrid levels must be affine operators");
+      return DDGX_ERR_UNSUPPORTED;
+    }
   }
   MG* mg = nullptr;
   const int rc = mg_create(n_levels, lv.data(), lv64.data(), transfer_kind, transfer, *cfg,
@@ -498,6 +553,10 @@ int ddgx_sipg_param_grad(ddgx_op op, const double* u, const double* lam, const d
       g_kappa == nullptr || g_tau == nullptr) {
     set_error("ddgx_sipg_param_grad: null argument");
     return DDGX_ERR_INVALID;
+  }
+  if (op->op.cv != nullptr) {
+    set_error("ddgx_sipg_param_grad: needs an affine operator");
+    return DDGX_ERR_UNSUPPORTED;
   }
   return launch_param_grad(op->op, u, lam, u_dir, kappa, tau_scale, g_kappa, g_tau,
                            (hipStream_t)stream);

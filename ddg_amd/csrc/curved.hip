@@ -1,0 +1,563 @@
+// SIPG apply on curved (non-affine) elements, for gfx950:
+//   y = (A_sipg + alpha M + shift I) u       (or b - that: residual mode)
+// with the reference's two quadratures (src/ddg/dg2d/sipg.py:142-337):
+//   nodal    (cub=None): vol = sum J (ux (M vx) + uy (M vy)) with per-node J
+//                        and metrics, faces  sum_side w sJ_0 X^T m1d Y;
+//   cubature (cub):      vol = sum_q w_q J_q (ux_q vx_q + uy_q vy_q), faces
+//                        sum_side w X^T (L1d^T diag(w_q sJ_q) L1d) Y
+//                        (sipg.py:190-201, 281-299).
+// In both, the face traces are nodal: u and grad u at the face nodes with
+// the per-node metrics, normal derivatives with the side's own nodal
+// normal; the penalty is tau_scale (N+1)^2 Fscale at face node 0.
+//
+// The apply is the reverse-mode gradient d a(u, v) / dv, in three launches:
+//   1. curved_trace_kernel: per face node, u, du/dx, du/dy       -> trace
+//   2. curved_side_kernel:  per element side, jump and average with the
+//      side's normal and face mass, then the cotangents of v, dv/dx, dv/dy
+//      at its own face nodes (cown) and at the matched neighbour nodes (cnbr)
+//   3. curved_elem_kernel:  per element, gathers its cown and the cnbr of
+//      the sides facing it, pulls dv/dx, dv/dy back through the per-node
+//      metrics and Dr^T, Ds^T, adds the nodal or cubature volume term and
+//      the mass.
+// Every element owns its output rows, so the result is deterministic.
+// Layouts: element-major, element k's data contiguous (see struct Curved).
+
+#include <string>
+#include <vector>
+
+#include "ddgx_internal.h"
+
+namespace ddgx {
+
+template <int N>
+struct CShape {
+  static constexpr int Np = (N + 1) * (N + 2) / 2;
+  static constexpr int Nfp = N + 1;
+};
+
+template <typename T, int N>
+__global__ void __launch_bounds__(256)
+curved_trace_kernel(const T* __restrict__ tab, const T* __restrict__ geo,
+                    const T* __restrict__ u, T* __restrict__ tr, int64_t K,
+                    const int* __restrict__ skip) {
+  if (skip != nullptr && *skip) return;
+  constexpr int Np = CShape<N>::Np, Nfp = CShape<N>::Nfp;
+  const int64_t nf = K * 3 * Nfp;
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < nf; t += stride) {
+    const int64_t k = t / (3 * Nfp);
+    const int fp = (int)(t - k * 3 * Nfp);
+    const int i = fmask<N>(fp / Nfp, fp % Nfp);
+    T ui = 0, ur = 0, us = 0;
+    if (u != nullptr) {
+      const T* uk = u + k * Np;
+      const T* dr = tab + i * Np;
+      const T* ds = tab + Np * Np + i * Np;
+#pragma unroll
+      for (int j = 0; j < Np; ++j) {
+        const T uj = uk[j];
+        ur += dr[j] * uj;
+        us += ds[j] * uj;
+      }
+      ui = uk[i];
+    }
+    const T* g = geo + k * 5 * Np;
+    tr[t] = ui;
+    tr[nf + t] = g[i] * ur + g[Np + i] * us;
+    tr[2 * nf + t] = g[2 * Np + i] * ur + g[3 * Np + i] * us;
+  }
+}
+
+template <typename T, int N>
+__global__ void __launch_bounds__(256)
+curved_side_kernel(const T* __restrict__ tab, const T* __restrict__ fnorm,
+                   const T* __restrict__ fpar, const T* __restrict__ fmass,
+                   const int* __restrict__ codes, const T* __restrict__ tr,
+                   const T* __restrict__ udir, T* __restrict__ cown, T* __restrict__ cnbr,
+                   int64_t K, const int* __restrict__ skip) {
+  if (skip != nullptr && *skip) return;
+  constexpr int Np = CShape<N>::Np, Nfp = CShape<N>::Nfp;
+  const int64_t nf = K * 3 * Nfp;
+  const int64_t nside = K * 3;
+  const T* m1d = tab + 3 * Np * Np;
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < nside; t += stride) {
+    const int64_t k = t / 3;
+    const int f = (int)(t - k * 3);
+    const int64_t base = t * Nfp;
+    const int c = codes[t];
+    const T w = fpar[t * 3], wtau = fpar[t * 3 + 1], sj0 = fpar[t * 3 + 2];
+    if (c == CODE_NONE || w == T(0)) {
+#pragma unroll
+      for (int p = 0; p < Nfp; ++p)
+        for (int q = 0; q < 3; ++q) {
+          cown[q * nf + base + p] = 0;
+          cnbr[q * nf + base + p] = 0;
+        }
+      continue;
+    }
+    const bool inner = c >= 0;
+    int64_t nb = 0;
+    int rev = 0;
+    if (inner) {
+      nb = ((int64_t)nbr_elem(c) * 3 + nbr_face(c)) * Nfp;
+      rev = nbr_rev(c);
+    }
+    T ju[Nfp], av[Nfp];
+#pragma unroll
+    for (int p = 0; p < Nfp; ++p) {
+      const T nx = fnorm[(base + p) * 2], ny = fnorm[(base + p) * 2 + 1];
+      const T uM = tr[base + p];
+      const T dM = nx * tr[nf + base + p] + ny * tr[2 * nf + base + p];
+      T uP, dP;
+      if (inner) {
+        const int64_t o = nb + (rev ? Nfp - 1 - p : p);
+        uP = tr[o];
+        dP = nx * tr[nf + o] + ny * tr[2 * nf + o];
+      } else {  // Dirichlet: ghost u_dir (or 0), ghost gradient = own
+        uP = udir != nullptr ? udir[k * Np + fmask<N>(f, p)] : T(0);
+        dP = dM;
+      }
+      ju[p] = uM - uP;
+      av[p] = T(0.5) * (dM + dP);
+    }
+#pragma unroll
+    for (int p = 0; p < Nfp; ++p) {
+      T mj = 0, ma = 0;
+#pragma unroll
+      for (int q = 0; q < Nfp; ++q) {
+        const T m = fmass != nullptr ? fmass[t * Nfp * Nfp + q * Nfp + p]
+                                     : sj0 * m1d[f * Nfp * Nfp + q * Nfp + p];
+        mj += m * ju[q];
+        ma += m * av[q];
+      }
+      const T gjv = wtau * mj - w * ma;  // d/d[v]
+      const T gav = -w * mj;             // d/d{dv/dn}
+      const T gvM = gjv, gvP = inner ? -gjv : T(0);
+      const T gdM = inner ? T(0.5) * gav : gav, gdP = inner ? T(0.5) * gav : T(0);
+      const T nx = fnorm[(base + p) * 2], ny = fnorm[(base + p) * 2 + 1];
+      cown[base + p] = gvM;
+      cown[nf + base + p] = nx * gdM;
+      cown[2 * nf + base + p] = ny * gdM;
+      cnbr[base + p] = gvP;
+      cnbr[nf + base + p] = nx * gdP;
+      cnbr[2 * nf + base + p] = ny * gdP;
+    }
+  }
+}
+
+__device__ inline double block_sum_256c(double v, double* s_red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const double t = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
+  __syncthreads();
+  return t;
+}
+
+// LDS per element: 8 vectors of Np (u, gv, gx, gy, a, b, vr, vs), then
+// 3 of ncub (cubature: wr, ws, mass).
+template <typename T, int N, bool CUB>
+__global__ void __launch_bounds__(256)
+curved_elem_kernel(const T* __restrict__ tab, const T* __restrict__ geo,
+                   const T* __restrict__ cubtab, const T* __restrict__ volw, int ncub,
+                   const int* __restrict__ codes, const T* __restrict__ cown,
+                   const T* __restrict__ cnbr, const T* __restrict__ u, T* __restrict__ y,
+                   T alpha, T shift, int epb, int64_t K, double* __restrict__ partial,
+                   const int* __restrict__ skip, const T* __restrict__ bvec) {
+  if (skip != nullptr && *skip) return;
+  constexpr int Np = CShape<N>::Np, Nfp = CShape<N>::Nfp;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ double s_red[4];
+  T* sh = reinterpret_cast<T*>(smem_raw);
+  const int L = epb * Np;
+  T *s_u = sh, *s_gv = sh + L, *s_gx = sh + 2 * L, *s_gy = sh + 3 * L;
+  T *s_a = sh + 4 * L, *s_b = sh + 5 * L, *s_vr = sh + 6 * L, *s_vs = sh + 7 * L;
+  T *s_wr = sh + 8 * L, *s_ws = s_wr + epb * ncub, *s_m = s_ws + epb * ncub;
+  const T* Dr = tab;
+  const T* Ds = tab + Np * Np;
+  const T* M = tab + 2 * Np * Np;
+  const T* Drq = cubtab;
+  const T* Dsq = cubtab + (size_t)ncub * Np;
+  const T* Lq = cubtab + 2 * (size_t)ncub * Np;
+  const int64_t nf = K * 3 * Nfp;
+  const int tid = threadIdx.x;
+  const int64_t ngroups = (K + epb - 1) / epb;
+  double dacc = 0.0;
+  for (int64_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
+    const int64_t k0 = g * epb;
+    const int ne = (int)((K - k0) < epb ? (K - k0) : epb);
+    __syncthreads();
+    for (int idx = tid; idx < ne * Np; idx += 256) {
+      s_u[idx] = u != nullptr ? u[k0 * Np + idx] : T(0);
+      s_gv[idx] = 0;
+      s_gx[idx] = 0;
+      s_gy[idx] = 0;
+    }
+    __syncthreads();
+    // face cotangents, one face at a time (a vertex node lies on two faces)
+    for (int f = 0; f < 3; ++f) {
+      for (int idx = tid; idx < ne * Nfp; idx += 256) {
+        const int e = idx / Nfp, p = idx - (idx / Nfp) * Nfp;
+        const int64_t side = (k0 + e) * 3 + f;
+        const int64_t t = side * Nfp + p;
+        T v0 = cown[t], v1 = cown[nf + t], v2 = cown[2 * nf + t];
+        const int c = codes[side];
+        if (c >= 0) {
+          const int64_t o = ((int64_t)nbr_elem(c) * 3 + nbr_face(c)) * Nfp +
+                            (nbr_rev(c) ? Nfp - 1 - p : p);
+          v0 += cnbr[o];
+          v1 += cnbr[nf + o];
+          v2 += cnbr[2 * nf + o];
+        }
+        const int i = e * Np + fmask<N>(f, p);
+        s_gv[i] += v0;
+        s_gx[i] += v1;
+        s_gy[i] += v2;
+      }
+      __syncthreads();
+    }
+    if constexpr (!CUB) {
+      for (int idx = tid; idx < ne * Np; idx += 256) {
+        const int e = idx / Np, i = idx - e * Np;
+        const T* ue = s_u + e * Np;
+        T ur = 0, us = 0;
+#pragma unroll
+        for (int j = 0; j < Np; ++j) {
+          ur += Dr[i * Np + j] * ue[j];
+          us += Ds[i * Np + j] * ue[j];
+        }
+        const T* gk = geo + (k0 + e) * 5 * Np;
+        const T J = gk[4 * Np + i];
+        s_a[idx] = J * (gk[i] * ur + gk[Np + i] * us);
+        s_b[idx] = J * (gk[2 * Np + i] * ur + gk[3 * Np + i] * us);
+      }
+      __syncthreads();
+      for (int idx = tid; idx < ne * Np; idx += 256) {
+        const int e = idx / Np, i = idx - e * Np;
+        const T* ae = s_a + e * Np;
+        const T* be = s_b + e * Np;
+        T wx = s_gx[idx], wy = s_gy[idx];
+#pragma unroll
+        for (int j = 0; j < Np; ++j) {
+          wx += M[i * Np + j] * ae[j];
+          wy += M[i * Np + j] * be[j];
+        }
+        const T* gk = geo + (k0 + e) * 5 * Np;
+        s_vr[idx] = gk[i] * wx + gk[2 * Np + i] * wy;
+        s_vs[idx] = gk[Np + i] * wx + gk[3 * Np + i] * wy;
+      }
+    } else {
+      for (int idx = tid; idx < ne * ncub; idx += 256) {
+        const int e = idx / ncub, q = idx - e * ncub;
+        const T* ue = s_u + e * Np;
+        T ur = 0, us = 0, uq = 0;
+        for (int j = 0; j < Np; ++j) {
+          const T uj = ue[j];
+          ur += Drq[q * Np + j] * uj;
+          us += Dsq[q * Np + j] * uj;
+          uq += Lq[q * Np + j] * uj;
+        }
+        const T* vw = volw + (k0 + e) * 4 * ncub;
+        s_wr[idx] = vw[q] * ur + vw[ncub + q] * us;
+        s_ws[idx] = vw[ncub + q] * ur + vw[2 * ncub + q] * us;
+        s_m[idx] = vw[3 * ncub + q] * uq;
+      }
+      for (int idx = tid; idx < ne * Np; idx += 256) {
+        const int e = idx / Np, i = idx - e * Np;
+        const T* gk = geo + (k0 + e) * 5 * Np;
+        const T wx = s_gx[idx], wy = s_gy[idx];
+        s_vr[idx] = gk[i] * wx + gk[2 * Np + i] * wy;
+        s_vs[idx] = gk[Np + i] * wx + gk[3 * Np + i] * wy;
+      }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < ne * Np; idx += 256) {
+      const int e = idx / Np, i = idx - e * Np;
+      const T* vre = s_vr + e * Np;
+      const T* vse = s_vs + e * Np;
+      T acc = s_gv[idx];
+#pragma unroll
+      for (int j = 0; j < Np; ++j) acc += Dr[j * Np + i] * vre[j] + Ds[j * Np + i] * vse[j];
+      if constexpr (CUB) {
+        const T* wre = s_wr + e * ncub;
+        const T* wse = s_ws + e * ncub;
+        const T* me = s_m + e * ncub;
+        T mass = 0;
+        for (int q = 0; q < ncub; ++q) {
+          acc += Drq[q * Np + i] * wre[q] + Dsq[q * Np + i] * wse[q];
+          mass += Lq[q * Np + i] * me[q];
+        }
+        acc += alpha * mass;
+      } else {
+        if (alpha != T(0)) {
+          const T* ue = s_u + e * Np;
+          T mu = 0;
+#pragma unroll
+          for (int j = 0; j < Np; ++j) mu += M[i * Np + j] * ue[j];
+          acc += alpha * geo[(k0 + e) * 5 * Np + 4 * Np + i] * mu;
+        }
+      }
+      acc += shift * s_u[idx];
+      const int64_t gi = k0 * Np + idx;
+      y[gi] = bvec != nullptr ? bvec[gi] - acc : acc;
+      if (partial != nullptr) dacc += (double)s_u[idx] * (double)acc;
+    }
+  }
+  if (partial != nullptr) {
+    dacc = block_sum_256c(dacc, s_red);
+    if (tid == 0) partial[blockIdx.x] = dacc;
+  }
+}
+
+// rhs = mass(f) + L_N(q): the nodal (J M f) or cubature mass and the Neumann
+// surface load sJ_0 m1d q on Neumann sides (sipg.py:420-453, 849-864;
+// cubature.py:242-245).  One thread per node of an element group in LDS.
+template <typename T, int N, bool CUB>
+__global__ void __launch_bounds__(256)
+curved_mass_kernel(const T* __restrict__ tab, const T* __restrict__ geo,
+                   const T* __restrict__ cubtab, const T* __restrict__ volw, int ncub,
+                   const int* __restrict__ codes, const T* __restrict__ fpar,
+                   const T* __restrict__ f, const T* __restrict__ q, T* __restrict__ out,
+                   int epb, int64_t K) {
+  constexpr int Np = CShape<N>::Np, Nfp = CShape<N>::Nfp;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* s_f = reinterpret_cast<T*>(smem_raw);
+  T* s_m = s_f + epb * Np;
+  const T* M = tab + 2 * Np * Np;
+  const T* m1d = tab + 3 * Np * Np;
+  const T* Lq = cubtab + 2 * (size_t)ncub * Np;
+  const int tid = threadIdx.x;
+  const int64_t ngroups = (K + epb - 1) / epb;
+  for (int64_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
+    const int64_t k0 = g * epb;
+    const int ne = (int)((K - k0) < epb ? (K - k0) : epb);
+    __syncthreads();
+    for (int idx = tid; idx < ne * Np; idx += 256) s_f[idx] = f != nullptr ? f[k0 * Np + idx] : T(0);
+    __syncthreads();
+    if constexpr (CUB) {
+      for (int idx = tid; idx < ne * ncub; idx += 256) {
+        const int e = idx / ncub, qq = idx - e * ncub;
+        T v = 0;
+        for (int j = 0; j < Np; ++j) v += Lq[qq * Np + j] * s_f[e * Np + j];
+        s_m[idx] = volw[(k0 + e) * 4 * ncub + 3 * ncub + qq] * v;
+      }
+      __syncthreads();
+    }
+    for (int idx = tid; idx < ne * Np; idx += 256) {
+      const int e = idx / Np, i = idx - e * Np;
+      const int64_t k = k0 + e;
+      T acc = 0;
+      if constexpr (CUB) {
+        for (int qq = 0; qq < ncub; ++qq) acc += Lq[qq * Np + i] * s_m[e * ncub + qq];
+      } else {
+        T mf = 0;
+#pragma unroll
+        for (int j = 0; j < Np; ++j) mf += M[i * Np + j] * s_f[e * Np + j];
+        acc = geo[k * 5 * Np + 4 * Np + i] * mf;
+      }
+      if (q != nullptr) {
+#pragma unroll
+        for (int fc = 0; fc < 3; ++fc) {
+          const int p = facepos<N>(fc, i);
+          if (p < 0 || codes[k * 3 + fc] != CODE_NONE) continue;
+          const T* qf = q + (k * 3 + fc) * Nfp;
+          T v = 0;
+#pragma unroll
+          for (int r = 0; r < Nfp; ++r) v += m1d[fc * Nfp * Nfp + p * Nfp + r] * qf[r];
+          acc += fpar[(k * 3 + fc) * 3 + 2] * v;
+        }
+      }
+      out[k0 * Np + idx] = acc;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+template <typename T>
+static int upload(const double* h, size_t n, void** d) {
+  std::vector<T> v(h, h + n);
+  DDGX_HIP(hipMalloc(d, n * sizeof(T)));
+  DDGX_HIP(hipMemcpy(*d, v.data(), n * sizeof(T), hipMemcpyHostToDevice));
+  return DDGX_OK;
+}
+
+void curved_free(Curved* cv) {
+  if (cv == nullptr) return;
+  void* ptrs[] = {cv->geo, cv->fnorm, cv->fpar, cv->fmass, cv->volw, cv->cubtab,
+                  cv->reftab, cv->codes, cv->trace, cv->cown, cv->cnbr};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete cv;
+}
+
+static size_t elem_lds(int Np, int ncub, int epb, size_t es) {
+  return (size_t)epb * (8 * (size_t)Np + 3 * (size_t)ncub) * es;
+}
+
+int curved_create(Op& op, int use_cub, int ncub, const double* reftab, const double* geo,
+                  const double* fnorm, const double* fpar, const int32_t* codes,
+                  const double* fmass, const double* volw, const double* cubtab) {
+  const int Np = op.Np, Nfp = op.Nfp;
+  const int64_t K = op.K;
+  if (use_cub && (ncub <= 0 || fmass == nullptr || volw == nullptr || cubtab == nullptr)) {
+    set_error("cubature operator needs ncub > 0, face masses, volume weights and tables");
+    return DDGX_ERR_INVALID;
+  }
+  for (int64_t t = 0; t < 3 * K; ++t) {
+    const int c = codes[t];
+    if (c >= 0 && (nbr_elem(c) >= K || nbr_face(c) > 2)) {
+      set_error("face code names a neighbour outside the mesh");
+      return DDGX_ERR_INVALID;
+    }
+    if (c < CODE_NONE) {
+      set_error("bad face code");
+      return DDGX_ERR_INVALID;
+    }
+  }
+  Curved* cv = new Curved();
+  cv->use_cub = use_cub ? 1 : 0;
+  cv->ncub = use_cub ? ncub : 0;
+  const bool f64 = op.dtype == DDGX_F64;
+  const size_t es = f64 ? 8 : 4;
+  auto up = [&](const double* h, size_t n, void** d) {
+    return f64 ? upload<double>(h, n, d) : upload<float>(h, n, d);
+  };
+  int rc = DDGX_OK;
+  const size_t nfn = (size_t)K * 3 * Nfp;
+  do {
+    if ((rc = up(reftab, 3 * (size_t)Np * Np + 3 * (size_t)Nfp * Nfp, &cv->reftab))) break;
+    if ((rc = up(geo, (size_t)K * 5 * Np, &cv->geo))) break;
+    if ((rc = up(fnorm, nfn * 2, &cv->fnorm))) break;
+    if ((rc = up(fpar, (size_t)K * 9, &cv->fpar))) break;
+    if (use_cub) {
+      if ((rc = up(fmass, nfn * Nfp, &cv->fmass))) break;
+      if ((rc = up(volw, (size_t)K * 4 * ncub, &cv->volw))) break;
+      if ((rc = up(cubtab, 3 * (size_t)ncub * Np, &cv->cubtab))) break;
+    }
+    hipError_t e = hipMalloc((void**)&cv->codes, 3 * (size_t)K * sizeof(int));
+    if (e == hipSuccess)
+      e = hipMemcpy(cv->codes, codes, 3 * (size_t)K * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&cv->trace, 3 * nfn * es);
+    if (e == hipSuccess) e = hipMalloc(&cv->cown, 3 * nfn * es);
+    if (e == hipSuccess) e = hipMalloc(&cv->cnbr, 3 * nfn * es);
+    if (e != hipSuccess) rc = hip_fail(e, "curved operator buffers");
+  } while (0);
+  if (rc != DDGX_OK) {
+    curved_free(cv);
+    return rc;
+  }
+  const int width = cv->ncub > Np ? cv->ncub : Np;
+  int epb = 512 / width;
+  epb = epb < 1 ? 1 : (epb > 64 ? 64 : epb);
+  cv->epb = epb;
+  const int64_t cap = 4 * (int64_t)device_cu_count();
+  auto grid = [&](int64_t items) {
+    const int64_t g = (items + 255) / 256;
+    return (int)(g < 1 ? 1 : (g < 8 * cap ? g : 8 * cap));
+  };
+  cv->grid_face = grid((int64_t)nfn);
+  cv->grid_side = grid(3 * K);
+  const int64_t ngroups = (K + epb - 1) / epb;
+  cv->grid_elem = (int)(ngroups < 2 * cap ? ngroups : 2 * cap);
+  op.cv = cv;
+  op.grid_apply = cv->grid_elem;
+  return DDGX_OK;
+}
+
+template <typename T, int N>
+static int curved_t(const Op& op, const void* u, void* y, const void* udir, double alpha,
+                    double shift, double* partial, const int* skip, hipStream_t s,
+                    const void* bvec, bool face_nodal) {
+  const Curved& c = *op.cv;
+  const int64_t K = op.K;
+  hipLaunchKernelGGL((curved_trace_kernel<T, N>), dim3(c.grid_face), dim3(256), 0, s,
+                     (const T*)c.reftab, (const T*)c.geo, (const T*)u, (T*)c.trace, K, skip);
+  DDGX_HIP(hipGetLastError());
+  const T* fm = (c.use_cub && !face_nodal) ? (const T*)c.fmass : nullptr;
+  hipLaunchKernelGGL((curved_side_kernel<T, N>), dim3(c.grid_side), dim3(256), 0, s,
+                     (const T*)c.reftab, (const T*)c.fnorm, (const T*)c.fpar, fm, c.codes,
+                     (const T*)c.trace, (const T*)udir, (T*)c.cown, (T*)c.cnbr, K, skip);
+  DDGX_HIP(hipGetLastError());
+  const bool cub = c.use_cub && !face_nodal;
+  const size_t lds = elem_lds(op.Np, cub ? c.ncub : 0, c.epb, sizeof(T));
+  if (cub)
+    hipLaunchKernelGGL((curved_elem_kernel<T, N, true>), dim3(c.grid_elem), dim3(256), lds, s,
+                       (const T*)c.reftab, (const T*)c.geo, (const T*)c.cubtab, (const T*)c.volw,
+                       c.ncub, c.codes, (const T*)c.cown, (const T*)c.cnbr, (const T*)u, (T*)y,
+                       (T)alpha, (T)shift, c.epb, K, partial, skip, (const T*)bvec);
+  else
+    hipLaunchKernelGGL((curved_elem_kernel<T, N, false>), dim3(c.grid_elem), dim3(256), lds, s,
+                       (const T*)c.reftab, (const T*)c.geo, (const T*)nullptr, (const T*)nullptr,
+                       0, c.codes, (const T*)c.cown, (const T*)c.cnbr, (const T*)u, (T*)y,
+                       (T)alpha, (T)shift, c.epb, K, partial, skip, (const T*)bvec);
+  DDGX_HIP(hipGetLastError());
+  return DDGX_OK;
+}
+
+#define CURVED_DISPATCH(FN, ...)                                       \
+  switch (op.N) {                                                      \
+    case 1: return op.dtype == DDGX_F64 ? FN<double, 1>(__VA_ARGS__) : FN<float, 1>(__VA_ARGS__); \
+    case 2: return op.dtype == DDGX_F64 ? FN<double, 2>(__VA_ARGS__) : FN<float, 2>(__VA_ARGS__); \
+    case 3: return op.dtype == DDGX_F64 ? FN<double, 3>(__VA_ARGS__) : FN<float, 3>(__VA_ARGS__); \
+    case 4: return op.dtype == DDGX_F64 ? FN<double, 4>(__VA_ARGS__) : FN<float, 4>(__VA_ARGS__); \
+    case 5: return op.dtype == DDGX_F64 ? FN<double, 5>(__VA_ARGS__) : FN<float, 5>(__VA_ARGS__); \
+    case 6: return op.dtype == DDGX_F64 ? FN<double, 6>(__VA_ARGS__) : FN<float, 6>(__VA_ARGS__); \
+    default: break;                                                    \
+  }
+
+int launch_curved(const Op& op, const void* u, void* y, const void* udir, double alpha,
+                  double shift, int mode, double* partial, const int* skip, hipStream_t s,
+                  const void* bvec, bool face_nodal) {
+  if (mode & (M_SMOOTH | M_PROLONG)) {
+    set_error("the multigrid smoother does not run on curved operators");
+    return DDGX_ERR_UNSUPPORTED;
+  }
+  if ((mode & M_RESID) && bvec == nullptr) {
+    set_error("residual mode needs the right-hand side");
+    return DDGX_ERR_INVALID;
+  }
+  const void* b = (mode & M_RESID) ? bvec : nullptr;
+  double* part = (mode & M_DOT) ? partial : nullptr;
+  CURVED_DISPATCH(curved_t, op, u, y, udir, alpha, shift, part, skip, s, b, face_nodal)
+  set_error("polynomial order " + std::to_string(op.N) + " not built (1..6)");
+  return DDGX_ERR_UNSUPPORTED;
+}
+
+template <typename T, int N>
+static int curved_rhs_t(const Op& op, const void* f, const void* udir, const void* q, void* rhs,
+                        hipStream_t s) {
+  const Curved& c = *op.cv;
+  const bool cub = c.use_cub != 0;
+  const size_t lds = (size_t)c.epb * (op.Np + (cub ? c.ncub : 0)) * sizeof(T);
+  if (cub)
+    hipLaunchKernelGGL((curved_mass_kernel<T, N, true>), dim3(c.grid_elem), dim3(256), lds, s,
+                       (const T*)c.reftab, (const T*)c.geo, (const T*)c.cubtab, (const T*)c.volw,
+                       c.ncub, c.codes, (const T*)c.fpar, (const T*)f, (const T*)q, (T*)rhs,
+                       c.epb, op.K);
+  else
+    hipLaunchKernelGGL((curved_mass_kernel<T, N, false>), dim3(c.grid_elem), dim3(256), lds, s,
+                       (const T*)c.reftab, (const T*)c.geo, (const T*)nullptr, (const T*)nullptr,
+                       0, c.codes, (const T*)c.fpar, (const T*)f, (const T*)q, (T*)rhs, c.epb,
+                       op.K);
+  DDGX_HIP(hipGetLastError());
+  // - L_D: the nodal-quadrature apply of the zero field with the Dirichlet
+  // ghost values (sipg_dirichlet_load_2d has no cubature, sipg.py:398-417),
+  // subtracted in place through the residual mode
+  if (udir != nullptr)
+    return curved_t<T, N>(op, nullptr, rhs, udir, 0.0, 0.0, nullptr, nullptr, s, rhs, true);
+  return DDGX_OK;
+}
+
+int launch_curved_rhs(const Op& op, const void* f, const void* udir, const void* q, void* rhs,
+                      hipStream_t s) {
+  CURVED_DISPATCH(curved_rhs_t, op, f, udir, q, rhs, s)
+  set_error("polynomial order " + std::to_string(op.N) + " not built (1..6)");
+  return DDGX_ERR_UNSUPPORTED;
+}
+
+#undef CURVED_DISPATCH
+
+}  // namespace ddgx

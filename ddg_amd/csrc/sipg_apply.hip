@@ -91,6 +91,8 @@ static int launch_mode(const Op& op, const void* u, void* y, const void* udir, d
 int launch_apply(const Op& op, const void* u, void* y, const void* udir, double alpha,
                  double shift, int mode, double* partial, const int* skip, hipStream_t s,
                  const void* bvec) {
+  if (op.cv != nullptr)
+    return launch_curved(op, u, y, udir, alpha, shift, mode, partial, skip, s, bvec, false);
   if ((mode & M_RESID) && bvec == nullptr) {
     set_error("residual mode needs the right-hand side");
     return DDGX_ERR_INVALID;
@@ -315,6 +317,7 @@ static int launch_rhs_t(const Op& op, const void* f, const void* udir, const voi
 
 int launch_rhs(const Op& op, const void* f, const void* udir, const void* q, void* rhs,
                hipStream_t s) {
+  if (op.cv != nullptr) return launch_curved_rhs(op, f, udir, q, rhs, s);
 #define DISPATCH_N(T)                                                 \
   switch (op.N) {                                                     \
     case 1: return launch_rhs_t<T, 1>(op, f, udir, q, rhs, s);        \

@@ -98,6 +98,89 @@ def pack(mesh, bc_types, tau_scale=200.0, kappa=None):
     return rec, nbr, fsj
 
 
+def is_affine(mesh, rtol=1e-9):
+    """Whether every element's metrics and Jacobian are constant over its
+    nodes (straight-sided elements), to ``rtol`` of their magnitude."""
+    for name in ("rx", "sx", "ry", "sy", "J"):
+        a = np.asarray(getattr(mesh, name), dtype=np.float64)
+        if np.abs(a - a[0]).max() > rtol * max(np.abs(a).max(), 1e-300):
+            return False
+    return True
+
+
+def is_conforming(mesh, rtol=1e-9):
+    """Whether the node pairs vmapM/vmapP of every face sit at the same place
+    (a curved-boundary projection moves nodes of one side only)."""
+    x = np.asarray(mesh.x, dtype=np.float64).T.reshape(-1)
+    y = np.asarray(mesh.y, dtype=np.float64).T.reshape(-1)
+    vM, vP = np.asarray(mesh.vmapM), np.asarray(mesh.vmapP)
+    scale = max(np.ptp(x), np.ptp(y), 1e-300)
+    return bool(np.hypot(x[vM] - x[vP], y[vM] - y[vP]).max() <= rtol * scale)
+
+
+def _side_codes(mesh, bc_types):
+    """(K, 3) int32 face codes of every element side: ``(k'<<3)|(f'<<1)|rev``
+    for sides coupled to a neighbour (a boundary face tagged interior couples
+    to itself), -1 Dirichlet, -2 Neumann; and the bc array."""
+    N, K, Nfp = int(mesh.N), int(mesh.K), int(mesh.Nfp)
+    bc = np.asarray(bc_types).astype(np.int64)
+    if bc.shape != (K, 3):
+        raise ValueError(f"bc_types must have shape (K, 3) = {(K, 3)}, got {bc.shape}")
+    if np.any((bc < 0) | (bc > 2)):
+        raise ValueError("bc_types entries must be BC_INTERIOR, BC_DIRICHLET or BC_NEUMANN")
+    kn = np.asarray(mesh.EToE, dtype=np.int64)
+    fn = np.asarray(mesh.EToF, dtype=np.int64)
+    inner = bc == BC_INTERIOR
+    selfn = kn == np.arange(K)[:, None]
+    if np.any(inner & ~selfn & (bc[kn, fn] != BC_INTERIOR)):
+        raise ValueError("interior faces must be tagged BC_INTERIOR on both sides")
+    vM = np.asarray(mesh.vmapM).reshape((Nfp, 3, K), order="F")
+    vP = np.asarray(mesh.vmapP).reshape((Nfp, 3, K), order="F")
+    nbM = vM[:, fn, kn]                                           # (Nfp, K, 3)
+    Pk = np.transpose(vP, (0, 2, 1))
+    fwd = np.all(Pk == nbM, axis=0)
+    rev = np.all(Pk == nbM[::-1], axis=0) & ~fwd
+    if np.any(inner & ~(fwd | rev)):
+        raise ValueError("vmapP does not match the neighbour face in either node order")
+    code = np.where(bc == BC_DIRICHLET, -1, -2)
+    code = np.where(inner, (kn << 3) | (fn << 1) | rev.astype(np.int64), code)
+    return code.astype(np.int32), bc
+
+
+def pack_curved(mesh, bc_types, tau_scale=200.0, cub=None):
+    """Host arrays of ``ddgx_op_create_curved``: per-node metrics, per-side
+    normals, weights and penalty, face codes, and with ``cub`` the face
+    cubature masses, volume cubature weights and point tables
+    (sipg.py:142-301; cubature.py:107-221)."""
+    from .cubature import face_mass, volume_weights
+    N, K, Nfp = int(mesh.N), int(mesh.K), int(mesh.Nfp)
+    ref = ref_element(N)
+    if not np.array_equal(np.asarray(mesh.Fmask), ref.Fmask):
+        raise ValueError("mesh.Fmask does not match the degree-N reference element")
+    codes, bc = _side_codes(mesh, bc_types)
+    reftab = np.concatenate([ref.Dr.ravel(), ref.Ds.ravel(), ref.M.ravel(), ref.m1d.ravel()])
+    geo = np.stack([np.asarray(getattr(mesh, n), dtype=np.float64).T
+                    for n in ("rx", "sx", "ry", "sy", "J")], axis=1)       # (K, 5, Np)
+    face = lambda a: np.transpose(np.asarray(a, dtype=np.float64).reshape((Nfp, 3, K),
+                                                                          order="F"), (2, 1, 0))
+    fnorm = np.stack([face(mesh.nx), face(mesh.ny)], axis=-1)             # (K, 3, Nfp, 2)
+    Fs0 = face(mesh.Fscale)[:, :, 0]
+    sJ0 = face(mesh.sJ)[:, :, 0]
+    tau = np.broadcast_to(np.asarray(tau_scale, dtype=np.float64), (3, K)).T
+    w = np.where(bc == BC_DIRICHLET, 1.0, np.where(bc == BC_NEUMANN, 0.0, 0.5))
+    fpar = np.stack([w, w * tau * (N + 1) ** 2 * Fs0, sJ0], axis=-1)      # (K, 3, 3)
+    out = dict(reftab=reftab, geo=geo, fnorm=fnorm, fpar=fpar, codes=codes, ncub=0,
+               fmass=None, volw=None, cubtab=None)
+    if cub is not None:
+        Lq = np.asarray(cub.Lq, dtype=np.float64)
+        if Lq.shape[1] != ref.Np or np.asarray(cub.J).shape[1] != K:
+            raise ValueError("cub was built for a different mesh or order")
+        out.update(ncub=int(Lq.shape[0]), fmass=face_mass(cub), volw=volume_weights(cub),
+                   cubtab=np.stack([np.asarray(cub.Drq, dtype=np.float64),
+                                    np.asarray(cub.Dsq, dtype=np.float64), Lq]))
+    return out
+
+
 TILE = 64          # elements per matrix-core workgroup
 HALO_MAX = 64      # out-of-tile neighbour elements a tile may have
 
@@ -168,10 +251,12 @@ class SipgOperator:
     """
 
     def __init__(self, mesh, bc_types, tau_scale=200.0, kappa=None, dtype=torch.float64,
-                 order="auto"):
+                 order="auto", cub=None):
         """``order``: element processing order of the apply's 64-element tiles:
         ``"auto"`` (``tile_order``), ``None`` (the mesh's own numbering) or a
-        permutation of ``range(K)``."""
+        permutation of ``range(K)``.  A mesh with curved elements, or a
+        ``cub`` (``MeshCubature``), gets the general operator
+        (``ddgx_op_create_curved``) instead of the affine one."""
         if not torch.cuda.is_available():
             raise RuntimeError("SipgOperator needs a HIP device (no CPU fallback)")
         self.N, self.K, self.Np, self.Nfp = int(mesh.N), int(mesh.K), int(mesh.Np), int(mesh.Nfp)
@@ -179,6 +264,16 @@ class SipgOperator:
         self.code = _lib.DDGX_F64 if dtype == torch.float64 else _lib.DDGX_F32
         if dtype not in (torch.float64, torch.float32):
             raise ValueError("dtype must be torch.float64 or torch.float32")
+        self._bj = {}
+        self.cub = cub
+        # the affine kernel merges both sides of a face and reads element
+        # constants: it needs straight-sided elements whose face nodes meet
+        self.curved = cub is not None or not (is_affine(mesh) and is_conforming(mesh))
+        if self.curved:
+            if kappa is not None:
+                raise NotImplementedError("kappa is not supported on curved operators")
+            self._create_curved(mesh, bc_types, tau_scale, cub)
+            return
         rec, nbr, fsj = pack(mesh, bc_types, tau_scale, kappa)
         ref = ref_element(self.N)
         tab = np.concatenate([ref.kernel_tables(),
@@ -205,7 +300,22 @@ class SipgOperator:
                                     C.byref(h)), "ddgx_op_create")
         self._h = h
         self.device = torch.device("cuda", torch.cuda.current_device())
-        self._bj = {}
+
+    def _create_curved(self, mesh, bc_types, tau_scale, cub):
+        d = pack_curved(mesh, bc_types, tau_scale, cub)
+        self.order = None
+        keep = {k: (None if v is None else np.ascontiguousarray(v))
+                for k, v in d.items() if k != "ncub"}
+        ptr = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
+        h = C.c_void_p()
+        torch.cuda.current_device()
+        _lib.check(_lib.lib().ddgx_op_create_curved(
+            self.N, self.K, self.code, int(cub is not None), int(d["ncub"]),
+            ptr(keep["reftab"]), ptr(keep["geo"]), ptr(keep["fnorm"]), ptr(keep["fpar"]),
+            ptr(keep["codes"]), ptr(keep["fmass"]), ptr(keep["volw"]), ptr(keep["cubtab"]),
+            C.byref(h)), "ddgx_op_create_curved")
+        self._h = h
+        self.device = torch.device("cuda", torch.cuda.current_device())
 
     def __del__(self):
         h = getattr(self, "_h", None)

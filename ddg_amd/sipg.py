@@ -13,7 +13,10 @@ repeated calls do not repeat the setup that the reference recomputes on
 every call (sipg.py:412-413, 849-850).
 
 Differences from the reference, by design:
-* ``cub`` (curved-element cubature) is not supported: NotImplementedError.
+* Curved meshes (per-node metrics) and ``cub=`` (a ``MeshCubature`` from
+  ``ddg_amd.cubature`` or the reference's) run on the general operator
+  (``ddgx_op_create_curved``); block-Jacobi and multigrid preconditioners
+  need affine meshes there (NotImplementedError).
 * ``iterative=None`` (direct LU) assembles the dense matrix from device
   applies and factors it with torch on the device; like the reference it
   is only practical for small meshes.
@@ -69,32 +72,28 @@ class IterativeConfig:
 _CACHE: "weakref.WeakKeyDictionary" = weakref.WeakKeyDictionary()
 
 
-def _key_of(bc_types, tau_scale, kappa, dtype):
+def _key_of(bc_types, tau_scale, kappa, dtype, cub=None):
     bc = np.ascontiguousarray(np.asarray(bc_types, dtype=np.int32))
     tau = np.ascontiguousarray(np.asarray(tau_scale, dtype=np.float64))
     kap = None if kappa is None else np.ascontiguousarray(np.asarray(kappa, dtype=np.float64))
     return (bc.tobytes(), bc.shape, tau.tobytes(), tau.shape,
-            None if kap is None else kap.tobytes(), str(dtype))
+            None if kap is None else kap.tobytes(), str(dtype), None if cub is None else id(cub))
 
 
-def get_operator(mesh, bc_types, tau_scale=200.0, kappa=None, dtype=torch.float64) -> SipgOperator:
-    """Cached device operator for this mesh / BC tagging / penalty."""
+def get_operator(mesh, bc_types, tau_scale=200.0, kappa=None, dtype=torch.float64,
+                 cub=None) -> SipgOperator:
+    """Cached device operator for this mesh / BC tagging / penalty / cubature
+    (the operator keeps ``cub`` alive, so its identity is a stable key)."""
     try:
         per_mesh = _CACHE.setdefault(mesh, {})
     except TypeError:  # mesh not weak-referenceable: no caching
-        return SipgOperator(mesh, bc_types, tau_scale, kappa, dtype)
-    key = _key_of(bc_types, tau_scale, kappa, dtype)
+        return SipgOperator(mesh, bc_types, tau_scale, kappa, dtype, cub=cub)
+    key = _key_of(bc_types, tau_scale, kappa, dtype, cub)
     op = per_mesh.get(key)
     if op is None:
-        op = SipgOperator(mesh, bc_types, tau_scale, kappa, dtype)
+        op = SipgOperator(mesh, bc_types, tau_scale, kappa, dtype, cub=cub)
         per_mesh[key] = op
     return op
-
-
-def _check_cub(cub):
-    if cub is not None:
-        raise NotImplementedError("curved-element cubature (cub=...) is not supported by the "
-                                  "MI355X operator; only affine elements")
 
 
 # ---------------------------------------------------------------------------
@@ -181,9 +180,8 @@ def sipg_apply_2d(mesh, u, *, bc_types, u_dir=None, tau_scale=200.0, M_ref=None,
     """``M (-Delta_h) u`` plus the Dirichlet-load offset when ``u_dir`` is
     given (sipg.py:304-337).  ``kappa`` (K,) is an extension for
     ``-div(kappa grad u)``; ``None`` is the reference operator."""
-    _check_cub(cub)
     _check_ref_mats(mesh, M_ref, m1d_per_face)
-    op = get_operator(mesh, bc_types, tau_scale, kappa)
+    op = get_operator(mesh, bc_types, tau_scale, kappa, cub=cub)
     fu = _field(u, op)
     ud = None if u_dir is None else _field(u_dir, op).flat
     return fu.back(op.apply(fu.flat, u_dir=ud))
@@ -195,9 +193,8 @@ def sipg_bilinear_form(mesh, u, v, *, bc_types, u_dir=None, tau_scale=200.0, M_r
     ghost values are homogeneous, so ``a(u, v) = sum(v * apply(u; u_dir))``:
     one device apply and one float64 dot product.  Returns a Python float
     (numpy inputs) or a 0-d device tensor (torch inputs)."""
-    _check_cub(cub)
     _check_ref_mats(mesh, M_ref, m1d_per_face)
-    op = get_operator(mesh, bc_types, tau_scale, kappa)
+    op = get_operator(mesh, bc_types, tau_scale, kappa, cub=cub)
     fu = _field(u, op)
     fv = _field(v, op)
     ud = None if u_dir is None else _field(u_dir, op).flat
@@ -247,14 +244,12 @@ def _dense(op, alpha=0.0, shift=0.0):
 
 def sipg_matrix_2d(mesh, *, bc_types, tau_scale=200.0, cub=None):
     """Dense (Np K)^2 SIPG matrix from device applies (sipg.py:344-364)."""
-    _check_cub(cub)
-    return _dense(get_operator(mesh, bc_types, tau_scale)).cpu().numpy()
+    return _dense(get_operator(mesh, bc_types, tau_scale, cub=cub)).cpu().numpy()
 
 
 def factor_sipg_2d(mesh, *, bc_types, tau_scale=200.0, regularize=False, eps=1.0e-8, cub=None):
     """(A, lu, piv) with the LU factors on the device (sipg.py:367-391)."""
-    _check_cub(cub)
-    A = _dense(get_operator(mesh, bc_types, tau_scale))
+    A = _dense(get_operator(mesh, bc_types, tau_scale, cub=cub))
     if regularize:
         A = A + eps * A.diagonal().abs().max() * torch.eye(A.shape[0], dtype=A.dtype, device=A.device)
     lu, piv = torch.linalg.lu_factor(A)
@@ -264,8 +259,7 @@ def factor_sipg_2d(mesh, *, bc_types, tau_scale=200.0, regularize=False, eps=1.0
 def sipg_helmholtz_matrix_2d(mesh, *, alpha, nu, bc_types, tau_scale=200.0, cub=None):
     """Dense ``alpha M + A`` from device applies (sipg.py:460-496); ``nu``
     is unused, as in the reference."""
-    _check_cub(cub)
-    return _dense(get_operator(mesh, bc_types, tau_scale), alpha=alpha).cpu().numpy()
+    return _dense(get_operator(mesh, bc_types, tau_scale, cub=cub), alpha=alpha).cpu().numpy()
 
 
 def _block_diag_mass(mesh, M_ref) -> np.ndarray:
@@ -283,8 +277,7 @@ def _block_diag_mass(mesh, M_ref) -> np.ndarray:
 
 def factor_sipg_helmholtz_2d(mesh, *, alpha, nu, bc_types, tau_scale=200.0, cub=None):
     """alpha M + A factored (sipg.py:518-533); ``nu`` is unused as there."""
-    _check_cub(cub)
-    A = _dense(get_operator(mesh, bc_types, tau_scale), alpha=alpha)
+    A = _dense(get_operator(mesh, bc_types, tau_scale, cub=cub), alpha=alpha)
     lu, piv = torch.linalg.lu_factor(A)
     return A, lu, piv
 
@@ -390,7 +383,7 @@ def _run_cg(op, b, x0, iterative, preconditioner, alpha=0.0, shift=0.0, refine=N
         raise ValueError(f"iterative method {iterative.method!r} not supported for SPD")
     if refine is not None:
         mesh, bc_types, tau_scale = key
-        op32 = get_operator(mesh, bc_types, tau_scale, dtype=torch.float32)
+        op32 = get_operator(mesh, bc_types, tau_scale, dtype=torch.float32, cub=op.cub)
         if preconditioner is not None and (not isinstance(preconditioner, DevicePreconditioner)
                                            or preconditioner.dtype != torch.float32):
             raise TypeError("refine needs a float32 device preconditioner (dtype=torch.float32)")
@@ -414,12 +407,11 @@ def solve_sipg_poisson_2d(mesh, source, *, bc_types, u_dir=None, q_neu=None, tau
                           cub=None, return_info=False, refine=None):
     """Solve ``-Delta u = source`` with mixed BCs (sipg.py:812-918).
     ``refine=RefineConfig(...)`` runs the mixed-precision refinement."""
-    _check_cub(cub)
-    op = get_operator(mesh, bc_types, tau_scale)
+    op = get_operator(mesh, bc_types, tau_scale, cub=cub)
     f, b = _rhs(op, source, u_dir, q_neu)
     if iterative is None:
         if A_factored is None:
-            A_factored = factor_sipg_2d(mesh, bc_types=bc_types, tau_scale=tau_scale)
+            A_factored = factor_sipg_2d(mesh, bc_types=bc_types, tau_scale=tau_scale, cub=cub)
         x = _lu_solve(A_factored, b).to(op.dtype)
         u = f.back(x)
         return (u, None) if return_info else u
@@ -445,13 +437,12 @@ def solve_sipg_helmholtz_2d(mesh, rhs_field, *, alpha, nu, bc_types, u_dir=None,
                             preconditioner=None, cub=None, return_info=False, refine=None):
     """Solve ``(alpha M - Delta) u = rhs_field`` (sipg.py:536-612).
     ``refine=RefineConfig(...)`` runs the mixed-precision refinement."""
-    _check_cub(cub)
-    op = get_operator(mesh, bc_types, tau_scale)
+    op = get_operator(mesh, bc_types, tau_scale, cub=cub)
     f, b = _rhs(op, rhs_field, u_dir, q_neu)
     if iterative is None:
         if A_factored is None:
             A_factored = factor_sipg_helmholtz_2d(mesh, alpha=alpha, nu=nu, bc_types=bc_types,
-                                                  tau_scale=tau_scale)
+                                                  tau_scale=tau_scale, cub=cub)
         u = f.back(_lu_solve(A_factored, b).to(op.dtype))
         return (u, None) if return_info else u
     xi = None if x0 is None else _field(x0, op).flat.clone()

@@ -81,6 +81,29 @@ const char* ddgx_last_error(void);
 int ddgx_op_create(int N, int64_t K, int dtype, const double* tables,
                    const double* rec, const int32_t* nbr, const double* fsj,
                    const int32_t* order, ddgx_op* out);
+/* General operator for curved (non-affine) elements: the reference's SIPG
+ * form with per-node metrics and, with use_cub, the cubature branch
+ * (sipg.py:142-301 with cub: volume 190-201, faces 281-299; the per-mesh data
+ * of cubature.py:107-221).  Host inputs (float64 / int32), element k's data
+ * contiguous, elements in the caller's order:
+ *   reftab  Dr, Ds, M (Np x Np, row-major), then m1d (3 x Nfp x Nfp)
+ *   geo     K x 5 x Np    rx, sx, ry, sy, J at the nodes
+ *   fnorm   K x 3 x Nfp x 2  outward normal (nx, ny) at the face nodes
+ *   fpar    K x 3 x 3     per element side: w (0.5 interior, 1 Dirichlet,
+ *                         0 Neumann), w tau_scale (N+1)^2 Fscale_0, sJ_0
+ *   codes   K x 3         (k'<<3)|(f'<<1)|reversed, -1 Dirichlet, -2 Neumann
+ *   use_cub = 1: fmass (K x 3 x Nfp x Nfp face cubature mass of each side),
+ *                volw (K x 4 x ncub: wJ |grad r|^2, wJ grad r.grad s,
+ *                wJ |grad s|^2, wJ), cubtab (3 x ncub x Np: Drq, Dsq, Lq);
+ *   use_cub = 0: the nodal quadrature (fmass, volw, cubtab unused).
+ * ddgx_sipg_apply, ddgx_sipg_rhs, ddgx_pcg_solve (without a preconditioner),
+ * ddgx_dot and the solver contexts accept the handle; one apply at a time per
+ * handle (its trace buffers are the handle's).  Block-Jacobi, multigrid and
+ * the parameter gradient need affine operators (DDGX_ERR_UNSUPPORTED). */
+int ddgx_op_create_curved(int N, int64_t K, int dtype, int use_cub, int ncub,
+                          const double* reftab, const double* geo, const double* fnorm,
+                          const double* fpar, const int32_t* codes, const double* fmass,
+                          const double* volw, const double* cubtab, ddgx_op* out);
 int ddgx_op_destroy(ddgx_op op);
 int ddgx_op_info(ddgx_op op, int* N, int64_t* K, int* dtype);
 

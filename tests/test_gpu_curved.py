@@ -1,0 +1,186 @@
+"""The general (curved-element) operator on the GPU against the oracle's
+nodal and cubature branches (sipg.py:142-337 with and without ``cub``).
+
+Mesh: the reference's annulus construction (test/test_curved_boundary_2d.py:
+43-59): [-2, 2]^2 without the square [-0.5, 0.5]^2, inner boundary bent onto
+the circle r = 0.5 with Gordon--Hall blending.  Bars: float64 applies 1e-12
+relative (max norm), float32 2e-5; solves 1e-8 against the oracle's LU.
+"""
+import numpy as np
+import pytest
+import torch
+
+from conftest import rel, unit_square
+from ddg_amd import cubature as pc
+from ddg_amd import curved as pcv
+from ddg_amd import mesh as pm
+from ddg_amd import sipg as D
+from ddg_amd.operator import SipgOperator, refine_solve
+from oracle import cubature as oc
+from oracle import curved as ocv
+from oracle import mesh as om
+from oracle import sipg as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _annulus(N, n=8):
+    args = (-2.0, 2.0, -2.0, 2.0, n, n, -0.5, 0.5, -0.5, 0.5)
+    VX, VY, E = pm.rectangle_with_square_hole_mesh_2d(*args)
+    m = pcv.apply_circular_hole_2d(pm.build_mesh_2d(VX, VY, E, N), cx=0.0, cy=0.0, radius=0.5,
+                                   detection_tol=0.5)
+    mo = ocv.apply_circular_hole_2d(om.build_mesh_2d(VX, VY, E, N), cx=0.0, cy=0.0,
+                                    radius=0.5, detection_tol=0.5)
+    return m, mo
+
+
+def _bc(m):
+    """Outer walls: Dirichlet on x = +-2, Neumann on y = +-2; the curved
+    inner boundary Dirichlet."""
+    bdy = pm.boundary_faces(m)
+    cx, cy = pm.face_centres(m)
+    outer_y = bdy & (np.abs(np.abs(cy) - 2.0) < 1e-9)
+    return np.where(outer_y, 2, np.where(bdy, 1, 0)).astype(np.int32)
+
+
+def _flat(u, dev, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(u.T).reshape(-1), device=dev).to(dtype)
+
+
+def _field(t, Np, K):
+    return t.double().cpu().numpy().reshape(K, Np).T
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("use_cub", [False, True])
+def test_curved_apply_parity(dev, N, use_cub):
+    m, mo = _annulus(N)
+    bc = _bc(m)
+    cub = pc.attach_cubature(m) if use_cub else None
+    cubo = oc.attach_cubature(mo) if use_cub else None
+    rng = np.random.default_rng(10 * N + use_cub)
+    u, ud = rng.standard_normal((m.Np, m.K)), rng.standard_normal((m.Np, m.K))
+    ref = S.sipg_apply_2d(mo, u, bc_types=bc, cub=cubo)
+    refd = S.sipg_apply_2d(mo, u, bc_types=bc, u_dir=ud, cub=cubo)
+    refh = ref + 3.5 * S.mass_apply(mo, u, cub=cubo) + 0.25 * u
+    for dtype, tol in ((torch.float64, 1e-12), (torch.float32, 2e-5)):
+        _say(f"curved apply N={N} cub={use_cub} {dtype} K={m.K}")
+        op = SipgOperator(m, bc, dtype=dtype, cub=cub)
+        # order 1 has straight edges, but the bent vertices leave the mesh
+        # non-conforming: the general operator as well
+        assert op.curved
+        ut, udt = _flat(u, dev, dtype), _flat(ud, dev, dtype)
+        assert rel(_field(op.apply(ut), m.Np, m.K), ref) < tol
+        assert rel(_field(op.apply(ut, u_dir=udt), m.Np, m.K), refd) < tol
+        assert rel(_field(op.apply(ut, alpha=3.5, shift=0.25), m.Np, m.K), refh) < tol
+
+
+@pytest.mark.parametrize("N", [2, 4])
+def test_cubature_operator_on_affine_mesh_equals_affine_operator(dev, N):
+    pmm, _ = unit_square(6, 5, N, jitter=0.1, seed=N)
+    bc = pm.dirichlet_x_neumann_y(pmm)
+    cub = pc.attach_cubature(pmm)
+    a = SipgOperator(pmm, bc)
+    c = SipgOperator(pmm, bc, cub=cub)
+    assert not a.curved and c.curved
+    u = torch.randn(pmm.K * pmm.Np, dtype=torch.float64, device=dev)
+    ud = torch.randn_like(u)
+    _say(f"cubature operator on affine mesh N={N}")
+    ya, yc = a.apply(u, u_dir=ud, alpha=2.0), c.apply(u, u_dir=ud, alpha=2.0)
+    assert float((ya - yc).abs().max()) <= 1e-12 * float(ya.abs().max())
+
+
+@pytest.mark.parametrize("use_cub", [False, True])
+def test_curved_rhs_parity(dev, use_cub):
+    """J M f (or the cubature mass) - L_D (always the nodal form,
+    sipg.py:398-417) + L_N."""
+    m, mo = _annulus(3)
+    bc = _bc(m)
+    cub = pc.attach_cubature(m) if use_cub else None
+    cubo = oc.attach_cubature(mo) if use_cub else None
+    rng = np.random.default_rng(5)
+    f, ud = rng.standard_normal((m.Np, m.K)), rng.standard_normal((m.Np, m.K))
+    q = rng.standard_normal((3 * m.Nfp, m.K))
+    op = SipgOperator(m, bc, cub=cub)
+    _say(f"curved rhs cub={use_cub}")
+    qd = torch.as_tensor(np.ascontiguousarray(q.T).reshape(-1), device=dev)
+    got = _field(op.rhs(_flat(f, dev), u_dir=_flat(ud, dev), q_neu=qd), m.Np, m.K)
+    want = S._rhs(mo, f, bc, ud, q, 200.0, cubo).reshape(m.K, m.Np).T
+    assert rel(got, want) < 1e-12
+
+
+@pytest.mark.parametrize("use_cub", [False, True])
+def test_curved_solves_against_lu(dev, use_cub):
+    """Matrix-free CG through the drop-in (no preconditioner, as the
+    reference's default on curved meshes), Poisson and Helmholtz, and
+    mixed-precision refinement, against the oracle's LU."""
+    m, mo = _annulus(3)
+    bc = _bc(m)
+    cub = pc.attach_cubature(m) if use_cub else None
+    cubo = oc.attach_cubature(mo) if use_cub else None
+    ue = np.sin(np.pi * mo.x / 4) * np.sin(np.pi * mo.y / 4)
+    src = 2 * (np.pi / 4) ** 2 * ue
+    it = D.IterativeConfig(tol=1e-12, maxiter=20000)
+    _say(f"curved Poisson CG cub={use_cub} K={m.K}")
+    u, info = D.solve_sipg_poisson_2d(m, src, bc_types=bc, u_dir=ue, cub=cub, iterative=it,
+                                      return_info=True)
+    ulu = S.solve_sipg_poisson_2d(mo, src, bc_types=bc, u_dir=ue, cub=cubo)
+    assert info["converged"] and rel(u, ulu) < 1e-8
+    _say(f"curved Helmholtz CG cub={use_cub}")
+    uh, info = D.solve_sipg_helmholtz_2d(m, src, alpha=20.0, nu=1.0, bc_types=bc, u_dir=ue,
+                                         cub=cub, iterative=it, return_info=True)
+    uhlu = S.solve_sipg_helmholtz_2d(mo, src, alpha=20.0, nu=1.0, bc_types=bc, u_dir=ue,
+                                     cub=cubo)
+    assert info["converged"] and rel(uh, uhlu) < 1e-8
+    _say(f"curved refinement cub={use_cub}")
+    op64 = D.get_operator(m, bc, cub=cub)
+    op32 = D.get_operator(m, bc, dtype=torch.float32, cub=cub)
+    b = torch.as_tensor(np.random.default_rng(6).standard_normal(m.K * m.Np), device=dev)
+    x, info = refine_solve(op64, op32, b, tol=1e-11, inner_tol=1e-3, inner_maxiter=20000)
+    xlu = np.linalg.solve(S.sipg_matrix_2d(mo, bc_types=bc, cub=cubo), b.cpu().numpy())
+    assert info["converged"] or info["at_floor"]
+    assert rel(x.cpu().numpy(), xlu) < 1e-8
+
+
+def test_curved_manufactured_solution(dev):
+    """test_curved_boundary_2d.py:98-116: 24x24 cells, N=3, all walls
+    Dirichlet with the exact values, cub=None; relative L2 error < 2e-2."""
+    m, _ = _annulus(3, 24)
+    bc = pm.all_dirichlet(m)
+    ue = np.sin(np.pi * m.x / 4) * np.sin(np.pi * m.y / 4)
+    _say(f"curved manufactured K={m.K}")
+    u, info = D.solve_sipg_poisson_2d(m, 2 * (np.pi / 4) ** 2 * ue, bc_types=bc, u_dir=ue,
+                                      iterative=D.IterativeConfig(tol=1e-10, maxiter=50000),
+                                      return_info=True)
+    err = np.sqrt(np.sum(m.J * (u - ue) ** 2) / np.sum(m.J * ue ** 2))
+    assert info["converged"] and err < 2e-2
+
+
+def test_large_curved_mesh_against_oracle(dev):
+    """2 x (128^2 - 16^2) elements, N=4, cubature: the oracle's apply on the
+    same mesh object, symmetry u.Av = v.Au and linearity."""
+    m, _ = _annulus(4, 128)
+    bc = _bc(m)
+    cub = pc.attach_cubature(m)
+    op = SipgOperator(m, bc, cub=cub)
+    rng = np.random.default_rng(11)
+    u, v = rng.standard_normal((m.Np, m.K)), rng.standard_normal((m.Np, m.K))
+    _say(f"curved large K={m.K}")
+    ut, vt = _flat(u, dev), _flat(v, dev)
+    Au, Av = op.apply(ut), op.apply(vt)
+    assert rel(_field(Au, m.Np, m.K), S.sipg_apply_2d(m, u, bc_types=bc, cub=cub)) < 1e-12
+    a, b = float(vt @ Au), float(ut @ Av)
+    assert abs(a - b) <= 1e-11 * abs(a)
+    w = op.apply(2.0 * ut - 3.0 * vt)
+    assert float((w - (2.0 * Au - 3.0 * Av)).abs().max()) <= 1e-11 * float(Au.abs().max())
+
+
+def test_curved_operator_refuses_affine_only_preconditioners(dev):
+    m, _ = _annulus(2)
+    op = SipgOperator(m, _bc(m))
+    with pytest.raises(NotImplementedError):
+        op.block_jacobi()

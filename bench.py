@@ -114,12 +114,13 @@ def apply_bytes(K, Np, s):
     return K * (s * (2 * Np + 14) + 12)
 
 
-def curved_apply_bytes(K, Np, Nfp, ncub, s):
+def curved_apply_bytes(K, Np, Nfp, cub, s):
     """Algorithmic bytes of one curved apply: read u, write y, the per-node
     metrics and Jacobian (5 Np), the face-node normals (6 Nfp), the per-side
     weight, penalty and sJ (9), with cubature the per-side face masses
-    (3 Nfp^2) and per-point volume weights (4 ncub), and 3 int32 face codes."""
-    words = 2 * Np + 5 * Np + 6 * Nfp + 9 + (3 * Nfp * Nfp + 4 * ncub if ncub else 0)
+    (3 Nfp^2) and the element's symmetric volume stiffness (Np(Np+1)/2), and
+    3 int32 face codes."""
+    words = 2 * Np + 5 * Np + 6 * Nfp + 9 + (3 * Nfp * Nfp + Np * (Np + 1) // 2 if cub else 0)
     return K * (s * words + 12)
 
 
@@ -144,7 +145,7 @@ def curved_block(n, N, steps, warmup):
         op = SipgOperator(m, bc, cub=cub)
         t, _ = time_apply(op, u, torch.empty_like(u), steps, warmup)
         ncub = 0 if cub is None else int(np.asarray(cub.w).size)
-        B = curved_apply_bytes(m.K, m.Np, m.Nfp, ncub, 8)
+        B = curved_apply_bytes(m.K, m.Np, m.Nfp, cub is not None, 8)
         out[name] = {"us": round(t * 1e6, 2), "GDOF/s": round(m.K * m.Np / t / 1e9, 2),
                      "bytes": int(B), "frac": round(B / t / 1e9 / HBM_PEAK_GBS, 4),
                      "ncub": ncub}

@@ -69,8 +69,8 @@ def lib():
         "ddgx_last_error": (C.c_char_p, []),
         "ddgx_table_size": (i32, [i32]),
         "ddgx_op_create": (i32, [i32, i64, i32, vp, vp, vp, vp, vp, C.POINTER(vp)]),
-        "ddgx_op_create_curved": (i32, [i32, i64, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp,
-                                         vp, C.POINTER(vp)]),
+        "ddgx_op_create_curved": (i32, [i32, i64, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp,
+                                         C.POINTER(vp)]),
         "ddgx_op_destroy": (i32, [vp]),
         "ddgx_op_info": (i32, [vp, C.POINTER(i32), C.POINTER(i64), C.POINTER(i32)]),
         "ddgx_sipg_apply": (i32, [vp, vp, vp, vp, dbl, dbl, vp]),

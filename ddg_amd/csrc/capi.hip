@@ -265,10 +265,10 @@ int ddgx_op_create(int N, int64_t K, int dtype, const double* tables, const doub
   return DDGX_OK;
 }
 
-int ddgx_op_create_curved(int N, int64_t K, int dtype, int use_cub, int ncub,
-                          const double* reftab, const double* geo, const double* fnorm,
-                          const double* fpar, const int32_t* codes, const double* fmass,
-                          const double* volw, const double* cubtab, ddgx_op* out) {
+int ddgx_op_create_curved(int N, int64_t K, int dtype, int use_cub, const double* reftab,
+                          const double* geo, const double* fnorm, const double* fpar,
+                          const int32_t* codes, const double* fmass, const double* estiff,
+                          const double* emass, ddgx_op* out) {
   if (out == nullptr || reftab == nullptr || geo == nullptr || fnorm == nullptr ||
       fpar == nullptr || codes == nullptr) {
     set_error("ddgx_op_create_curved: null argument");
@@ -284,8 +284,8 @@ int ddgx_op_create_curved(int N, int64_t K, int dtype, int use_cub, int ncub,
     set_error("ddgx_op_create_curved: dtype must be DDGX_F64 or DDGX_F32");
     return DDGX_ERR_INVALID;
   }
-  if (K < 1 || K >= (int64_t(1) << 28) || (use_cub && (ncub < 1 || ncub > 4096))) {
-    set_error("ddgx_op_create_curved: element or cubature point count out of range");
+  if (K < 1 || K >= (int64_t(1) << 28)) {
+    set_error("ddgx_op_create_curved: element count out of range [1, 2^28)");
     return DDGX_ERR_INVALID;
   }
   auto* h = new ddgx_op_s();
@@ -298,7 +298,7 @@ int ddgx_op_create_curved(int N, int64_t K, int dtype, int use_cub, int ncub,
   int rc = DDGX_OK;
   if (hipGetDevice(&o.device) != hipSuccess) rc = hip_fail(hipGetLastError(), "hipGetDevice");
   if (rc == DDGX_OK)
-    rc = curved_create(o, use_cub, ncub, reftab, geo, fnorm, fpar, codes, fmass, volw, cubtab);
+    rc = curved_create(o, use_cub, reftab, geo, fnorm, fpar, codes, fmass, estiff, emass);
   if (rc != DDGX_OK) {
     free_op(o);
     delete h;

@@ -68,6 +68,9 @@ curved_trace_kernel(const T* __restrict__ tab, const T* __restrict__ geo,
   }
 }
 
+// One thread per (element side, face node): the side's jump and normal-
+// derivative average at its node, exchanged through LDS, then row p of the
+// side's face-mass products.  SPB sides per 256-thread block.
 template <typename T, int N>
 __global__ void __launch_bounds__(256)
 curved_side_kernel(const T* __restrict__ tab, const T* __restrict__ fnorm,
@@ -77,72 +80,77 @@ curved_side_kernel(const T* __restrict__ tab, const T* __restrict__ fnorm,
                    int64_t K, const int* __restrict__ skip) {
   if (skip != nullptr && *skip) return;
   constexpr int Np = CShape<N>::Np, Nfp = CShape<N>::Nfp;
+  constexpr int SPB = 256 / Nfp;
+  __shared__ T s_ju[SPB * Nfp], s_av[SPB * Nfp];
   const int64_t nf = K * 3 * Nfp;
   const int64_t nside = K * 3;
   const T* m1d = tab + 3 * Np * Np;
-  const int64_t stride = (int64_t)gridDim.x * 256;
-  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < nside; t += stride) {
-    const int64_t k = t / 3;
-    const int f = (int)(t - k * 3);
-    const int64_t base = t * Nfp;
-    const int c = codes[t];
-    const T w = fpar[t * 3], wtau = fpar[t * 3 + 1], sj0 = fpar[t * 3 + 2];
-    if (c == CODE_NONE || w == T(0)) {
-#pragma unroll
-      for (int p = 0; p < Nfp; ++p)
-        for (int q = 0; q < 3; ++q) {
-          cown[q * nf + base + p] = 0;
-          cnbr[q * nf + base + p] = 0;
+  const int ls = threadIdx.x / Nfp, p = threadIdx.x - (threadIdx.x / Nfp) * Nfp;
+  for (int64_t s0 = (int64_t)blockIdx.x * SPB; s0 < nside; s0 += (int64_t)gridDim.x * SPB) {
+    const int64_t t = s0 + ls;
+    const bool act = ls < SPB && t < nside;
+    int c = CODE_NONE;
+    T w = 0, wtau = 0, sj0 = 0, nx = 0, ny = 0;
+    __syncthreads();
+    if (act) {
+      const int64_t k = t / 3;
+      const int f = (int)(t - k * 3);
+      const int64_t at = t * Nfp + p;
+      c = codes[t];
+      w = fpar[t * 3];
+      wtau = fpar[t * 3 + 1];
+      sj0 = fpar[t * 3 + 2];
+      nx = fnorm[at * 2];
+      ny = fnorm[at * 2 + 1];
+      T ju = 0, av = 0;
+      if (c != CODE_NONE && w != T(0)) {
+        const T uM = tr[at];
+        const T dM = nx * tr[nf + at] + ny * tr[2 * nf + at];
+        T uP, dP;
+        if (c >= 0) {
+          const int64_t o = ((int64_t)nbr_elem(c) * 3 + nbr_face(c)) * Nfp +
+                            (nbr_rev(c) ? Nfp - 1 - p : p);
+          uP = tr[o];
+          dP = nx * tr[nf + o] + ny * tr[2 * nf + o];
+        } else {  // Dirichlet: ghost u_dir (or 0), ghost gradient = own
+          uP = udir != nullptr ? udir[k * Np + fmask<N>(f, p)] : T(0);
+          dP = dM;
         }
+        ju = uM - uP;
+        av = T(0.5) * (dM + dP);
+      }
+      s_ju[ls * Nfp + p] = ju;
+      s_av[ls * Nfp + p] = av;
+    }
+    __syncthreads();
+    if (!act) continue;
+    const int64_t at = t * Nfp + p;
+    if (c == CODE_NONE || w == T(0)) {
+      for (int q = 0; q < 3; ++q) {
+        cown[q * nf + at] = 0;
+        cnbr[q * nf + at] = 0;
+      }
       continue;
     }
+    const int f = (int)(t % 3);
+    T mj = 0, ma = 0;
+#pragma unroll
+    for (int q = 0; q < Nfp; ++q) {
+      const T m = fmass != nullptr ? fmass[t * Nfp * Nfp + q * Nfp + p]
+                                   : sj0 * m1d[f * Nfp * Nfp + q * Nfp + p];
+      mj += m * s_ju[ls * Nfp + q];
+      ma += m * s_av[ls * Nfp + q];
+    }
     const bool inner = c >= 0;
-    int64_t nb = 0;
-    int rev = 0;
-    if (inner) {
-      nb = ((int64_t)nbr_elem(c) * 3 + nbr_face(c)) * Nfp;
-      rev = nbr_rev(c);
-    }
-    T ju[Nfp], av[Nfp];
-#pragma unroll
-    for (int p = 0; p < Nfp; ++p) {
-      const T nx = fnorm[(base + p) * 2], ny = fnorm[(base + p) * 2 + 1];
-      const T uM = tr[base + p];
-      const T dM = nx * tr[nf + base + p] + ny * tr[2 * nf + base + p];
-      T uP, dP;
-      if (inner) {
-        const int64_t o = nb + (rev ? Nfp - 1 - p : p);
-        uP = tr[o];
-        dP = nx * tr[nf + o] + ny * tr[2 * nf + o];
-      } else {  // Dirichlet: ghost u_dir (or 0), ghost gradient = own
-        uP = udir != nullptr ? udir[k * Np + fmask<N>(f, p)] : T(0);
-        dP = dM;
-      }
-      ju[p] = uM - uP;
-      av[p] = T(0.5) * (dM + dP);
-    }
-#pragma unroll
-    for (int p = 0; p < Nfp; ++p) {
-      T mj = 0, ma = 0;
-#pragma unroll
-      for (int q = 0; q < Nfp; ++q) {
-        const T m = fmass != nullptr ? fmass[t * Nfp * Nfp + q * Nfp + p]
-                                     : sj0 * m1d[f * Nfp * Nfp + q * Nfp + p];
-        mj += m * ju[q];
-        ma += m * av[q];
-      }
-      const T gjv = wtau * mj - w * ma;  // d/d[v]
-      const T gav = -w * mj;             // d/d{dv/dn}
-      const T gvM = gjv, gvP = inner ? -gjv : T(0);
-      const T gdM = inner ? T(0.5) * gav : gav, gdP = inner ? T(0.5) * gav : T(0);
-      const T nx = fnorm[(base + p) * 2], ny = fnorm[(base + p) * 2 + 1];
-      cown[base + p] = gvM;
-      cown[nf + base + p] = nx * gdM;
-      cown[2 * nf + base + p] = ny * gdM;
-      cnbr[base + p] = gvP;
-      cnbr[nf + base + p] = nx * gdP;
-      cnbr[2 * nf + base + p] = ny * gdP;
-    }
+    const T gjv = wtau * mj - w * ma;  // d/d[v]
+    const T gav = -w * mj;             // d/d{dv/dn}
+    const T gdM = inner ? T(0.5) * gav : gav, gdP = inner ? T(0.5) * gav : T(0);
+    cown[at] = gjv;
+    cown[nf + at] = nx * gdM;
+    cown[2 * nf + at] = ny * gdM;
+    cnbr[at] = inner ? -gjv : T(0);
+    cnbr[nf + at] = nx * gdP;
+    cnbr[2 * nf + at] = ny * gdP;
   }
 }
 
@@ -155,34 +163,34 @@ __device__ inline double block_sum_256c(double v, double* s_red) {
   return t;
 }
 
-// LDS per element: 8 vectors of Np (u, gv, gx, gy, a, b, vr, vs), then
-// 3 of ncub (cubature: wr, ws, mass).
+// LDS per element: 8 vectors of Np (u, gv, gx, gy, a, b, vr, vs), then with
+// cubature the element's stiffness and mass matrices (symmetric, packed upper
+// triangles of NPK words each).
+
 template <typename T, int N, bool CUB>
 __global__ void __launch_bounds__(256)
 curved_elem_kernel(const T* __restrict__ tab, const T* __restrict__ geo,
-                   const T* __restrict__ cubtab, const T* __restrict__ volw, int ncub,
+                   const T* __restrict__ estiff, const T* __restrict__ emass,
                    const int* __restrict__ codes, const T* __restrict__ cown,
                    const T* __restrict__ cnbr, const T* __restrict__ u, T* __restrict__ y,
                    T alpha, T shift, int epb, int64_t K, double* __restrict__ partial,
                    const int* __restrict__ skip, const T* __restrict__ bvec) {
   if (skip != nullptr && *skip) return;
-  constexpr int Np = CShape<N>::Np, Nfp = CShape<N>::Nfp;
+  constexpr int Np = CShape<N>::Np, Nfp = CShape<N>::Nfp, NN = SymPack<N>::NPK;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ double s_red[4];
   T* sh = reinterpret_cast<T*>(smem_raw);
   const int L = epb * Np;
   T *s_u = sh, *s_gv = sh + L, *s_gx = sh + 2 * L, *s_gy = sh + 3 * L;
   T *s_a = sh + 4 * L, *s_b = sh + 5 * L, *s_vr = sh + 6 * L, *s_vs = sh + 7 * L;
-  T *s_wr = sh + 8 * L, *s_ws = s_wr + epb * ncub, *s_m = s_ws + epb * ncub;
+  T *s_S = sh + 8 * L, *s_Q = s_S + epb * NN;
   const T* Dr = tab;
   const T* Ds = tab + Np * Np;
   const T* M = tab + 2 * Np * Np;
-  const T* Drq = cubtab;
-  const T* Dsq = cubtab + (size_t)ncub * Np;
-  const T* Lq = cubtab + 2 * (size_t)ncub * Np;
   const int64_t nf = K * 3 * Nfp;
   const int tid = threadIdx.x;
   const int64_t ngroups = (K + epb - 1) / epb;
+  const bool mass = alpha != T(0);
   double dacc = 0.0;
   for (int64_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
     const int64_t k0 = g * epb;
@@ -193,6 +201,12 @@ curved_elem_kernel(const T* __restrict__ tab, const T* __restrict__ geo,
       s_gv[idx] = 0;
       s_gx[idx] = 0;
       s_gy[idx] = 0;
+    }
+    if constexpr (CUB) {
+      // the group's element matrices are contiguous: coalesced loads
+      for (int idx = tid; idx < ne * NN; idx += 256) s_S[idx] = estiff[k0 * NN + idx];
+      if (mass)
+        for (int idx = tid; idx < ne * NN; idx += 256) s_Q[idx] = emass[k0 * NN + idx];
     }
     __syncthreads();
     // face cotangents, one face at a time (a vertex node lies on two faces)
@@ -218,6 +232,7 @@ curved_elem_kernel(const T* __restrict__ tab, const T* __restrict__ geo,
       __syncthreads();
     }
     if constexpr (!CUB) {
+      // nodal volume: d/dv of sum J (ux (M vx) + uy (M vy)), per-node metrics
       for (int idx = tid; idx < ne * Np; idx += 256) {
         const int e = idx / Np, i = idx - e * Np;
         const T* ue = s_u + e * Np;
@@ -233,70 +248,51 @@ curved_elem_kernel(const T* __restrict__ tab, const T* __restrict__ geo,
         s_b[idx] = J * (gk[2 * Np + i] * ur + gk[3 * Np + i] * us);
       }
       __syncthreads();
-      for (int idx = tid; idx < ne * Np; idx += 256) {
-        const int e = idx / Np, i = idx - e * Np;
+    }
+    for (int idx = tid; idx < ne * Np; idx += 256) {
+      const int e = idx / Np, i = idx - e * Np;
+      T wx = s_gx[idx], wy = s_gy[idx];
+      if constexpr (!CUB) {
         const T* ae = s_a + e * Np;
         const T* be = s_b + e * Np;
-        T wx = s_gx[idx], wy = s_gy[idx];
 #pragma unroll
         for (int j = 0; j < Np; ++j) {
           wx += M[i * Np + j] * ae[j];
           wy += M[i * Np + j] * be[j];
         }
-        const T* gk = geo + (k0 + e) * 5 * Np;
-        s_vr[idx] = gk[i] * wx + gk[2 * Np + i] * wy;
-        s_vs[idx] = gk[Np + i] * wx + gk[3 * Np + i] * wy;
       }
-    } else {
-      for (int idx = tid; idx < ne * ncub; idx += 256) {
-        const int e = idx / ncub, q = idx - e * ncub;
-        const T* ue = s_u + e * Np;
-        T ur = 0, us = 0, uq = 0;
-        for (int j = 0; j < Np; ++j) {
-          const T uj = ue[j];
-          ur += Drq[q * Np + j] * uj;
-          us += Dsq[q * Np + j] * uj;
-          uq += Lq[q * Np + j] * uj;
-        }
-        const T* vw = volw + (k0 + e) * 4 * ncub;
-        s_wr[idx] = vw[q] * ur + vw[ncub + q] * us;
-        s_ws[idx] = vw[ncub + q] * ur + vw[2 * ncub + q] * us;
-        s_m[idx] = vw[3 * ncub + q] * uq;
-      }
-      for (int idx = tid; idx < ne * Np; idx += 256) {
-        const int e = idx / Np, i = idx - e * Np;
-        const T* gk = geo + (k0 + e) * 5 * Np;
-        const T wx = s_gx[idx], wy = s_gy[idx];
-        s_vr[idx] = gk[i] * wx + gk[2 * Np + i] * wy;
-        s_vs[idx] = gk[Np + i] * wx + gk[3 * Np + i] * wy;
-      }
+      const T* gk = geo + (k0 + e) * 5 * Np;
+      s_vr[idx] = gk[i] * wx + gk[2 * Np + i] * wy;
+      s_vs[idx] = gk[Np + i] * wx + gk[3 * Np + i] * wy;
     }
     __syncthreads();
     for (int idx = tid; idx < ne * Np; idx += 256) {
       const int e = idx / Np, i = idx - e * Np;
       const T* vre = s_vr + e * Np;
       const T* vse = s_vs + e * Np;
+      const T* ue = s_u + e * Np;
       T acc = s_gv[idx];
 #pragma unroll
       for (int j = 0; j < Np; ++j) acc += Dr[j * Np + i] * vre[j] + Ds[j * Np + i] * vse[j];
       if constexpr (CUB) {
-        const T* wre = s_wr + e * ncub;
-        const T* wse = s_ws + e * ncub;
-        const T* me = s_m + e * ncub;
-        T mass = 0;
-        for (int q = 0; q < ncub; ++q) {
-          acc += Drq[q * Np + i] * wre[q] + Dsq[q * Np + i] * wse[q];
-          mass += Lq[q * Np + i] * me[q];
-        }
-        acc += alpha * mass;
-      } else {
-        if (alpha != T(0)) {
-          const T* ue = s_u + e * Np;
-          T mu = 0;
+        // volume cubature: the element's stiffness row, and its mass row
+        const T* Se = s_S + e * NN;
+        T sv = 0;
 #pragma unroll
-          for (int j = 0; j < Np; ++j) mu += M[i * Np + j] * ue[j];
-          acc += alpha * geo[(k0 + e) * 5 * Np + 4 * Np + i] * mu;
+        for (int j = 0; j < Np; ++j) sv += Se[SymPack<N>::pk(i, j)] * ue[j];
+        acc += sv;
+        if (mass) {
+          const T* Qe = s_Q + e * NN;
+          T mv = 0;
+#pragma unroll
+          for (int j = 0; j < Np; ++j) mv += Qe[SymPack<N>::pk(i, j)] * ue[j];
+          acc += alpha * mv;
         }
+      } else if (mass) {
+        T mu = 0;
+#pragma unroll
+        for (int j = 0; j < Np; ++j) mu += M[i * Np + j] * ue[j];
+        acc += alpha * geo[(k0 + e) * 5 * Np + 4 * Np + i] * mu;
       }
       acc += shift * s_u[idx];
       const int64_t gi = k0 * Np + idx;
@@ -310,66 +306,51 @@ curved_elem_kernel(const T* __restrict__ tab, const T* __restrict__ geo,
   }
 }
 
-// rhs = mass(f) + L_N(q): the nodal (J M f) or cubature mass and the Neumann
-// surface load sJ_0 m1d q on Neumann sides (sipg.py:420-453, 849-864;
-// cubature.py:242-245).  One thread per node of an element group in LDS.
+// rhs = mass(f) + L_N(q): the nodal (J M f) or cubature (element matrix)
+// mass and the Neumann surface load sJ_0 m1d q on Neumann sides
+// (sipg.py:420-453, 849-864; cubature.py:242-245).  One thread per node.
 template <typename T, int N, bool CUB>
 __global__ void __launch_bounds__(256)
 curved_mass_kernel(const T* __restrict__ tab, const T* __restrict__ geo,
-                   const T* __restrict__ cubtab, const T* __restrict__ volw, int ncub,
-                   const int* __restrict__ codes, const T* __restrict__ fpar,
-                   const T* __restrict__ f, const T* __restrict__ q, T* __restrict__ out,
-                   int epb, int64_t K) {
+                   const T* __restrict__ emass, const int* __restrict__ codes,
+                   const T* __restrict__ fpar, const T* __restrict__ f, const T* __restrict__ q,
+                   T* __restrict__ out, int64_t K) {
   constexpr int Np = CShape<N>::Np, Nfp = CShape<N>::Nfp;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* s_f = reinterpret_cast<T*>(smem_raw);
-  T* s_m = s_f + epb * Np;
   const T* M = tab + 2 * Np * Np;
   const T* m1d = tab + 3 * Np * Np;
-  const T* Lq = cubtab + 2 * (size_t)ncub * Np;
-  const int tid = threadIdx.x;
-  const int64_t ngroups = (K + epb - 1) / epb;
-  for (int64_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
-    const int64_t k0 = g * epb;
-    const int ne = (int)((K - k0) < epb ? (K - k0) : epb);
-    __syncthreads();
-    for (int idx = tid; idx < ne * Np; idx += 256) s_f[idx] = f != nullptr ? f[k0 * Np + idx] : T(0);
-    __syncthreads();
-    if constexpr (CUB) {
-      for (int idx = tid; idx < ne * ncub; idx += 256) {
-        const int e = idx / ncub, qq = idx - e * ncub;
-        T v = 0;
-        for (int j = 0; j < Np; ++j) v += Lq[qq * Np + j] * s_f[e * Np + j];
-        s_m[idx] = volw[(k0 + e) * 4 * ncub + 3 * ncub + qq] * v;
-      }
-      __syncthreads();
-    }
-    for (int idx = tid; idx < ne * Np; idx += 256) {
-      const int e = idx / Np, i = idx - e * Np;
-      const int64_t k = k0 + e;
-      T acc = 0;
+  const int64_t n = K * Np;
+  for (int64_t gi = (int64_t)blockIdx.x * 256 + threadIdx.x; gi < n;
+       gi += (int64_t)gridDim.x * 256) {
+    const int64_t k = gi / Np;
+    const int i = (int)(gi - k * Np);
+    T acc = 0;
+    if (f != nullptr) {
+      const T* fk = f + k * Np;
+      T mf = 0;
       if constexpr (CUB) {
-        for (int qq = 0; qq < ncub; ++qq) acc += Lq[qq * Np + i] * s_m[e * ncub + qq];
-      } else {
-        T mf = 0;
+        const T* Qk = emass + k * SymPack<N>::NPK;
 #pragma unroll
-        for (int j = 0; j < Np; ++j) mf += M[i * Np + j] * s_f[e * Np + j];
+        for (int j = 0; j < Np; ++j) mf += Qk[SymPack<N>::pk(i, j)] * fk[j];
+        acc = mf;
+      } else {
+#pragma unroll
+        for (int j = 0; j < Np; ++j) mf += M[i * Np + j] * fk[j];
         acc = geo[k * 5 * Np + 4 * Np + i] * mf;
       }
-      if (q != nullptr) {
-#pragma unroll
-        for (int fc = 0; fc < 3; ++fc) {
-          const int p = facepos<N>(fc, i);
-          if (p < 0 || codes[k * 3 + fc] != CODE_NONE) continue;
-          const T* qf = q + (k * 3 + fc) * Nfp;
-          T v = 0;
-#pragma unroll
-          for (int r = 0; r < Nfp; ++r) v += m1d[fc * Nfp * Nfp + p * Nfp + r] * qf[r];
-          acc += fpar[(k * 3 + fc) * 3 + 2] * v;
-        }
-      }
-      out[k0 * Np + idx] = acc;
     }
+    if (q != nullptr) {
+#pragma unroll
+      for (int fc = 0; fc < 3; ++fc) {
+        const int p = facepos<N>(fc, i);
+        if (p < 0 || codes[k * 3 + fc] != CODE_NONE) continue;
+        const T* qf = q + (k * 3 + fc) * Nfp;
+        T v = 0;
+#pragma unroll
+        for (int r = 0; r < Nfp; ++r) v += m1d[fc * Nfp * Nfp + p * Nfp + r] * qf[r];
+        acc += fpar[(k * 3 + fc) * 3 + 2] * v;
+      }
+    }
+    out[gi] = acc;
   }
 }
 
@@ -386,24 +367,20 @@ static int upload(const double* h, size_t n, void** d) {
 
 void curved_free(Curved* cv) {
   if (cv == nullptr) return;
-  void* ptrs[] = {cv->geo, cv->fnorm, cv->fpar, cv->fmass, cv->volw, cv->cubtab,
-                  cv->reftab, cv->codes, cv->trace, cv->cown, cv->cnbr};
+  void* ptrs[] = {cv->geo,    cv->fnorm,  cv->fpar,  cv->fmass, cv->estiff, cv->emass,
+                  cv->reftab, cv->codes,  cv->trace, cv->cown,  cv->cnbr};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete cv;
 }
 
-static size_t elem_lds(int Np, int ncub, int epb, size_t es) {
-  return (size_t)epb * (8 * (size_t)Np + 3 * (size_t)ncub) * es;
-}
-
-int curved_create(Op& op, int use_cub, int ncub, const double* reftab, const double* geo,
+int curved_create(Op& op, int use_cub, const double* reftab, const double* geo,
                   const double* fnorm, const double* fpar, const int32_t* codes,
-                  const double* fmass, const double* volw, const double* cubtab) {
+                  const double* fmass, const double* estiff, const double* emass) {
   const int Np = op.Np, Nfp = op.Nfp;
   const int64_t K = op.K;
-  if (use_cub && (ncub <= 0 || fmass == nullptr || volw == nullptr || cubtab == nullptr)) {
-    set_error("cubature operator needs ncub > 0, face masses, volume weights and tables");
+  if (use_cub && (fmass == nullptr || estiff == nullptr || emass == nullptr)) {
+    set_error("cubature operator needs face masses and element stiffness and mass matrices");
     return DDGX_ERR_INVALID;
   }
   for (int64_t t = 0; t < 3 * K; ++t) {
@@ -419,7 +396,6 @@ int curved_create(Op& op, int use_cub, int ncub, const double* reftab, const dou
   }
   Curved* cv = new Curved();
   cv->use_cub = use_cub ? 1 : 0;
-  cv->ncub = use_cub ? ncub : 0;
   const bool f64 = op.dtype == DDGX_F64;
   const size_t es = f64 ? 8 : 4;
   auto up = [&](const double* h, size_t n, void** d) {
@@ -427,6 +403,7 @@ int curved_create(Op& op, int use_cub, int ncub, const double* reftab, const dou
   };
   int rc = DDGX_OK;
   const size_t nfn = (size_t)K * 3 * Nfp;
+  const size_t nn = (size_t)K * (Np * (Np + 1) / 2);
   do {
     if ((rc = up(reftab, 3 * (size_t)Np * Np + 3 * (size_t)Nfp * Nfp, &cv->reftab))) break;
     if ((rc = up(geo, (size_t)K * 5 * Np, &cv->geo))) break;
@@ -434,8 +411,8 @@ int curved_create(Op& op, int use_cub, int ncub, const double* reftab, const dou
     if ((rc = up(fpar, (size_t)K * 9, &cv->fpar))) break;
     if (use_cub) {
       if ((rc = up(fmass, nfn * Nfp, &cv->fmass))) break;
-      if ((rc = up(volw, (size_t)K * 4 * ncub, &cv->volw))) break;
-      if ((rc = up(cubtab, 3 * (size_t)ncub * Np, &cv->cubtab))) break;
+      if ((rc = up(estiff, nn, &cv->estiff))) break;
+      if ((rc = up(emass, nn, &cv->emass))) break;
     }
     hipError_t e = hipMalloc((void**)&cv->codes, 3 * (size_t)K * sizeof(int));
     if (e == hipSuccess)
@@ -449,8 +426,10 @@ int curved_create(Op& op, int use_cub, int ncub, const double* reftab, const dou
     curved_free(cv);
     return rc;
   }
-  const int width = cv->ncub > Np ? cv->ncub : Np;
-  int epb = 512 / width;
+  // elements per workgroup of the element kernel: its LDS (elem_words per
+  // element) within 4096 words
+  const int words = 8 * Np + (use_cub ? Np * (Np + 1) : 0);
+  int epb = 4096 / words;
   epb = epb < 1 ? 1 : (epb > 64 ? 64 : epb);
   cv->epb = epb;
   const int64_t cap = 4 * (int64_t)device_cu_count();
@@ -459,7 +438,8 @@ int curved_create(Op& op, int use_cub, int ncub, const double* reftab, const dou
     return (int)(g < 1 ? 1 : (g < 8 * cap ? g : 8 * cap));
   };
   cv->grid_face = grid((int64_t)nfn);
-  cv->grid_side = grid(3 * K);
+  cv->grid_side = grid((int64_t)nfn);   // one thread per side node
+  cv->grid_node = grid((int64_t)K * Np);
   const int64_t ngroups = (K + epb - 1) / epb;
   cv->grid_elem = (int)(ngroups < 2 * cap ? ngroups : 2 * cap);
   op.cv = cv;
@@ -476,23 +456,25 @@ static int curved_t(const Op& op, const void* u, void* y, const void* udir, doub
   hipLaunchKernelGGL((curved_trace_kernel<T, N>), dim3(c.grid_face), dim3(256), 0, s,
                      (const T*)c.reftab, (const T*)c.geo, (const T*)u, (T*)c.trace, K, skip);
   DDGX_HIP(hipGetLastError());
-  const T* fm = (c.use_cub && !face_nodal) ? (const T*)c.fmass : nullptr;
+  const bool cub = c.use_cub && !face_nodal;
+  const T* fm = cub ? (const T*)c.fmass : nullptr;
   hipLaunchKernelGGL((curved_side_kernel<T, N>), dim3(c.grid_side), dim3(256), 0, s,
                      (const T*)c.reftab, (const T*)c.fnorm, (const T*)c.fpar, fm, c.codes,
                      (const T*)c.trace, (const T*)udir, (T*)c.cown, (T*)c.cnbr, K, skip);
   DDGX_HIP(hipGetLastError());
-  const bool cub = c.use_cub && !face_nodal;
-  const size_t lds = elem_lds(op.Np, cub ? c.ncub : 0, c.epb, sizeof(T));
+  const size_t lds = (size_t)c.epb * (8 * op.Np + (cub ? op.Np * (op.Np + 1) : 0)) * sizeof(T);
   if (cub)
     hipLaunchKernelGGL((curved_elem_kernel<T, N, true>), dim3(c.grid_elem), dim3(256), lds, s,
-                       (const T*)c.reftab, (const T*)c.geo, (const T*)c.cubtab, (const T*)c.volw,
-                       c.ncub, c.codes, (const T*)c.cown, (const T*)c.cnbr, (const T*)u, (T*)y,
-                       (T)alpha, (T)shift, c.epb, K, partial, skip, (const T*)bvec);
+                       (const T*)c.reftab, (const T*)c.geo, (const T*)c.estiff,
+                       (const T*)c.emass, c.codes, (const T*)c.cown, (const T*)c.cnbr,
+                       (const T*)u, (T*)y, (T)alpha, (T)shift, c.epb, K, partial, skip,
+                       (const T*)bvec);
   else
     hipLaunchKernelGGL((curved_elem_kernel<T, N, false>), dim3(c.grid_elem), dim3(256), lds, s,
-                       (const T*)c.reftab, (const T*)c.geo, (const T*)nullptr, (const T*)nullptr,
-                       0, c.codes, (const T*)c.cown, (const T*)c.cnbr, (const T*)u, (T*)y,
-                       (T)alpha, (T)shift, c.epb, K, partial, skip, (const T*)bvec);
+                       (const T*)c.reftab, (const T*)c.geo, (const T*)nullptr,
+                       (const T*)nullptr, c.codes, (const T*)c.cown, (const T*)c.cnbr,
+                       (const T*)u, (T*)y, (T)alpha, (T)shift, c.epb, K, partial, skip,
+                       (const T*)bvec);
   DDGX_HIP(hipGetLastError());
   return DDGX_OK;
 }
@@ -530,18 +512,14 @@ template <typename T, int N>
 static int curved_rhs_t(const Op& op, const void* f, const void* udir, const void* q, void* rhs,
                         hipStream_t s) {
   const Curved& c = *op.cv;
-  const bool cub = c.use_cub != 0;
-  const size_t lds = (size_t)c.epb * (op.Np + (cub ? c.ncub : 0)) * sizeof(T);
-  if (cub)
-    hipLaunchKernelGGL((curved_mass_kernel<T, N, true>), dim3(c.grid_elem), dim3(256), lds, s,
-                       (const T*)c.reftab, (const T*)c.geo, (const T*)c.cubtab, (const T*)c.volw,
-                       c.ncub, c.codes, (const T*)c.fpar, (const T*)f, (const T*)q, (T*)rhs,
-                       c.epb, op.K);
+  if (c.use_cub)
+    hipLaunchKernelGGL((curved_mass_kernel<T, N, true>), dim3(c.grid_node), dim3(256), 0, s,
+                       (const T*)c.reftab, (const T*)c.geo, (const T*)c.emass, c.codes,
+                       (const T*)c.fpar, (const T*)f, (const T*)q, (T*)rhs, op.K);
   else
-    hipLaunchKernelGGL((curved_mass_kernel<T, N, false>), dim3(c.grid_elem), dim3(256), lds, s,
-                       (const T*)c.reftab, (const T*)c.geo, (const T*)nullptr, (const T*)nullptr,
-                       0, c.codes, (const T*)c.fpar, (const T*)f, (const T*)q, (T*)rhs, c.epb,
-                       op.K);
+    hipLaunchKernelGGL((curved_mass_kernel<T, N, false>), dim3(c.grid_node), dim3(256), 0, s,
+                       (const T*)c.reftab, (const T*)c.geo, (const T*)nullptr, c.codes,
+                       (const T*)c.fpar, (const T*)f, (const T*)q, (T*)rhs, op.K);
   DDGX_HIP(hipGetLastError());
   // - L_D: the nodal-quadrature apply of the zero field with the Dirichlet
   // ghost values (sipg_dirichlet_load_2d has no cubature, sipg.py:398-417),

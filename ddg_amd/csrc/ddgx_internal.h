@@ -147,20 +147,19 @@ __host__ __device__ inline int fc_pack(int slot, int face, int rev, int type) {
 // runs one apply at a time.
 struct Curved {
   int use_cub = 0;         // volume and face integrals by cubature (sipg.py:190-201, 281-299)
-  int ncub = 0;            // volume cubature points
   void* geo = nullptr;     // K x 5 x Np: rx, sx, ry, sy, J at the nodes (dtype)
   void* fnorm = nullptr;   // K x 3 x Nfp x 2: nx, ny at the face nodes
   void* fpar = nullptr;    // K x 3 x 3: w, w tau, sJ at face node 0 of each element side
   void* fmass = nullptr;   // K x 3 x Nfp x Nfp face cubature mass of each side (cubature)
-  void* volw = nullptr;    // K x 4 x ncub: wJ |grad r|^2, wJ grad r.grad s, wJ |grad s|^2, wJ
-  void* cubtab = nullptr;  // 3 x ncub x Np: Drq, Dsq, Lq (row-major)
+  void* estiff = nullptr;  // K x NPK volume cubature stiffness of each element, packed upper
+  void* emass = nullptr;   // K x NPK cubature mass of each element, packed upper (cubature)
   void* reftab = nullptr;  // Dr, Ds, M (Np x Np, row-major), then m1d (3 x Nfp x Nfp)
   int* codes = nullptr;    // K x 3 face codes (nbr_elem / nbr_face / nbr_rev, CODE_*)
   void* trace = nullptr;   // 3 planes of K*3*Nfp: u, du/dx, du/dy at the face nodes
   void* cown = nullptr;    // 3 planes: face cotangents of the own element (v, dv/dx, dv/dy)
   void* cnbr = nullptr;    // 3 planes: face cotangents for the neighbour across each side
   int epb = 1;             // elements per workgroup of the element kernel
-  int grid_face = 0, grid_side = 0, grid_elem = 0;
+  int grid_face = 0, grid_side = 0, grid_elem = 0, grid_node = 0;
 };
 
 struct Op {
@@ -304,9 +303,9 @@ void mg_free(MG* mg);
 int launch_rhs(const Op& op, const void* f, const void* udir, const void* q, void* rhs,
                hipStream_t s);
 // curved operator (csrc/curved.hip)
-int curved_create(Op& op, int use_cub, int ncub, const double* reftab, const double* geo,
+int curved_create(Op& op, int use_cub, const double* reftab, const double* geo,
                   const double* fnorm, const double* fpar, const int32_t* codes,
-                  const double* fmass, const double* volw, const double* cubtab);
+                  const double* fmass, const double* estiff, const double* emass);
 void curved_free(Curved* cv);
 int launch_curved(const Op& op, const void* u, void* y, const void* udir, double alpha,
                   double shift, int mode, double* partial, const int* skip, hipStream_t s,

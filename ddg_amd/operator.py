@@ -169,16 +169,37 @@ def pack_curved(mesh, bc_types, tau_scale=200.0, cub=None):
     tau = np.broadcast_to(np.asarray(tau_scale, dtype=np.float64), (3, K)).T
     w = np.where(bc == BC_DIRICHLET, 1.0, np.where(bc == BC_NEUMANN, 0.0, 0.5))
     fpar = np.stack([w, w * tau * (N + 1) ** 2 * Fs0, sJ0], axis=-1)      # (K, 3, 3)
-    out = dict(reftab=reftab, geo=geo, fnorm=fnorm, fpar=fpar, codes=codes, ncub=0,
-               fmass=None, volw=None, cubtab=None)
+    out = dict(reftab=reftab, geo=geo, fnorm=fnorm, fpar=fpar, codes=codes, fmass=None,
+               estiff=None, emass=None)
     if cub is not None:
         Lq = np.asarray(cub.Lq, dtype=np.float64)
         if Lq.shape[1] != ref.Np or np.asarray(cub.J).shape[1] != K:
             raise ValueError("cub was built for a different mesh or order")
-        out.update(ncub=int(Lq.shape[0]), fmass=face_mass(cub), volw=volume_weights(cub),
-                   cubtab=np.stack([np.asarray(cub.Drq, dtype=np.float64),
-                                    np.asarray(cub.Dsq, dtype=np.float64), Lq]))
+        out.update(fmass=face_mass(cub), **element_matrices(cub))
     return out
+
+
+def element_matrices(cub):
+    """Each element's volume cubature stiffness ``sum_q w J grad L_i . grad
+    L_j`` and mass ``sum_q w J L_i L_j`` (sipg.py:190-201,
+    cubature.py:228-239), both symmetric: (K, Np(Np+1)/2) upper triangles,
+    row after row, from the per-point weights of
+    :func:`ddg_amd.cubature.volume_weights`."""
+    from .cubature import volume_weights
+    vw = volume_weights(cub)                                   # (K, 4, Ncub)
+    Dr = np.asarray(cub.Drq, dtype=np.float64)
+    Ds = np.asarray(cub.Dsq, dtype=np.float64)
+    Lq = np.asarray(cub.Lq, dtype=np.float64)
+    Nq, Np = Lq.shape
+    outer = lambda A, B: (A[:, :, None] * B[:, None, :]).reshape(Nq, Np * Np)
+    rs = outer(Dr, Ds)
+    stiff = (vw[:, 0] @ outer(Dr, Dr) + vw[:, 1] @ (rs + outer(Ds, Dr))
+             + vw[:, 2] @ outer(Ds, Ds))
+    mass = vw[:, 3] @ outer(Lq, Lq)
+    K = vw.shape[0]
+    iu = np.triu_indices(Np)
+    up = lambda a: np.ascontiguousarray(a.reshape(K, Np, Np)[:, iu[0], iu[1]])
+    return {"estiff": up(stiff), "emass": up(mass)}
 
 
 TILE = 64          # elements per matrix-core workgroup
@@ -304,16 +325,15 @@ class SipgOperator:
     def _create_curved(self, mesh, bc_types, tau_scale, cub):
         d = pack_curved(mesh, bc_types, tau_scale, cub)
         self.order = None
-        keep = {k: (None if v is None else np.ascontiguousarray(v))
-                for k, v in d.items() if k != "ncub"}
+        keep = {k: (None if v is None else np.ascontiguousarray(v)) for k, v in d.items()}
         ptr = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
         h = C.c_void_p()
         torch.cuda.current_device()
         _lib.check(_lib.lib().ddgx_op_create_curved(
-            self.N, self.K, self.code, int(cub is not None), int(d["ncub"]),
-            ptr(keep["reftab"]), ptr(keep["geo"]), ptr(keep["fnorm"]), ptr(keep["fpar"]),
-            ptr(keep["codes"]), ptr(keep["fmass"]), ptr(keep["volw"]), ptr(keep["cubtab"]),
-            C.byref(h)), "ddgx_op_create_curved")
+            self.N, self.K, self.code, int(cub is not None), ptr(keep["reftab"]),
+            ptr(keep["geo"]), ptr(keep["fnorm"]), ptr(keep["fpar"]), ptr(keep["codes"]),
+            ptr(keep["fmass"]), ptr(keep["estiff"]), ptr(keep["emass"]), C.byref(h)),
+            "ddgx_op_create_curved")
         self._h = h
         self.device = torch.device("cuda", torch.cuda.current_device())
 

@@ -93,17 +93,18 @@ int ddgx_op_create(int N, int64_t K, int dtype, const double* tables,
  *                         0 Neumann), w tau_scale (N+1)^2 Fscale_0, sJ_0
  *   codes   K x 3         (k'<<3)|(f'<<1)|reversed, -1 Dirichlet, -2 Neumann
  *   use_cub = 1: fmass (K x 3 x Nfp x Nfp face cubature mass of each side),
- *                volw (K x 4 x ncub: wJ |grad r|^2, wJ grad r.grad s,
- *                wJ |grad s|^2, wJ), cubtab (3 x ncub x Np: Drq, Dsq, Lq);
- *   use_cub = 0: the nodal quadrature (fmass, volw, cubtab unused).
+ *                estiff, emass (K x Np(Np+1)/2, upper triangles row after
+ *                row): each element's volume cubature stiffness
+ *                sum_q w_q J_q grad L_i . grad L_j and mass sum_q w_q J_q L_i L_j;
+ *   use_cub = 0: the nodal quadrature (fmass, estiff, emass unused).
  * ddgx_sipg_apply, ddgx_sipg_rhs, ddgx_pcg_solve (without a preconditioner),
  * ddgx_dot and the solver contexts accept the handle; one apply at a time per
  * handle (its trace buffers are the handle's).  Block-Jacobi, multigrid and
  * the parameter gradient need affine operators (DDGX_ERR_UNSUPPORTED). */
-int ddgx_op_create_curved(int N, int64_t K, int dtype, int use_cub, int ncub,
-                          const double* reftab, const double* geo, const double* fnorm,
-                          const double* fpar, const int32_t* codes, const double* fmass,
-                          const double* volw, const double* cubtab, ddgx_op* out);
+int ddgx_op_create_curved(int N, int64_t K, int dtype, int use_cub, const double* reftab,
+                          const double* geo, const double* fnorm, const double* fpar,
+                          const int32_t* codes, const double* fmass, const double* estiff,
+                          const double* emass, ddgx_op* out);
 int ddgx_op_destroy(ddgx_op op);
 int ddgx_op_info(ddgx_op op, int* N, int64_t* K, int* dtype);
 

@@ -64,3 +64,31 @@ def test_attach_cubature_matches_oracle(N):
                                rtol=1e-11, atol=1e-14)
     np.testing.assert_allclose(pc.per_element_mass(cub), oc.per_element_mass(cubo),
                                rtol=1e-11, atol=1e-15)
+
+
+@pytest.mark.parametrize("N", [2, 5])
+def test_element_matrices_match_oracle_cubature(N):
+    """Packed per-element cubature stiffness and mass (what the device
+    operator reads) against the oracle's volume cubature term and mass."""
+    from ddg_amd.operator import element_matrices
+    m, mo = _pair(N)
+    cub, cubo = pc.attach_cubature(m), oc.attach_cubature(mo)
+    em = element_matrices(cub)
+    iu = np.triu_indices(m.Np)
+    S = _fill(em["estiff"], m.K, m.Np, iu)
+    Q = _fill(em["emass"], m.K, m.Np, iu)
+    u = np.random.default_rng(N).standard_normal((m.Np, m.K))
+    urq, usq = cubo.Drq @ u, cubo.Dsq @ u
+    cx = cubo.w[:, None] * cubo.J * (cubo.rx * urq + cubo.sx * usq)
+    cy = cubo.w[:, None] * cubo.J * (cubo.ry * urq + cubo.sy * usq)
+    vol = cubo.Drq.T @ (cubo.rx * cx + cubo.ry * cy) + cubo.Dsq.T @ (cubo.sx * cx + cubo.sy * cy)
+    got = np.einsum("kij,jk->ik", S, u)
+    assert np.abs(got - vol).max() <= 1e-11 * np.abs(vol).max()
+    np.testing.assert_allclose(Q, oc.per_element_mass(cubo), rtol=1e-11, atol=1e-15)
+
+
+def _fill(packed, K, Np, iu):
+    F = np.zeros((K, Np, Np))
+    F[:, iu[0], iu[1]] = packed
+    F[:, iu[1], iu[0]] = packed
+    return F

@@ -313,17 +313,23 @@ def plain_block(m, bc, b, tol, mixed_pc_ms):
     return res
 
 
-def serve_block(n_requests, slots, method="pcg32", calib=200, load=0.8, sample=8, seed=0):
+def serve_block(n_requests, slots, method="pcg32", calib=200, load=0.8, sample=8, seed=0,
+                max_batch=8, batch_dofs=1 << 19, ab_requests=1000):
     """The request stream: Helmholtz solves on 2 n^2 meshes, n in {16, 32, 64,
     128}, N in {2..6}, k^2 in [0, 50], Fourier right-hand sides, mixed
     precision to 1e-10 (``method``: "refine", iterative refinement; "pcg32",
-    float64 CG with the float32 V-cycle).  A closed-loop burst of ``calib`` requests measures
-    the capacity C; then ``n_requests`` arrive as a Poisson stream at
-    ``load * C`` per second.  A sample of results is checked against a
-    float64 block-Jacobi PCG solve to tol 1e-13."""
+    float64 CG with the float32 V-cycle, requests of one class batched up to
+    ``max_batch`` / ``batch_dofs``).  A closed-loop burst of ``calib``
+    requests measures the capacity C; then ``n_requests`` arrive as a Poisson
+    stream at ``load * C`` per second.  With ``ab_requests`` (method "pcg32"):
+    the capacity C1 without batching (one request per slot), and two streams
+    of that many requests at ``load * C1``, without and with batching (the
+    same arrivals).  A sample of results is checked against a float64
+    block-Jacobi PCG solve to tol 1e-13."""
     import torch
     from ddg_amd import serve as SV
-    srv = SV.SolveServer(SV.ServeConfig(slots=slots, method=method))
+    srv = SV.SolveServer(SV.ServeConfig(slots=slots, method=method, max_batch=max_batch,
+                                        batch_dofs=batch_dofs))
     reqs = SV.make_requests(n_requests, seed=seed)
     t0 = time.perf_counter()
     srv.prepare(reqs)
@@ -339,6 +345,23 @@ def serve_block(n_requests, slots, method="pcg32", calib=200, load=0.8, sample=8
     rep = srv.run(reqs, progress_every=500)
     st = rep.stats()
     print(f"[bench] serve stream at {rate:.1f}/s: {st}", file=sys.stderr, flush=True)
+    ab = None
+    if ab_requests and method == "pcg32":
+        keep = ["solves_per_s", "p50_ms", "p99_ms", "service_p50_ms", "service_p99_ms",
+                "converged", "mean_batch"]
+        srv.cfg.max_batch = 1
+        cap1 = srv.run(SV.poisson_arrivals(SV.make_requests(calib, seed=seed + 1), None)).stats()
+        rate1 = load * cap1["solves_per_s"]
+        ab = {"capacity_unbatched_per_s": cap1["solves_per_s"], "rate_per_s": round(rate1, 2),
+              "requests": ab_requests}
+        for label, mb in (("unbatched", 1), ("batched", max_batch)):
+            srv.cfg.max_batch = mb
+            rs = SV.poisson_arrivals(SV.make_requests(ab_requests, seed=seed + 3), rate1,
+                                     seed=seed + 4)
+            sab = srv.run(rs).stats()
+            print(f"[bench] serve {label} at {rate1:.1f}/s: {sab}", file=sys.stderr, flush=True)
+            ab[label] = {k: sab[k] for k in keep}
+        srv.cfg.max_batch = max_batch
     errs = []
     for r in reqs[:sample]:
         pcl = srv.problem_class(r.n, r.N)
@@ -354,8 +377,12 @@ def serve_block(n_requests, slots, method="pcg32", calib=200, load=0.8, sample=8
            "service_p50_ms": st["service_p50_ms"], "service_p99_ms": st["service_p99_ms"],
            "converged": st["converged"], "at_residual_floor": st["at_residual_floor"],
            "max_rel_residual": st["max_rel_residual"], "slots": slots,
-           "peak_slots_busy": st["peak_slots_busy"],
+           "peak_slots_busy": st["peak_slots_busy"], "max_batch_allowed": max_batch,
+           "batch_dofs": batch_dofs,
+           "mean_batch": st["mean_batch"], "max_batch": st["max_batch"],
+           "burst_mean_batch": cap["mean_batch"],
            "closed_loop_capacity_per_s": cap["solves_per_s"], "setup_s": round(setup, 2),
+           "same_rate_ab": ab,
            "tol": 1e-10, "sample_max_rel_err_vs_fp64": max(errs) if errs else None,
            "sample": f"first {len(errs)} requests re-solved by float64 block-Jacobi PCG to 1e-13"}
     return out
@@ -440,6 +467,10 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=2000,
                     help="requests in the serving stream (0: skip the serving leg)")
     ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-max-batch", type=int, default=8,
+                    help="requests of one class run together in a slot, at most this many")
+    ap.add_argument("--serve-batch-dofs", type=int, default=1 << 19,
+                    help="... and at most this many unknowns per batch")
     ap.add_argument("--serve-method", default="pcg32", choices=["pcg32", "refine"])
     ap.add_argument("--serve-only", action="store_true", help="run only the serving leg")
     ap.add_argument("--n", type=int, default=256)
@@ -509,7 +540,10 @@ def main():
         print(json.dumps({"curved": curved_block(args.n, 4, args.steps, args.warmup)}), flush=True)
         return
     if args.serve_only:
-        print(json.dumps({"serve": serve_block(args.serve_requests, args.serve_slots, args.serve_method)}),
+        print(json.dumps({"serve": serve_block(args.serve_requests, args.serve_slots,
+                                               args.serve_method,
+                                               max_batch=args.serve_max_batch,
+                                               batch_dofs=args.serve_batch_dofs)}),
               flush=True)
         return
     m, bc = workload_mesh(args.n, order=args.order)
@@ -609,7 +643,8 @@ def main():
 
     serve = None
     if args.serve_requests > 0:
-        serve = serve_block(args.serve_requests, args.serve_slots, args.serve_method)
+        serve = serve_block(args.serve_requests, args.serve_slots, args.serve_method,
+                            max_batch=args.serve_max_batch, batch_dofs=args.serve_batch_dofs)
 
     cpu = None
     if not args.no_cpu:

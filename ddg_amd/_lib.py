@@ -23,6 +23,7 @@ EXPORTS = (
     "ddgx_dot", "ddgx_mg_create", "ddgx_ir_solve", "ddgx_sipg_param_grad",
     "ddgx_solver_create", "ddgx_solver_bytes", "ddgx_solver_pcg", "ddgx_solver_ir",
     "ddgx_solver_destroy", "ddgx_pcg_solve_noalloc", "ddgx_op_create_curved",
+    "ddgx_batch_create", "ddgx_batch_bytes", "ddgx_batch_pcg", "ddgx_batch_destroy",
 )
 
 
@@ -97,6 +98,11 @@ def lib():
         "ddgx_solver_ir": (i32, [vp, dbl, dbl, vp, vp, C.POINTER(IRConfig),
                                  C.POINTER(IRInfo), vp]),
         "ddgx_solver_destroy": (i32, [vp]),
+        "ddgx_batch_create": (i32, [vp, vp, i32, vp, C.POINTER(vp)]),
+        "ddgx_batch_bytes": (i32, [vp, C.POINTER(i64)]),
+        "ddgx_batch_pcg": (i32, [vp, C.POINTER(dbl), vp, vp, C.POINTER(CGConfig),
+                                 C.POINTER(CGInfo), vp]),
+        "ddgx_batch_destroy": (i32, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -16,6 +16,9 @@ struct ddgx_op_s {
 struct ddgx_pc_s {
   ddgx::PC pc;
 };
+struct ddgx_batch_s {
+  ddgx::Batch* b = nullptr;
+};
 struct ddgx_solver_s {
   ddgx::Op* op = nullptr;     // operator of the solve (float64 for refinement)
   ddgx::Op* op32 = nullptr;   // float32 twin (refinement solver) or null
@@ -78,14 +81,18 @@ static int upload_as(int dtype, const double* src, size_t n, void** dst) {
   return dtype == DDGX_F64 ? upload<double>(src, n, dst) : upload<float>(src, n, dst);
 }
 
-static void free_op(Op& o) {
-  void* ptrs[] = {o.tab, o.rec, o.nbr, o.nbrp, o.fcode, o.perm, o.hpos, o.hab, o.fsj};
+void op_release(Op* op) {
+  Op& o = *op;
+  void* ptrs[] = {o.owns_tab ? o.tab : nullptr, o.rec, o.nbr, o.nbrp, o.fcode, o.perm, o.hpos,
+                  o.hab, o.fsj};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   work_free(o.ws);
   curved_free(o.cv);
   o = Op{};
 }
+
+static void free_op(Op& o) { op_release(&o); }
 
 }  // namespace ddgx
 
@@ -676,6 +683,51 @@ int ddgx_solver_ir(ddgx_solver sv, double alpha, double shift, const void* b, vo
 int ddgx_solver_destroy(ddgx_solver sv) {
   if (sv == nullptr) return DDGX_OK;
   free_solver(sv);
+  return DDGX_OK;
+}
+
+int ddgx_batch_create(ddgx_op op, ddgx_pc pc, int nseg, void* stream, ddgx_batch* out) {
+  if (op == nullptr || out == nullptr) {
+    set_error("ddgx_batch_create: null argument");
+    return DDGX_ERR_INVALID;
+  }
+  *out = nullptr;
+  Batch* b = nullptr;
+  const int rc = batch_create(op->op, pc ? &pc->pc : nullptr, nseg, (hipStream_t)stream, &b);
+  if (rc != DDGX_OK) return rc;
+  auto* h = new ddgx_batch_s();
+  h->b = b;
+  *out = h;
+  return DDGX_OK;
+}
+
+int ddgx_batch_bytes(ddgx_batch bt, int64_t* bytes) {
+  if (bt == nullptr || bytes == nullptr) {
+    set_error("ddgx_batch_bytes: null argument");
+    return DDGX_ERR_INVALID;
+  }
+  *bytes = (int64_t)bt->b->bytes;
+  return DDGX_OK;
+}
+
+int ddgx_batch_pcg(ddgx_batch bt, const double* alpha, const void* b, void* x,
+                   const ddgx_cg_cfg* cfg, ddgx_cg_info* info, void* stream) {
+  if (bt == nullptr || alpha == nullptr || b == nullptr || x == nullptr || cfg == nullptr) {
+    set_error("ddgx_batch_pcg: null argument");
+    return DDGX_ERR_INVALID;
+  }
+  if (cfg->maxiter < 0) {
+    set_error("ddgx_batch_pcg: maxiter must be >= 0");
+    return DDGX_ERR_INVALID;
+  }
+  DDGX_HIP(hipSetDevice(bt->b->op->device));
+  return pcg_batch(*bt->b, alpha, b, x, *cfg, info, (hipStream_t)stream);
+}
+
+int ddgx_batch_destroy(ddgx_batch bt) {
+  if (bt == nullptr) return DDGX_OK;
+  batch_free(bt->b);
+  delete bt;
   return DDGX_OK;
 }
 

@@ -17,6 +17,12 @@
 // Inside an iteration the second stage runs redundantly in every workgroup
 // of the kernel that consumes the result (alpha in the x/r update, beta in
 // the p update), so an iteration has no single-workgroup launch.
+//
+// Batched solves (pcg_batch): the vector and scalar kernels below take a
+// segment index from blockIdx.y -- segment s is one request, with its vectors
+// at offset s*n, its scalars at sc + s and its partial sums at offset
+// s*pstride -- so one launch serves every request of a batch with each
+// request's arithmetic exactly that of its solve alone (gridDim.y = 1).
 
 #include <cmath>
 #include <mutex>
@@ -43,8 +49,18 @@ template <typename T>
 __global__ void __launch_bounds__(256)
 dot2_kernel(const T* __restrict__ a, const T* __restrict__ b, const T* __restrict__ c,
             const T* __restrict__ d, size_t n, double* __restrict__ partial,
-            const int* __restrict__ skip) {
+            const int* __restrict__ skip, int pstride) {
   if (skip != nullptr && *skip) return;
+  {
+    const size_t so = (size_t)blockIdx.y * n;
+    a += so;
+    b += so;
+    if (c != nullptr) {
+      c += so;
+      d += so;
+    }
+    partial += (size_t)blockIdx.y * pstride;
+  }
   __shared__ double s_red[4];
   double s0 = 0.0, s1 = 0.0;
   const size_t stride = (size_t)gridDim.x * 256;
@@ -152,7 +168,10 @@ __device__ inline void cg_advance(CGScalars* sc, double s0, double s1, double g_
 
 // One workgroup: sum the partials in fixed order and advance the scalars.
 __global__ void __launch_bounds__(256)
-finalize_kernel(const double* __restrict__ partial, int nparts, CGScalars* sc, int mode) {
+finalize_kernel(const double* __restrict__ partial, int nparts, CGScalars* sc, int mode,
+                int pstride) {
+  partial += (size_t)blockIdx.y * pstride;
+  sc += blockIdx.y;
   __shared__ double s_red[4];
   double s0, s1;
   block_sum2_partials(partial, nparts, s_red, s0, s1);
@@ -209,7 +228,18 @@ __global__ void __launch_bounds__(256)
 update_kernel(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
               const T* __restrict__ Ap, const CGScalars* __restrict__ sc, size_t n,
               double* __restrict__ partial, float* __restrict__ r32,
-              const double* __restrict__ apart, int napart, int q) {
+              const double* __restrict__ apart, int napart, int q, int pstride) {
+  {
+    const size_t seg = blockIdx.y, so = seg * n;
+    x += so;
+    r += so;
+    p += so;
+    Ap += so;
+    if (r32 != nullptr) r32 += so;
+    if (partial != nullptr) partial += seg * pstride;
+    apart += seg * napart;
+    sc += seg;
+  }
   if (sc->done) return;
   __shared__ double s_red[4];
   const T alpha = cg_alpha<T>(apart, napart, sc, s_red, q);
@@ -325,6 +355,9 @@ __global__ void __launch_bounds__(256)
 pc_down_kernel(const double* __restrict__ r, float* __restrict__ r32,
                const CGScalars* __restrict__ sc, size_t n, const int* __restrict__ skip) {
   if (skip != nullptr && *skip) return;
+  r += blockIdx.y * n;
+  r32 += blockIdx.y * n;
+  sc += blockIdx.y;
   const double inv = 1.0 / pc_scale(sc);
   const size_t stride = (size_t)gridDim.x * 256;
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
@@ -335,6 +368,9 @@ __global__ void __launch_bounds__(256)
 pc_up_kernel(const float* __restrict__ z32, double* __restrict__ z,
              const CGScalars* __restrict__ sc, size_t n, const int* __restrict__ skip) {
   if (skip != nullptr && *skip) return;
+  z32 += blockIdx.y * n;
+  z += blockIdx.y * n;
+  sc += blockIdx.y;
   const double sc_ = pc_scale(sc);
   const size_t stride = (size_t)gridDim.x * 256;
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
@@ -347,8 +383,16 @@ pc_up_kernel(const float* __restrict__ z32, double* __restrict__ z,
 __global__ void __launch_bounds__(256)
 pc_up_dot_kernel(const float* __restrict__ z32, double* __restrict__ z,
                  const double* __restrict__ r, const CGScalars* __restrict__ sc, size_t n,
-                 double* __restrict__ partial, const int* __restrict__ skip) {
+                 double* __restrict__ partial, const int* __restrict__ skip, int pstride) {
   if (skip != nullptr && *skip) return;
+  {
+    const size_t so = blockIdx.y * n;
+    z32 += so;
+    z += so;
+    r += so;
+    sc += blockIdx.y;
+    partial += (size_t)blockIdx.y * pstride;
+  }
   __shared__ double s_red[4];
   const double sc_ = pc_scale(sc);
   double s0 = 0.0, s1 = 0.0;
@@ -398,7 +442,11 @@ pc_up_dot_kernel(const float* __restrict__ z32, double* __restrict__ z,
 template <typename T>
 __global__ void __launch_bounds__(256)
 pupdate_fin_kernel(T* __restrict__ p, const T* __restrict__ z, CGScalars* sc, size_t n,
-                   const double* __restrict__ partial, int nparts, int q) {
+                   const double* __restrict__ partial, int nparts, int q, int pstride) {
+  p += blockIdx.y * n;
+  z += blockIdx.y * n;
+  sc += blockIdx.y;
+  partial += (size_t)blockIdx.y * pstride;
   __shared__ int s_done;
   if (threadIdx.x == 0)
     s_done = __hip_atomic_load(&sc->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -538,14 +586,14 @@ static int dot2(const Op& op, Work& w, const void* a, const void* b, const void*
                 const void* d, const int* skip, hipStream_t s) {
   const size_t n = (size_t)op.K * op.Np;
   hipLaunchKernelGGL((dot2_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (const T*)a,
-                     (const T*)b, (const T*)c, (const T*)d, n, w.partials, skip);
+                     (const T*)b, (const T*)c, (const T*)d, n, w.partials, skip, 0);
   DDGX_HIP(hipGetLastError());
   return DDGX_OK;
 }
 
 static int finalize(Work& w, int nparts, int mode, hipStream_t s) {
   hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, s, (const double*)w.partials,
-                     nparts, w.scal, mode);
+                     nparts, w.scal, mode, 0);
   DDGX_HIP(hipGetLastError());
   return DDGX_OK;
 }
@@ -593,7 +641,7 @@ static int check_work(const Op& op, const Work& w) {
 // the context's chunk (updating the existing executable graph in place when
 // the topology is unchanged).
 template <typename F>
-static int capture_chunk(Work& w, int every, F&& iteration, hipStream_t s) {
+static int capture_chunk(hipGraphExec_t& chunk, int every, F&& iteration, hipStream_t s) {
   DDGX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   int rc = DDGX_OK;
   for (int c = 0; c < every && rc == DDGX_OK; ++c) rc = iteration();
@@ -605,22 +653,22 @@ static int capture_chunk(Work& w, int every, F&& iteration, hipStream_t s) {
   }
   if (ec != hipSuccess) return hip_fail(ec, "hipStreamEndCapture");
   bool updated = false;
-  if (w.chunk) {
+  if (chunk) {
     hipGraphNode_t bad = nullptr;
     hipGraphExecUpdateResult res;
-    updated = hipGraphExecUpdate(w.chunk, g, &bad, &res) == hipSuccess &&
+    updated = hipGraphExecUpdate(chunk, g, &bad, &res) == hipSuccess &&
               res == hipGraphExecUpdateSuccess;
     if (!updated) {
       (void)hipGetLastError();
-      (void)hipGraphExecDestroy(w.chunk);
-      w.chunk = nullptr;
+      (void)hipGraphExecDestroy(chunk);
+      chunk = nullptr;
     }
   }
   hipError_t e = hipSuccess;
-  if (!updated) e = hipGraphInstantiate(&w.chunk, g, nullptr, nullptr, 0);
+  if (!updated) e = hipGraphInstantiate(&chunk, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) {
-    w.chunk = nullptr;
+    chunk = nullptr;
     return hip_fail(e, "hipGraphInstantiate");
   }
   return DDGX_OK;
@@ -698,13 +746,13 @@ static int pcg_t(const Op& op, Work& w, double alpha, double shift, const PC* pc
                          (T*)w.r, (const T*)w.p, (const T*)w.Ap,
                          (const CGScalars*)w.scal, n, has_pc ? nullptr : w.partials,
                          mixed ? (float*)w.pc_r : nullptr,
-                         (const double*)(w.partials + 2 * w.npart), (int)op.grid_apply, q);
+                         (const double*)(w.partials + 2 * w.npart), (int)op.grid_apply, q, 0);
       DDGX_HIP(hipGetLastError());
       if (mixed) {
         CHECK(mg_vcycle(*pc->mg, *mgw, w.pc_r, w.pc_z, done, s));
         hipLaunchKernelGGL(pc_up_dot_kernel, dim3(op.grid_vec), dim3(256), 0, s,
                            (const float*)w.pc_z, (double*)w.z, (const double*)w.r,
-                           (const CGScalars*)w.scal, n, w.partials, done);
+                           (const CGScalars*)w.scal, n, w.partials, done, 0);
         DDGX_HIP(hipGetLastError());
       } else if (has_pc) {
         CHECK(precondition(done));
@@ -713,7 +761,7 @@ static int pcg_t(const Op& op, Work& w, double alpha, double shift, const PC* pc
     }
     // beta, p = z + beta p and the scalars' advance in one launch
     hipLaunchKernelGGL((pupdate_fin_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (T*)w.p,
-                       (const T*)z, w.scal, n, (const double*)w.partials, (int)op.grid_vec, q);
+                       (const T*)z, w.scal, n, (const double*)w.partials, (int)op.grid_vec, q, 0);
     DDGX_HIP(hipGetLastError());
     return DDGX_OK;
   };
@@ -729,7 +777,7 @@ static int pcg_t(const Op& op, Work& w, double alpha, double shift, const PC* pc
     const bool same = w.chunk && w.g_x == x && w.g_pc == (const void*)pc &&
                       w.g_mgw == (const void*)mgw && w.g_alpha == alpha &&
                       w.g_shift == shift && w.g_every == every && w.g_stream == s;
-    if (!same) CHECK(capture_chunk(w, every, iteration, s));
+    if (!same) CHECK(capture_chunk(w.chunk, every, iteration, s));
     w.g_x = x;
     w.g_pc = pc;
     w.g_mgw = mgw;
@@ -780,6 +828,153 @@ int pcg_solve(const Op& op, Work& w, double alpha, double shift, const PC* pc, M
   if (pc != nullptr && pc->type == 1 && mgw == nullptr) mgw = pc->mg->ws;
   if (op.dtype == DDGX_F64) return pcg_t<double>(op, w, alpha, shift, pc, mgw, b, x, cfg, info, s);
   return pcg_t<float>(op, w, alpha, shift, pc, mgw, b, x, cfg, info, s);
+}
+
+// ---------------------------------------------------------------------------
+// Batched PCG (ddgx_batch_pcg): pcg_t's float64 iteration for every segment
+// of a Batch at once.  The vector and scalar kernels run on a (g1, nseg)
+// grid with each segment's offsets (the single solve's grid per segment, so
+// the same partial sums in the same order); the apply and the V-cycle run on
+// the replicated operators; the apply's <p, Ap> partials are per tile, so a
+// segment's are the contiguous range [s t1, (s+1) t1).  Each segment stops on
+// its own test: its x and p freeze once its flag is set, while the apply and
+// V-cycle keep running over every segment until all are done.
+// ---------------------------------------------------------------------------
+int pcg_batch(Batch& bt, const double* alpha, const void* b, void* x, const ddgx_cg_cfg& cfg,
+              ddgx_cg_info* info, hipStream_t s) {
+  using T = double;
+  const int B = bt.nseg;
+  const size_t n = (size_t)bt.n1, nall = n * B;
+  const int G = bt.g1, P2 = 2 * G;
+  const Op& op = *bt.op;
+  const PC* pc = bt.pc;
+  const bool has_pc = pc != nullptr;
+  const bool mixed = has_pc && pc->dtype != DDGX_F64;
+  bool helm = false;
+  for (int i = 0; i < B; ++i) {
+    if (!std::isfinite(alpha[i])) {
+      set_error("ddgx_batch_pcg: non-finite Helmholtz coefficient");
+      return DDGX_ERR_INVALID;
+    }
+    bt.alpha_host[i] = alpha[i];
+    helm = helm || alpha[i] != 0.0;
+  }
+  // per-segment alpha_s J in the Jacobian words; the apply then runs with
+  // alpha = 1 and forms alpha_s J exactly as a single solve's apply does
+  if (helm) CHECK(launch_scale_jacobian(bt, s));
+  const double a_apply = helm ? 1.0 : 0.0;
+  for (int i = 0; i < B; ++i) {
+    CGScalars& c = bt.scal_host[i];
+    c = CGScalars{};
+    c.tol = cfg.tol;
+    c.atol = cfg.atol;
+    c.maxiter = cfg.maxiter;
+    c.use_pc = has_pc ? 1 : 0;
+  }
+  DDGX_HIP(hipMemcpyAsync(bt.scal, bt.scal_host, B * sizeof(CGScalars), hipMemcpyHostToDevice, s));
+  const dim3 vg(G, B), fg(1, B), blk(256);
+  void* z = has_pc ? bt.z : bt.r;
+  auto dot2b = [&](const void* a1, const void* b1, const void* c1, const void* d1) -> int {
+    hipLaunchKernelGGL((dot2_kernel<T>), vg, blk, 0, s, (const T*)a1, (const T*)b1,
+                       (const T*)c1, (const T*)d1, n, bt.partials, nullptr, P2);
+    DDGX_HIP(hipGetLastError());
+    return DDGX_OK;
+  };
+  auto finb = [&](int mode) -> int {
+    hipLaunchKernelGGL(finalize_kernel, fg, blk, 0, s, (const double*)bt.partials, G, bt.scal,
+                       mode, P2);
+    DDGX_HIP(hipGetLastError());
+    return DDGX_OK;
+  };
+  CHECK(dot2b(b, b, nullptr, nullptr));
+  CHECK(finb(F_INIT_BB));
+  CHECK(launch_apply(op, x, bt.Ap, nullptr, a_apply, 0.0, 0, nullptr, nullptr, s));
+  hipLaunchKernelGGL((residual_kernel<T>), dim3(op.grid_vec), blk, 0, s, (const T*)b,
+                     (const T*)bt.Ap, (T*)bt.r, nall);
+  DDGX_HIP(hipGetLastError());
+  if (mixed) {
+    CHECK(dot2b(bt.r, bt.r, bt.r, bt.r));
+    CHECK(finb(F_SUM));
+    hipLaunchKernelGGL(pc_down_kernel, vg, blk, 0, s, (const double*)bt.r, (float*)bt.pc_r,
+                       (const CGScalars*)bt.scal, n, nullptr);
+    DDGX_HIP(hipGetLastError());
+    CHECK(mg_vcycle(*pc->mg, *bt.mgw, bt.pc_r, bt.pc_z, nullptr, s));
+    hipLaunchKernelGGL(pc_up_kernel, vg, blk, 0, s, (const float*)bt.pc_z, (double*)bt.z,
+                       (const CGScalars*)bt.scal, n, nullptr);
+    DDGX_HIP(hipGetLastError());
+  } else if (has_pc) {
+    CHECK(mg_vcycle(*pc->mg, *bt.mgw, bt.r, bt.z, nullptr, s));
+  }
+  hipLaunchKernelGGL((copy_kernel<T>), dim3(op.grid_vec), blk, 0, s, (const T*)z, (T*)bt.p,
+                     nall);
+  DDGX_HIP(hipGetLastError());
+  CHECK(dot2b(bt.r, z, bt.r, bt.r));
+  CHECK(finb(F_INIT_GAMMA));
+
+  int iter_no = 0;
+  auto iteration = [&]() -> int {
+    const int q = iter_no & 1;
+    ++iter_no;
+    CHECK(launch_apply(op, bt.p, bt.Ap, nullptr, a_apply, 0.0, M_DOT, bt.apart, nullptr, s));
+    hipLaunchKernelGGL((update_kernel<T>), vg, blk, 0, s, (T*)x, (T*)bt.r, (const T*)bt.p,
+                       (const T*)bt.Ap, (const CGScalars*)bt.scal, n,
+                       has_pc ? nullptr : bt.partials, mixed ? (float*)bt.pc_r : nullptr,
+                       (const double*)bt.apart, bt.t1, q, P2);
+    DDGX_HIP(hipGetLastError());
+    if (mixed) {
+      CHECK(mg_vcycle(*pc->mg, *bt.mgw, bt.pc_r, bt.pc_z, nullptr, s));
+      hipLaunchKernelGGL(pc_up_dot_kernel, vg, blk, 0, s, (const float*)bt.pc_z, (double*)bt.z,
+                         (const double*)bt.r, (const CGScalars*)bt.scal, n, bt.partials,
+                         nullptr, P2);
+      DDGX_HIP(hipGetLastError());
+    } else if (has_pc) {
+      CHECK(mg_vcycle(*pc->mg, *bt.mgw, bt.r, bt.z, nullptr, s));
+      CHECK(dot2b(bt.r, bt.z, bt.r, bt.r));
+    }
+    hipLaunchKernelGGL((pupdate_fin_kernel<T>), vg, blk, 0, s, (T*)bt.p, (const T*)z, bt.scal,
+                       n, (const double*)bt.partials, G, q, P2);
+    DDGX_HIP(hipGetLastError());
+    return DDGX_OK;
+  };
+  int every = cfg.check_every > 0 ? cfg.check_every : 16;
+  // on a non-null stream a captured chunk of `every` iterations is replayed
+  // as one graph launch (even length: a chunk fixes its iterations' parities)
+  const bool graph = s != nullptr && cfg.maxiter > 0;
+  if (graph && (every & 1)) ++every;
+  if (graph) {
+    const bool same = bt.chunk && bt.g_x == x && bt.g_a == a_apply && bt.g_every == every &&
+                      bt.g_stream == s;
+    if (!same) CHECK(capture_chunk(bt.chunk, every, iteration, s));
+    bt.g_x = x;
+    bt.g_a = a_apply;
+    bt.g_every = every;
+    bt.g_stream = s;
+  }
+  int launched = 0;
+  while (true) {
+    DDGX_HIP(hipMemcpyAsync(bt.scal_host, bt.scal, B * sizeof(CGScalars), hipMemcpyDeviceToHost,
+                            s));
+    DDGX_HIP(hipStreamSynchronize(s));
+    bool all = true;
+    for (int i = 0; i < B; ++i) all = all && bt.scal_host[i].done;
+    if (all || launched >= cfg.maxiter) break;
+    if (graph) {
+      DDGX_HIP(hipGraphLaunch(bt.chunk, s));
+      launched += every;
+    } else {
+      for (int c = 0; c < every && launched < cfg.maxiter; ++c, ++launched) CHECK(iteration());
+    }
+  }
+  if (info != nullptr) {
+    for (int i = 0; i < B; ++i) {
+      const CGScalars& h = bt.scal_host[i];
+      info[i].iterations = h.k;
+      info[i].residual2 = h.rs;
+      info[i].atol2 = h.atol2;
+      info[i].converged = h.rs <= h.atol2 ? 1 : 0;
+    }
+  }
+  return DDGX_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -186,6 +186,10 @@ struct Op {
   int halo_cap = 0;      // HALO_SMALL or HALO_MAX: halo entries stored per tile
   int grid_vec = 0;      // grid for vector kernels / reductions
   struct Work* ws = nullptr;  // default Krylov state (ddgx_pcg_solve / ddgx_ir_solve / ddgx_dot)
+  // A batch replica (op_replicate): nseg disjoint copies of a source operator
+  // of K / nseg elements each, sharing the source's table (not owned).
+  int nseg = 1;
+  bool owns_tab = true;
 };
 
 // Krylov state of one solve on an operator's vector shape and dtype.  The
@@ -264,8 +268,13 @@ struct MG {
   MGLevel* lv = nullptr;
   int nlev = 0;
   double* Ainv = nullptr;       // coarsest level dense inverse (float64), n x n
-  int nc = 0;                   // unknowns of the coarsest level
+  int nc = 0;                   // unknowns of the coarsest level (of one segment)
   MGWork* ws = nullptr;         // default cycle buffers (ddgx_pc_apply, ddgx_pcg_solve)
+  // A batch replica (mg_replicate): every level is nseg disjoint copies and
+  // the coarse solve applies Ainv to each segment; the level operators are
+  // then owned by the hierarchy.
+  int nseg = 1;
+  bool owns_ops = false;
 };
 int mgwork_create(const MG& mg, MGWork** out);
 void mgwork_free(MGWork* w);
@@ -318,6 +327,41 @@ int launch_invert_blocks(int N, int64_t K, int dtype, const double* blocks, void
 int launch_bj_apply(int N, int64_t K, int dtype, const void* binv, const void* r, void* z,
                     hipStream_t s);
 size_t packed_block_words(int N);
+
+// Batched solves: nseg requests of one problem class in every launch.
+int op_replicate(const Op& src, int nseg, hipStream_t s, Op** out);
+void op_release(Op* op);  // frees what the operator owns (any Op made here)
+int mg_replicate(const MG& src, int nseg, hipStream_t s, MG** out);
+struct Batch {
+  int nseg = 0;
+  Op* op = nullptr;       // float64 operator on nseg copies (owned)
+  PC* pc = nullptr;       // multigrid on nseg copies (owned), or null
+  MGWork* mgw = nullptr;  // its cycle buffers
+  int64_t n1 = 0;         // unknowns of one segment
+  int g1 = 0, t1 = 0;     // vector grid and tiles of one segment (the source operator's)
+  void* j1 = nullptr;     // the source operator's Jacobian record plane (tile order)
+  double* alpha = nullptr;  // nseg Helmholtz coefficients on the device
+  double* alpha_host = nullptr;
+  void *r = nullptr, *p = nullptr, *z = nullptr, *Ap = nullptr;
+  void *pc_r = nullptr, *pc_z = nullptr;  // float32 copies around a float32 V-cycle
+  double* partials = nullptr;  // nseg x 2 g1 vector reductions
+  double* apart = nullptr;     // nseg x t1 apply <p, Ap> partials (one per tile)
+  CGScalars* scal = nullptr;
+  CGScalars* scal_host = nullptr;  // pinned
+  size_t bytes = 0;
+  // captured chunk of batched iterations and what it was captured for
+  hipGraphExec_t chunk = nullptr;
+  const void* g_x = nullptr;
+  double g_a = 0.0;
+  int g_every = 0;
+  hipStream_t g_stream = nullptr;
+};
+int batch_create(const Op& op, const PC* pc, int nseg, hipStream_t s, Batch** out);
+void batch_free(Batch* b);
+int pcg_batch(Batch& bt, const double* alpha, const void* b, void* x, const ddgx_cg_cfg& cfg,
+              ddgx_cg_info* info, hipStream_t s);
+// rec plane R_J of segment s (tile order) = alpha[s] * j1, s < nseg
+int launch_scale_jacobian(Batch& bt, hipStream_t s);
 
 int pcg_solve_noalloc(const Op& op, double alpha, double shift, const PC* pc, const void* b,
                       void* x, const ddgx_cg_cfg& cfg, ddgx_cg_info* info, hipStream_t s);

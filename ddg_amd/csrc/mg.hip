@@ -153,11 +153,15 @@ static int gemv_jc(int n) {
   return jc < 16 ? 16 : jc;
 }
 
+// Segment blockIdx.z (batched hierarchies: one coarse vector per request,
+// the same inverse for all) reads b and writes part at its own offsets.
 template <typename T>
 __global__ void __launch_bounds__(256)
 coarse_gemv_partial_kernel(const double* __restrict__ W, const T* __restrict__ b,
                            double* __restrict__ part, int n, int jc, const int* __restrict__ skip) {
   if (skip != nullptr && *skip) return;
+  b += (size_t)blockIdx.z * n;
+  part += (size_t)blockIdx.z * gridDim.y * n;
   __shared__ double sb[GEMV_JC];
   const int j0 = blockIdx.y * jc;
   const int nj = min(jc, n - j0);
@@ -177,6 +181,8 @@ __global__ void __launch_bounds__(64)
 coarse_gemv_sum_kernel(const double* __restrict__ part, T* __restrict__ x, int n, int nchunks,
                        const int* __restrict__ skip) {
   if (skip != nullptr && *skip) return;
+  part += (size_t)blockIdx.y * nchunks * n;  // segment blockIdx.y
+  x += (size_t)blockIdx.y * n;
   const int i = blockIdx.x * 64 + threadIdx.x;
   if (i >= n) return;
   double acc = 0.0;
@@ -296,10 +302,11 @@ static int coarse_solve(const MG& mg, MGWork& w, const void* b, void* x, const i
   const int n = mg.nc;
   const int jc = gemv_jc(n);
   const int nch = (n + jc - 1) / jc;
-  hipLaunchKernelGGL((coarse_gemv_partial_kernel<T>), dim3((n + 255) / 256, nch), dim3(256), 0,
-                     s, (const double*)mg.Ainv, (const T*)b, w.gemv_part, n, jc, skip);
+  hipLaunchKernelGGL((coarse_gemv_partial_kernel<T>), dim3((n + 255) / 256, nch, mg.nseg),
+                     dim3(256), 0, s, (const double*)mg.Ainv, (const T*)b, w.gemv_part, n, jc,
+                     skip);
   DDGX_HIP(hipGetLastError());
-  hipLaunchKernelGGL((coarse_gemv_sum_kernel<T>), dim3((n + 63) / 64), dim3(64), 0, s,
+  hipLaunchKernelGGL((coarse_gemv_sum_kernel<T>), dim3((n + 63) / 64, mg.nseg), dim3(64), 0, s,
                      (const double*)w.gemv_part, (T*)x, n, nch, skip);
   DDGX_HIP(hipGetLastError());
   return DDGX_OK;
@@ -442,7 +449,7 @@ int mgwork_create(const MG& mg, MGWork** out) {
   }
   const int jc = gemv_jc(mg.nc);
   const int nch = (mg.nc + jc - 1) / jc;
-  const size_t ng = (size_t)nch * mg.nc * sizeof(double);
+  const size_t ng = (size_t)nch * mg.nc * mg.nseg * sizeof(double);
   if (e == hipSuccess) e = hipMalloc((void**)&w->gemv_part, ng);
   w->bytes += ng;
   if (e != hipSuccess) {
@@ -468,6 +475,10 @@ void mg_free(MG* mg) {
     MGLevel& L = mg->lv[l];
     if (L.binv) (void)hipFree(L.binv);
     if (L.P) (void)hipFree(L.P);
+    if (mg->owns_ops && L.op) {
+      op_release(L.op);
+      delete L.op;
+    }
   }
   delete[] mg->lv;
   if (mg->Ainv) (void)hipFree(mg->Ainv);

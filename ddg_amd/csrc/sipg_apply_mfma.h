@@ -509,7 +509,9 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
     for (int off = 32; off > 0; off >>= 1) dacc += __shfl_down(dacc, off, 64);
     if (lane == 0) s_red[wv] = dacc;
     __syncthreads();
-    if (tid == 0) partial[blockIdx.x] = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
+    // one partial per tile, at the tile's index (the tiles of a batched
+    // solve's request are then one contiguous range of partials)
+    if (tid == 0) partial[tile] = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
   }
 }
 

@@ -284,7 +284,7 @@ sipg_apply_valu_kernel(const T* __restrict__ tab, const T* __restrict__ rec,
   }
   if constexpr (DOT) {
     for (int off = 32; off > 0; off >>= 1) dacc += __shfl_down(dacc, off, 64);
-    if (el == 0) partial[blockIdx.x] = dacc;
+    if (el == 0) partial[tile] = dacc;  // one partial per tile, at the tile's index
   }
 }
 

@@ -426,6 +426,68 @@ class SipgOperator:
                    "residual2": info.residual2, "atol2": info.atol2}
 
 
+class BatchSolver:
+    """``ddgx_batch``: ``nseg`` problems of one class -- the float64 operator
+    ``op`` and the multigrid ``pc`` (float32 or float64, or None) -- solved
+    together, one kernel launch per step for all of them.  Each problem has
+    its own right-hand side and Helmholtz coefficient; its solution is bitwise
+    the one ``op.pcg(b, alpha=a, pc=pc)`` returns for it alone.  The batch
+    keeps device copies of ``op`` and ``pc`` (``nbytes`` in all)."""
+
+    def __init__(self, op: SipgOperator, pc=None, nseg: int = 1):
+        if op.dtype != torch.float64:
+            raise ValueError("batched solves take a float64 operator")
+        if pc is not None and not getattr(pc, "is_multigrid", False):
+            raise ValueError("batched solves take a multigrid preconditioner or none")
+        self.nseg, self.K, self.Np = int(nseg), op.K, op.Np
+        self.n = op.K * op.Np
+        self.device = op.device
+        h = C.c_void_p()
+        _lib.check(_lib.lib().ddgx_batch_create(op.handle, None if pc is None else pc.handle,
+                                                self.nseg, _stream(), C.byref(h)),
+                   "ddgx_batch_create")
+        self._h = h
+        nb = C.c_int64()
+        _lib.check(_lib.lib().ddgx_batch_bytes(h, C.byref(nb)), "ddgx_batch_bytes")
+        self.nbytes = int(nb.value)
+
+    def pcg(self, b, alpha, x=None, *, tol=1e-9, atol=0.0, maxiter=1000, check_every=16,
+            stream=None):
+        """Solve ``(A + alpha[s] M) x_s = b_s`` for every segment s.  ``b``, ``x``:
+        contiguous float64 device tensors of ``nseg * K * Np`` (segment s at
+        offset ``s * K * Np``); returns ``(x, [info_dict per segment])``."""
+        if not (isinstance(b, torch.Tensor) and b.is_cuda and b.dtype == torch.float64
+                and b.is_contiguous() and b.numel() == self.nseg * self.n):
+            raise ValueError(f"b must be contiguous float64 on the device with "
+                             f"{self.nseg} x {self.n} entries")
+        x = torch.zeros_like(b) if x is None else x
+        if not (x.is_cuda and x.dtype == torch.float64 and x.is_contiguous()
+                and x.numel() == b.numel()):
+            raise ValueError("x must match b")
+        a = [float(v) for v in alpha]
+        if len(a) != self.nseg:
+            raise ValueError(f"need {self.nseg} Helmholtz coefficients")
+        av = (C.c_double * self.nseg)(*a)
+        cfg = _lib.CGConfig(float(tol), float(atol), int(maxiter), int(check_every))
+        infos = (_lib.CGInfo * self.nseg)()
+        st = _stream() if stream is None else C.c_void_p(stream)
+        _lib.check(_lib.lib().ddgx_batch_pcg(self._h, av, _ptr(b), _ptr(x), C.byref(cfg),
+                                             infos, st), "ddgx_batch_pcg")
+        return x, [{"iterations": i.iterations, "converged": bool(i.converged),
+                    "residual2": i.residual2, "atol2": i.atol2} for i in infos]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.lib().ddgx_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def refine_solve(op64: SipgOperator, op32: SipgOperator, b, x=None, *, alpha=0.0, shift=0.0,
                  pc32=None, tol=1e-10, atol=0.0, max_outer=20, inner_tol=1e-4,
                  inner_maxiter=1000, check_every=16):

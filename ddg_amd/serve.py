@@ -17,12 +17,19 @@ a fixed set of worker slots that run solves side by side on the device:
   and the footprint fits the HBM budget (:class:`Admission`); requests that
   arrive while others are on the GPU join immediately, so their kernels
   overlap on the device (each slot has its own HIP stream).
-* Each solve is the mixed-precision refinement (float32 MG-PCG inside a
-  float64 residual loop, ``ddgx_solver_ir``) to a relative residual ``tol``.
+* Requests of one class that have arrived by the time a slot frees up run
+  as one batch in that slot (up to ``max_batch``): one kernel launch per
+  step serves all of them (``ddgx_batch``, :class:`ddg_amd.operator.
+  BatchSolver`), so small problems stop paying one launch latency each.
+* Each solve is float64 CG preconditioned by the class's float32 V-cycle
+  ("pcg32", batched), or mixed-precision refinement ("refine": float32
+  MG-PCG steps in a float64 residual loop, ``ddgx_solver_ir``, one request
+  per slot), to a relative residual ``tol``.
 
 A solve reads only its own vectors and the read-only class data, and every
-reduction has a fixed order, so a request's result does not depend on what
-else is in flight: alone and in a mix it is bitwise the same.
+reduction has a fixed order per request, so a request's result does not
+depend on what else is in flight, nor on the batch it runs in: alone and in
+a mix it is bitwise the same.
 
 The scheduling logic (:class:`Admission`, :func:`poisson_arrivals`) is pure
 Python and needs no GPU.
@@ -139,32 +146,63 @@ class Admission:
             self.used -= nbytes
 
 
-def dispatch(requests, admission: Admission, footprint, start_fn, wait_fn, clock):
+def dispatch_groups(requests, admission: Admission, footprint, start_fn, wait_fn, clock,
+                    key=None, max_group=1):
     """Admission loop shared by the GPU server and its CPU tests.
 
-    Requests (sorted by arrival) are admitted first-come first-served as
-    soon as ``clock() >= arrival``, a slot is idle and ``footprint(req)``
-    bytes fit.  ``start_fn(req, nbytes, t_admit)`` hands a request to a
-    worker (which calls ``admission.release(nbytes)`` when done);
+    The earliest arrived request not yet started (the head) is admitted as
+    soon as ``clock() >= arrival``, a slot is idle and its group's bytes fit:
+    the group is the head and, up to ``max_group`` in all, the next arrived
+    requests with the head's ``key(req)``, in arrival order (a group that
+    would not fit the budget is cut down); ``max_group`` is a number or a
+    function of the key.  ``footprint(group)`` gives the
+    bytes, ``start_fn(group, nbytes, t_admit)`` hands the group to a worker
+    (which calls ``admission.release(nbytes)`` when done), and
     ``wait_fn(timeout)`` blocks until a worker finishes or ``timeout``
     seconds pass.  Returns when every request has been started."""
-    pending = deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
-    for r in pending:
-        if not admission.fits_ever(footprint(r)):
-            raise ValueError(f"request {r.rid} needs {footprint(r)} bytes, more than the "
+    key = (lambda r: r.rid) if key is None else key
+    future = deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
+    for r in future:
+        if not admission.fits_ever(footprint([r])):
+            raise ValueError(f"request {r.rid} needs {footprint([r])} bytes, more than the "
                              f"HBM budget of {admission.budget}")
-    while pending:
+    arrived = deque()   # arrived and not started, in arrival order
+    by_key = {}         # key -> the same, per key
+    started = set()
+    while future or arrived:
         now = clock()
-        head = pending[0]
-        if head.arrival > now:
-            wait_fn(head.arrival - now)
+        while future and future[0].arrival <= now:
+            r = future.popleft()
+            arrived.append(r)
+            by_key.setdefault(key(r), deque()).append(r)
+        while arrived and arrived[0].rid in started:
+            arrived.popleft()
+        if not arrived:
+            if not future:
+                break
+            wait_fn(future[0].arrival - now)
             continue
-        nb = footprint(head)
+        k = key(arrived[0])
+        q = by_key[k]      # q[0] is the head
+        mg = max_group(k) if callable(max_group) else max_group
+        group = [q[i] for i in range(min(max(1, mg), len(q)))]
+        while len(group) > 1 and not admission.fits_ever(footprint(group)):
+            group.pop()
+        nb = footprint(group)
         if admission.try_admit(nb):
-            pending.popleft()
-            start_fn(head, nb, now)
+            for r in group:
+                q.popleft()
+                started.add(r.rid)
+            start_fn(group, nb, now)
         else:
             wait_fn(0.05)
+
+
+def dispatch(requests, admission: Admission, footprint, start_fn, wait_fn, clock):
+    """:func:`dispatch_groups` one request at a time: ``footprint(req)``,
+    ``start_fn(req, nbytes, t_admit)``."""
+    dispatch_groups(requests, admission, lambda g: footprint(g[0]),
+                    lambda g, nb, t: start_fn(g[0], nb, t), wait_fn, clock)
 
 
 # ---------------------------------------------------------------------------
@@ -209,7 +247,8 @@ class ProblemClass:
     """Everything of an (n, N) class that requests share: the forest-ordered
     2 n^2 mesh with homogeneous Dirichlet walls, float64/float32 operators,
     a float32 hp-multigrid (h-levels down to 2 x n_base^2), node coordinates
-    on the device, and a pool of idle solver contexts."""
+    on the device, and pools of idle solve contexts: batches of each size
+    (method "pcg32") or solver contexts (method "refine")."""
 
     def __init__(self, n, N, n_base=16, tau_scale=200.0, method="pcg32"):
         import torch
@@ -237,32 +276,58 @@ class ProblemClass:
         dev = self.op64.device
         self.x = torch.as_tensor(np.ascontiguousarray(self.mesh.x.T).reshape(-1), device=dev)
         self.y = torch.as_tensor(np.ascontiguousarray(self.mesh.y.T).reshape(-1), device=dev)
-        self._idle = []
+        self._idle = []      # refinement solver contexts
+        self._batches = {}   # batch size -> idle BatchSolvers
         self._lock = threading.Lock()
         self._all = []
-        probe = self._new_solver()
+        if method == "pcg32":
+            probe = self._new_batch(1)
+            self._batches[1] = [probe]
+        else:
+            probe = self._new_solver()
+            self._idle.append(probe)
         self.solver_bytes = probe.nbytes
-        self._idle.append(probe)
-        # per request: the solver context plus f, b and x (float64)
+        # per request: its solve context (a batch's device copies of the
+        # operator and multigrid and its Krylov state grow by this much per
+        # request) plus f, b and x (float64)
         self.footprint = self.solver_bytes + 3 * self.K * self.Np * 8
 
+    def group_footprint(self, count):
+        return count * self.footprint
+
     def _new_solver(self):
-        if self.method == "pcg32":
-            s = SolverContext(self.op64, None, self.pc)
-        else:
-            s = SolverContext(self.op64, self.op32, self.pc)
-        self._all.append(s)
+        s = SolverContext(self.op64, self.op32, self.pc)
+        with self._lock:
+            self._all.append(s)
         return s
+
+    def _new_batch(self, count):
+        from .operator import BatchSolver
+        b = BatchSolver(self.op64, self.pc, count)
+        with self._lock:
+            self._all.append(b)
+        return b
 
     def acquire(self):
         with self._lock:
             if self._idle:
                 return self._idle.pop()
-            return self._new_solver()
+        return self._new_solver()
 
     def release(self, s):
         with self._lock:
             self._idle.append(s)
+
+    def acquire_batch(self, count):
+        with self._lock:
+            pool = self._batches.setdefault(count, [])
+            if pool:
+                return pool.pop()
+        return self._new_batch(count)
+
+    def release_batch(self, b):
+        with self._lock:
+            self._batches.setdefault(b.nseg, []).append(b)
 
     def close(self):
         for s in self._all:
@@ -347,6 +412,13 @@ class ServeConfig:
     # stream's closed-loop capacity is about 71/s with 4 slots, 87/s with
     # 8 and 84/s with 12
     slots: int = 8
+    # "pcg32": requests of one class that have arrived when a slot frees up
+    # run together in that slot, up to max_batch and while the batch holds at
+    # most batch_dofs unknowns (a batch of small problems costs about what
+    # one does -- each launch is latency-bound -- but a batch of large ones
+    # costs the sum of their bandwidth; a class above batch_dofs runs alone)
+    max_batch: int = 8
+    batch_dofs: int = 1 << 19
     hbm_budget: int = 64 << 30      # bytes of per-request state in flight
     tol: float = 1e-10              # outer (float64) relative residual
     inner_tol: float = 1e-4         # float32 MG-PCG relative residual per step
@@ -363,6 +435,7 @@ class ServeReport:
     tol: float = 1e-10
     peak_slots: int = 0
     peak_bytes: int = 0
+    batch_sizes: list = field(default_factory=list)
 
     def stats(self):
         lat = np.array([r.latency for r in self.results])
@@ -379,7 +452,11 @@ class ServeReport:
                 "max_rel_residual": float(max(np.sqrt(r.residual2 / r.atol2)
                                               for r in self.results)) * self.tol,
                 "peak_slots_busy": self.peak_slots,
-                "peak_request_bytes": self.peak_bytes}
+                "peak_request_bytes": self.peak_bytes,
+                "batches": len(self.batch_sizes),
+                "mean_batch": round(float(np.mean(self.batch_sizes)), 2)
+                if self.batch_sizes else 0.0,
+                "max_batch": int(max(self.batch_sizes)) if self.batch_sizes else 0}
 
 
 class SolveServer:
@@ -411,13 +488,42 @@ class SolveServer:
         return _lib.IRConfig(c.tol, 0.0, c.max_outer, c.inner_tol, c.inner_maxiter,
                              c.check_every)
 
-    def _cg_cfg(self):
+    def _max_group(self, key):
+        """Largest batch of class ``key = (n, N)``."""
+        pc = self.problem_class(*key)
         c = self.cfg
-        return _lib.CGConfig(c.tol, 0.0, c.inner_maxiter * c.max_outer, c.check_every)
+        return max(1, min(c.max_batch, c.batch_dofs // (pc.K * pc.Np)))
+
+    def solve_group(self, reqs, stream):
+        """Run requests of one class as one batch on ``stream`` (blocking;
+        method "pcg32"); returns [(x, info)] in the order of ``reqs``."""
+        import torch
+        from types import SimpleNamespace
+        pc = self.problem_class(reqs[0].n, reqs[0].N)
+        if any((r.n, r.N) != (pc.n, pc.N) for r in reqs):
+            raise ValueError("a batch holds requests of one problem class")
+        n1 = pc.K * pc.Np
+        bs = pc.acquire_batch(len(reqs))
+        try:
+            with torch.cuda.stream(stream):
+                b = torch.empty(len(reqs) * n1, dtype=torch.float64, device=pc.op64.device)
+                for i, r in enumerate(reqs):
+                    f = fourier_field(pc.x, pc.y, r.seed)
+                    pc.op64.rhs(f, out=b[i * n1:(i + 1) * n1])
+                c = self.cfg
+                x, infos = bs.pcg(b, [r.k2 for r in reqs], tol=c.tol,
+                                  maxiter=c.inner_maxiter * c.max_outer,
+                                  check_every=c.check_every, stream=stream.cuda_stream)
+        finally:
+            pc.release_batch(bs)
+        return [(x[i * n1:(i + 1) * n1], _CGAsIR(SimpleNamespace(**infos[i])))
+                for i in range(len(reqs))]
 
     def solve_one(self, req: Request, stream):
         """Run one request on ``stream`` (blocking); returns (x, info)."""
         import torch
+        if self.cfg.method == "pcg32":
+            return self.solve_group([req], stream)[0]
         pc = self.problem_class(req.n, req.N)
         ctx = pc.acquire()
         try:
@@ -425,10 +531,7 @@ class SolveServer:
                 f = fourier_field(pc.x, pc.y, req.seed)
                 b = pc.op64.rhs(f)
                 x = torch.zeros_like(b)
-                if pc.method == "pcg32":
-                    info = _CGAsIR(ctx.pcg(b, x, req.k2, self._cg_cfg(), stream.cuda_stream))
-                else:
-                    info = ctx.solve(b, x, req.k2, self._ir_cfg(), stream.cuda_stream)
+                info = ctx.solve(b, x, req.k2, self._ir_cfg(), stream.cuda_stream)
         finally:
             pc.release(ctx)
         return x, info
@@ -442,6 +545,7 @@ class SolveServer:
         self.prepare(requests)
         torch.cuda.synchronize()
         cfg = self.cfg
+        batched = cfg.method == "pcg32"
         adm = Admission(cfg.slots, cfg.hbm_budget)
         report = ServeReport(tol=cfg.tol)
         work = queue.Queue()
@@ -459,21 +563,28 @@ class SolveServer:
                 item = work.get()
                 if item is None:
                     return
-                req, nb, t_admit = item
-                res = None
+                group, nb, t_admit = item
+                res = []
                 try:
-                    x, info = self.solve_one(req, self.streams[slot])
-                    res = Result(req.rid, req.arrival, t_admit, clock(), info.outer,
-                                 info.inner_total, bool(info.converged), info.residual2,
-                                 info.atol2, bool(info.at_floor),
-                                 x if cfg.keep_solutions else None)
+                    if batched:
+                        outs = self.solve_group(group, self.streams[slot])
+                    else:
+                        outs = [self.solve_one(group[0], self.streams[slot])]
+                    t_end = clock()
+                    for req, (x, info) in zip(group, outs):
+                        res.append(Result(req.rid, req.arrival, t_admit, t_end, info.outer,
+                                          info.inner_total, bool(info.converged),
+                                          info.residual2, info.atol2, bool(info.at_floor),
+                                          x if cfg.keep_solutions else None))
                 except Exception as e:  # surfaced after the loop
                     errors.append(e)
                 with cond:
-                    if res is not None:
-                        report.results.append(res)
+                    if res:
+                        before = len(report.results)
+                        report.results.extend(res)
+                        report.batch_sizes.append(len(res))
                         done = len(report.results)
-                        if progress_every > 0 and done % progress_every == 0:
+                        if progress_every > 0 and done // progress_every > before // progress_every:
                             print(f"[serve] {done}/{len(requests)} done at {clock():.1f} s",
                                   file=sys.stderr, flush=True)
                     adm.release(nb)
@@ -484,16 +595,19 @@ class SolveServer:
         for th in threads:
             th.start()
 
-        def start(req, nb, t_admit):
-            work.put((req, nb, t_admit))
+        def start(group, nb, t_admit):
+            work.put((group, nb, t_admit))
 
         def wait(timeout):
             with cond:
                 cond.wait(timeout=max(0.0, min(timeout, 0.05)))
 
         try:
-            dispatch(requests, adm, lambda r: self.problem_class(r.n, r.N).footprint,
-                     start, wait, clock)
+            dispatch_groups(
+                requests, adm,
+                lambda g: self.problem_class(g[0].n, g[0].N).group_footprint(len(g)),
+                start, wait, clock, key=lambda r: (r.n, r.N),
+                max_group=self._max_group if batched else 1)
         finally:
             for _ in threads:
                 work.put(None)

@@ -272,6 +272,32 @@ int ddgx_solver_ir(ddgx_solver solver, double alpha, double shift, const void* b
                    const ddgx_ir_cfg* cfg, ddgx_ir_info* info, void* stream);
 int ddgx_solver_destroy(ddgx_solver solver);
 
+/* Batched solves: nseg requests of one problem class (one mesh, order and
+ * preconditioner; each request its own right-hand side and Helmholtz
+ * coefficient) advanced together, one launch per kernel for all of them.
+ * Like the solver contexts this has no counterpart in the reference (each
+ * solve_sipg_helmholtz_2d call, sipg.py:536-612, solves one problem); it is
+ * the batched launch of the request loop in ddg_amd/serve.py.
+ *   ddgx_batch_create: op float64 (affine, a multiple of 64 elements), pc a
+ *     multigrid of op (float32 or float64) or NULL.  The batch holds device
+ *     copies of both for nseg (1..64) problems; op and pc may be destroyed
+ *     afterwards.
+ *   ddgx_batch_pcg: alpha[nseg] (host) Helmholtz coefficients, b and x
+ *     device float64 vectors of nseg * K * Np, request s at offset s*K*Np; x
+ *     holds the initial guesses.  info[nseg] receives each request's CG
+ *     information.  Each request follows ddgx_pcg_solve's iteration with the
+ *     same stopping rule and the same arithmetic: its x is bitwise the x that
+ *     ddgx_pcg_solve (op, alpha[s], the same pc and cfg) returns for it alone,
+ *     whatever the other requests of the batch.  On a non-NULL stream chunks
+ *     of check_every iterations replay as HIP graphs (check_every raised to
+ *     even).  One solve at a time per batch. */
+typedef struct ddgx_batch_s* ddgx_batch;
+int ddgx_batch_create(ddgx_op op, ddgx_pc pc, int nseg, void* stream, ddgx_batch* out);
+int ddgx_batch_bytes(ddgx_batch batch, int64_t* bytes);
+int ddgx_batch_pcg(ddgx_batch batch, const double* alpha, const void* b, void* x,
+                   const ddgx_cg_cfg* cfg, ddgx_cg_info* info, void* stream);
+int ddgx_batch_destroy(ddgx_batch batch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,8 +1,9 @@
 """Request loop on the GPU, with both solve methods (iterative refinement and
-float64 CG with the float32 V-cycle): every result is bitwise the same alone
-and in a mix of concurrent requests (stated tolerance: 0), served results
-meet the float64 tolerance against the oracle's dense solve, and admission
-keeps the slot and HBM limits."""
+float64 CG with the float32 V-cycle, the latter in batches of one class):
+every result is bitwise the same alone and in a mix of concurrent, batched
+requests (stated tolerance: 0), served results meet the float64 tolerance
+against the oracle's dense solve, and admission keeps the slot and HBM
+limits."""
 import numpy as np
 import pytest
 import torch
@@ -112,6 +113,8 @@ def test_pcg32_method_alone_equals_mix_and_matches_oracle(dev):
         poisson_arrivals(reqs, None)
         rep = srv.run(reqs)
         assert len(rep.results) == len(reqs) and rep.peak_slots > 1
+        # requests of one class that arrived together ran as batches
+        assert rep.stats()["max_batch"] > 1, rep.batch_sizes
         for res in rep.results:
             assert res.converged
             assert torch.equal(res.x, alone[res.rid]), f"request {res.rid} differs in the mix"

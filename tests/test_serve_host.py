@@ -101,3 +101,64 @@ def test_poisson_arrivals_and_request_mix():
     again = make_requests(50, seed=3)[7]                    # seeded
     assert (again.n, again.N, again.k2, again.seed) == (reqs[7].n, reqs[7].N, reqs[7].k2,
                                                         reqs[7].seed)
+
+
+def _run_groups(requests, slots, budget, footprint, service, max_group):
+    from ddg_amd.serve import dispatch_groups
+    adm = Admission(slots, budget)
+    cond = threading.Condition()
+    log = []          # (group rids, t_admit)
+    t0 = time.perf_counter()
+
+    def clock():
+        return time.perf_counter() - t0
+
+    def start(group, nb, t_admit):
+        with cond:
+            log.append(([r.rid for r in group], t_admit))
+
+        def work():
+            time.sleep(service(group))
+            with cond:
+                adm.release(nb)
+                cond.notify_all()
+        threading.Thread(target=work, daemon=True).start()
+
+    def wait(timeout):
+        with cond:
+            cond.wait(timeout=max(0.0, min(timeout, 0.05)))
+
+    dispatch_groups(requests, adm, footprint, start, wait, clock, key=lambda r: (r.n, r.N),
+                    max_group=max_group)
+    deadline = time.time() + 10
+    while adm.busy and time.time() < deadline:
+        time.sleep(0.005)
+    return adm, log
+
+
+def test_groups_take_arrived_requests_of_the_head_class():
+    # all arrive at once: the head's class is grouped up to max_group, in
+    # arrival order; heads go first come, first served
+    classes = [(16, 2), (32, 3), (16, 2), (16, 2), (32, 3), (16, 2), (16, 2), (64, 4)]
+    reqs = [Request(i, n, N, 0.0, i, arrival=0.0) for i, (n, N) in enumerate(classes)]
+    _, log = _run_groups(reqs, slots=1, budget=1 << 30, footprint=lambda g: len(g),
+                         service=lambda g: 0.005, max_group=3)
+    assert [g for g, _ in log] == [[0, 2, 3], [1, 4], [5, 6], [7]]
+
+
+def test_groups_never_hold_requests_not_yet_arrived():
+    reqs = [Request(0, 16, 2, 0.0, 0, arrival=0.0), Request(1, 16, 2, 0.0, 1, arrival=0.2),
+            Request(2, 16, 2, 0.0, 2, arrival=0.0)]
+    _, log = _run_groups(reqs, slots=2, budget=1 << 30, footprint=lambda g: len(g),
+                         service=lambda g: 0.01, max_group=8)
+    assert [g for g, _ in log] == [[0, 2], [1]]
+    assert log[1][1] >= 0.2
+
+
+def test_groups_cut_to_the_budget():
+    reqs = [Request(i, 16, 2, 0.0, i, arrival=0.0) for i in range(7)]
+    adm, log = _run_groups(reqs, slots=2, budget=3, footprint=lambda g: len(g),
+                           service=lambda g: 0.01, max_group=8)
+    assert all(len(g) <= 3 for g, _ in log)
+    assert sorted(r for g, _ in log for r in g) == list(range(7))
+    assert adm.peak_used <= 3

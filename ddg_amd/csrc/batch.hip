@@ -166,7 +166,8 @@ int mg_replicate(const MG& src, int nseg, hipStream_t s, MG** out) {
       L.tkind = S.tkind;
       L.np_lo = S.np_lo;
       if (S.binv != nullptr) {
-        const size_t nb = (size_t)S.op->K * packed_block_words(S.op->N) * ds;
+        L.full = S.full;
+        const size_t nb = (size_t)S.op->K * block_words(S.op->N, S.full) * ds;
         hipError_t e = hipMalloc(&L.binv, nb * nseg);
         if (e != hipSuccess) { rc = hip_fail(e, "batch multigrid blocks"); break; }
         rc = copy_rep(L.binv, S.binv, nb, nseg, s);
@@ -287,7 +288,7 @@ int batch_create(const Op& op, const PC* pc, int nseg, hipStream_t s, Batch** ou
     if (b->pc) {
       const MG& m = *b->pc->mg;
       for (int l = 0; l < m.nlev; ++l)
-        b->bytes += (size_t)m.lv[l].op->K * (packed_block_words(m.lv[l].op->N) +
+        b->bytes += (size_t)m.lv[l].op->K * (block_words(m.lv[l].op->N, m.lv[l].full) +
                                              DDGX_REC_WORDS + 4) * dsz_b(m.dtype);
       b->bytes += m.ws->bytes + (size_t)m.nc * m.nc * 8;
     }

@@ -9,7 +9,10 @@
 // inverted in float64 by Gauss-Jordan elimination (SPD blocks, no pivoting),
 // one wavefront per element with the augmented block in LDS, and stored as
 // packed symmetric upper triangles in the precision of the vectors they
-// will multiply.
+// will multiply.  Blocks of a general (curved-element) operator come from
+// an actual colour probe instead (launch_block_probe_general).
+
+#include <vector>
 
 #include "ddgx_internal.h"
 
@@ -107,7 +110,86 @@ int launch_block_probe(const Op& op, double* blocks, double alpha, hipStream_t s
   return DDGX_ERR_UNSUPPORTED;
 }
 
-template <typename T, int N>
+// The general (curved-element) operator has per-node metrics, so its blocks
+// come from the reference's colour probe itself (sipg.py:651-704): elements
+// coloured so that face neighbours differ, e_j in every element of colour c,
+// one apply, and column j of each such element's block read off its own
+// output rows (a neighbour of another colour contributes nothing there).
+__global__ void __launch_bounds__(256)
+probe_vector_kernel(double* __restrict__ u, const int* __restrict__ colour, int c, int j,
+                    int64_t K, int Np) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= K * Np) return;
+  const int64_t k = i / Np;
+  u[i] = (colour[k] == c && i - k * Np == j) ? 1.0 : 0.0;
+}
+
+__global__ void __launch_bounds__(256)
+probe_gather_kernel(const double* __restrict__ y, const int* __restrict__ colour, int c, int j,
+                    double* __restrict__ blocks, int64_t K, int Np) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= K * Np) return;
+  const int64_t k = i / Np;
+  if (colour[k] == c) blocks[k * Np * Np + (i - k * Np) * Np + j] = y[i];
+}
+
+int launch_block_probe_general(const Op& op, double* blocks, double alpha, hipStream_t s) {
+  if (op.dtype != DDGX_F64 || op.cv == nullptr) {
+    set_error("the colour probe takes a float64 general operator");
+    return DDGX_ERR_INVALID;
+  }
+  const int64_t K = op.K;
+  const int Np = op.Np;
+  std::vector<int> codes((size_t)K * 3), colour((size_t)K, -1);
+  DDGX_HIP(hipMemcpyAsync(codes.data(), op.cv->codes, codes.size() * sizeof(int),
+                          hipMemcpyDeviceToHost, s));
+  DDGX_HIP(hipStreamSynchronize(s));
+  // greedy colouring in element order: the smallest colour no coloured face
+  // neighbour has (at most 4 for triangles)
+  int ncol = 0;
+  for (int64_t k = 0; k < K; ++k) {
+    unsigned used = 0;
+    for (int f = 0; f < 3; ++f) {
+      const int c = codes[(size_t)k * 3 + f];
+      if (c >= 0 && colour[nbr_elem(c)] >= 0) used |= 1u << colour[nbr_elem(c)];
+    }
+    int cc = 0;
+    while (used & (1u << cc)) ++cc;
+    colour[k] = cc;
+    ncol = cc + 1 > ncol ? cc + 1 : ncol;
+  }
+  double *u = nullptr, *y = nullptr;
+  int* dcol = nullptr;
+  int rc = DDGX_OK;
+  do {
+    hipError_t e = hipMalloc((void**)&u, (size_t)K * Np * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void**)&y, (size_t)K * Np * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void**)&dcol, (size_t)K * sizeof(int));
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(dcol, colour.data(), (size_t)K * sizeof(int), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) { rc = hip_fail(e, "colour probe workspace"); break; }
+    const int g = (int)((K * Np + 255) / 256);
+    for (int c = 0; c < ncol && rc == DDGX_OK; ++c)
+      for (int j = 0; j < Np && rc == DDGX_OK; ++j) {
+        hipLaunchKernelGGL(probe_vector_kernel, dim3(g), dim3(256), 0, s, u,
+                           (const int*)dcol, c, j, K, Np);
+        rc = launch_apply(op, u, y, nullptr, alpha, 0.0, 0, nullptr, nullptr, s);
+        if (rc != DDGX_OK) break;
+        hipLaunchKernelGGL(probe_gather_kernel, dim3(g), dim3(256), 0, s, (const double*)y,
+                           (const int*)dcol, c, j, blocks, K, Np);
+      }
+    if (rc != DDGX_OK) break;
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) rc = hip_fail(e, "colour probe");
+  } while (false);
+  if (u) (void)hipFree(u);
+  if (y) (void)hipFree(y);
+  if (dcol) (void)hipFree(dcol);
+  return rc;
+}
+
+template <typename T, int N, bool FULL>
 __global__ void __launch_bounds__(64)
 invert_blocks_kernel(const double* __restrict__ blocks, T* __restrict__ binv, int K) {
   constexpr int Np = Shape<N>::Np;
@@ -137,28 +219,44 @@ invert_blocks_kernel(const double* __restrict__ blocks, T* __restrict__ binv, in
       }
       __syncthreads();
     }
-    // upper triangle of the symmetric part, packed (the inverse of a
-    // symmetric block, symmetric up to rounding)
-    T* out = binv + (size_t)k * SymPack<N>::NPK;
-    for (int idx = t; idx < Np * Np; idx += 64) {
-      const int i = idx / Np, j = idx % Np;
-      if (i <= j) out[SymPack<N>::pk(i, j)] = (T)(0.5 * (A[i][Np + j] + A[j][Np + i]));
+    if constexpr (FULL) {
+      // the whole inverse, row-major (a non-symmetric block)
+      T* out = binv + (size_t)k * Np * Np;
+      for (int idx = t; idx < Np * Np; idx += 64) {
+        const int i = idx / Np, j = idx % Np;
+        out[idx] = (T)A[i][Np + j];
+      }
+    } else {
+      // upper triangle of the symmetric part, packed (the inverse of a
+      // symmetric block, symmetric up to rounding)
+      T* out = binv + (size_t)k * SymPack<N>::NPK;
+      for (int idx = t; idx < Np * Np; idx += 64) {
+        const int i = idx / Np, j = idx % Np;
+        if (i <= j) out[SymPack<N>::pk(i, j)] = (T)(0.5 * (A[i][Np + j] + A[j][Np + i]));
+      }
     }
   }
 }
 
-template <int N>
-static int invert_t(int64_t K, int dtype, const double* blocks, void* binv, hipStream_t s) {
+template <int N, bool FULL>
+static int invert_tf(int64_t K, int dtype, const double* blocks, void* binv, hipStream_t s) {
   int grid = 16 * device_cu_count();
   if ((int64_t)grid > K) grid = (int)K;
   if (dtype == DDGX_F64)
-    hipLaunchKernelGGL((invert_blocks_kernel<double, N>), dim3(grid), dim3(64), 0, s, blocks,
-                       (double*)binv, (int)K);
+    hipLaunchKernelGGL((invert_blocks_kernel<double, N, FULL>), dim3(grid), dim3(64), 0, s,
+                       blocks, (double*)binv, (int)K);
   else
-    hipLaunchKernelGGL((invert_blocks_kernel<float, N>), dim3(grid), dim3(64), 0, s, blocks,
-                       (float*)binv, (int)K);
+    hipLaunchKernelGGL((invert_blocks_kernel<float, N, FULL>), dim3(grid), dim3(64), 0, s,
+                       blocks, (float*)binv, (int)K);
   DDGX_HIP(hipGetLastError());
   return DDGX_OK;
+}
+
+template <int N>
+static int invert_t(int64_t K, int dtype, const double* blocks, void* binv, hipStream_t s,
+                    bool full) {
+  return full ? invert_tf<N, true>(K, dtype, blocks, binv, s)
+              : invert_tf<N, false>(K, dtype, blocks, binv, s);
 }
 
 size_t packed_block_words(int N) {
@@ -166,15 +264,20 @@ size_t packed_block_words(int N) {
   return Np * (Np + 1) / 2;
 }
 
+size_t block_words(int N, bool full) {
+  const size_t Np = (size_t)(N + 1) * (N + 2) / 2;
+  return full ? Np * Np : packed_block_words(N);
+}
+
 int launch_invert_blocks(int N, int64_t K, int dtype, const double* blocks, void* binv,
-                         hipStream_t s) {
+                         hipStream_t s, bool full) {
   switch (N) {
-    case 1: return invert_t<1>(K, dtype, blocks, binv, s);
-    case 2: return invert_t<2>(K, dtype, blocks, binv, s);
-    case 3: return invert_t<3>(K, dtype, blocks, binv, s);
-    case 4: return invert_t<4>(K, dtype, blocks, binv, s);
-    case 5: return invert_t<5>(K, dtype, blocks, binv, s);
-    case 6: return invert_t<6>(K, dtype, blocks, binv, s);
+    case 1: return invert_t<1>(K, dtype, blocks, binv, s, full);
+    case 2: return invert_t<2>(K, dtype, blocks, binv, s, full);
+    case 3: return invert_t<3>(K, dtype, blocks, binv, s, full);
+    case 4: return invert_t<4>(K, dtype, blocks, binv, s, full);
+    case 5: return invert_t<5>(K, dtype, blocks, binv, s, full);
+    case 6: return invert_t<6>(K, dtype, blocks, binv, s, full);
   }
   set_error("polynomial order not built");
   return DDGX_ERR_UNSUPPORTED;

@@ -368,10 +368,6 @@ int ddgx_block_jacobi_create(ddgx_op op64, double alpha, int dtype, void* stream
     set_error("ddgx_block_jacobi_create: blocks are built from a float64 operator");
     return DDGX_ERR_INVALID;
   }
-  if (op64->op.cv != nullptr) {
-    set_error("ddgx_block_jacobi_create: the closed-form blocks need affine elements");
-    return DDGX_ERR_UNSUPPORTED;
-  }
   if (dtype != DDGX_F64 && dtype != DDGX_F32) {
     set_error("ddgx_block_jacobi_create: bad dtype");
     return DDGX_ERR_INVALID;
@@ -387,11 +383,13 @@ int ddgx_block_jacobi_create(ddgx_op op64, double alpha, int dtype, void* stream
   do {
     hipError_t e = hipMalloc((void**)&pc.blocks, nb * sizeof(double));
     if (e != hipSuccess) { rc = hip_fail(e, "hipMalloc(blocks)"); break; }
-    e = hipMalloc(&pc.binv, (size_t)o.K * packed_block_words(o.N) * (dtype == DDGX_F64 ? 8 : 4));
+    pc.full = o.cv != nullptr;  // a curved operator's blocks are not symmetric
+    e = hipMalloc(&pc.binv, (size_t)o.K * block_words(o.N, pc.full) * (dtype == DDGX_F64 ? 8 : 4));
     if (e != hipSuccess) { rc = hip_fail(e, "hipMalloc(binv)"); break; }
-    rc = launch_block_probe(o, pc.blocks, alpha, s);
+    rc = o.cv != nullptr ? launch_block_probe_general(o, pc.blocks, alpha, s)
+                         : launch_block_probe(o, pc.blocks, alpha, s);
     if (rc != DDGX_OK) break;
-    rc = launch_invert_blocks(o.N, o.K, dtype, pc.blocks, pc.binv, s);
+    rc = launch_invert_blocks(o.N, o.K, dtype, pc.blocks, pc.binv, s, pc.full);
     if (rc != DDGX_OK) break;
     e = hipStreamSynchronize(s);
     if (e != hipSuccess) { rc = hip_fail(e, "block-Jacobi build"); break; }
@@ -433,7 +431,8 @@ int ddgx_pc_apply(ddgx_pc pc, const void* r, void* z, void* stream) {
   const Op& o = *pc->pc.op;
   if (pc->pc.type == 1)
     return mg_vcycle(*pc->pc.mg, *pc->pc.mg->ws, r, z, nullptr, (hipStream_t)stream);
-  return launch_bj_apply(o.N, o.K, pc->pc.dtype, pc->pc.binv, r, z, (hipStream_t)stream);
+  return launch_bj_apply(o.N, o.K, pc->pc.dtype, pc->pc.binv, r, z, (hipStream_t)stream,
+                         pc->pc.full);
 }
 
 int ddgx_pc_destroy(ddgx_pc pc) {
@@ -465,10 +464,6 @@ int ddgx_mg_create(int n_levels, const ddgx_op* ops, const ddgx_op* ops64,
     }
     lv[l] = &ops[l]->op;
     lv64[l] = &ops64[l]->op;
-    if (lv[l]->cv != nullptr || lv64[l]->cv != nullptr) {
-      set_error("ddgx_mg_create: multigrid levels must be affine operators");
-      return DDGX_ERR_UNSUPPORTED;
-    }
   }
   MG* mg = nullptr;
   const int rc = mg_create(n_levels, lv.data(), lv64.data(), transfer_kind, transfer, *cfg,

@@ -285,7 +285,7 @@ update_kernel(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
 }
 
 // x += alpha p ; r -= alpha Ap ; z = Binv r (per element) ; partial <r,z>, <r,r>
-template <typename T, int N>
+template <typename T, int N, bool FULL>
 __global__ void __launch_bounds__(256)
 update_bj_kernel(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
                  const T* __restrict__ Ap, const T* __restrict__ binv, T* __restrict__ z,
@@ -295,7 +295,7 @@ update_bj_kernel(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
   constexpr int Np = S::Np, NL = S::NL, EPB = S::EPB;
   if (sc->done) return;
   __shared__ T s_r[EPB * Np];
-  __shared__ T s_b[EPB * SymPack<N>::NPK];
+  __shared__ T s_b[EPB * BlockWords<N, FULL>::value];
   __shared__ double s_red[4];
   const T alpha = cg_alpha<T>(apart, napart, sc, s_red, q);
   const int tid = threadIdx.x, e = tid / NL, li = tid % NL;
@@ -312,10 +312,10 @@ update_bj_kernel(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
       r[g] = ri;
       s_r[i] = ri;
     }
-    stage_blocks<T, N>(binv, s_b, k0, ne, tid);
+    stage_blocks<T, N, FULL>(binv, s_b, k0, ne, tid);
     __syncthreads();
     if (li < Np && e < ne) {
-      const T acc = block_row<T, N>(s_b, s_r, e, li);
+      const T acc = block_row<T, N, FULL>(s_b, s_r, e, li);
       z[(size_t)(k0 + e) * Np + li] = acc;
       const double ri = (double)s_r[e * Np + li];
       rz += ri * (double)acc;
@@ -331,13 +331,13 @@ update_bj_kernel(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
 }
 
 // standalone block-Jacobi apply z = Binv r
-template <typename T, int N>
+template <typename T, int N, bool FULL>
 __global__ void __launch_bounds__(256)
 bj_apply_kernel(const T* __restrict__ binv, const T* __restrict__ r, T* __restrict__ z, int K) {
   using S = Shape<N>;
   constexpr int Np = S::Np, NL = S::NL, EPB = S::EPB;
   __shared__ T s_r[EPB * Np];
-  __shared__ T s_b[EPB * SymPack<N>::NPK];
+  __shared__ T s_b[EPB * BlockWords<N, FULL>::value];
   const int tid = threadIdx.x, e = tid / NL, li = tid % NL;
   const int ntiles = (K + EPB - 1) / EPB;
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -345,9 +345,10 @@ bj_apply_kernel(const T* __restrict__ binv, const T* __restrict__ r, T* __restri
     const int ne = min(EPB, K - k0);
     __syncthreads();
     for (int i = tid; i < ne * Np; i += 256) s_r[i] = r[(size_t)k0 * Np + i];
-    stage_blocks<T, N>(binv, s_b, k0, ne, tid);
+    stage_blocks<T, N, FULL>(binv, s_b, k0, ne, tid);
     __syncthreads();
-    if (li < Np && e < ne) z[(size_t)(k0 + e) * Np + li] = block_row<T, N>(s_b, s_r, e, li);
+    if (li < Np && e < ne)
+      z[(size_t)(k0 + e) * Np + li] = block_row<T, N, FULL>(s_b, s_r, e, li);
   }
 }
 
@@ -476,31 +477,38 @@ pupdate_fin_kernel(T* __restrict__ p, const T* __restrict__ z, CGScalars* sc, si
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-template <int N>
-static int bj_apply_t(int64_t K, int dtype, const void* binv, const void* r, void* z,
-                      hipStream_t s) {
+template <int N, bool FULL>
+static int bj_apply_tf(int64_t K, int dtype, const void* binv, const void* r, void* z,
+                       hipStream_t s) {
   const int ntiles = (int)((K + Shape<N>::EPB - 1) / Shape<N>::EPB);
   int grid = 8 * device_cu_count();
   grid = ntiles < grid ? ntiles : grid;
   if (dtype == DDGX_F64)
-    hipLaunchKernelGGL((bj_apply_kernel<double, N>), dim3(grid), dim3(256), 0, s,
+    hipLaunchKernelGGL((bj_apply_kernel<double, N, FULL>), dim3(grid), dim3(256), 0, s,
                        (const double*)binv, (const double*)r, (double*)z, (int)K);
   else
-    hipLaunchKernelGGL((bj_apply_kernel<float, N>), dim3(grid), dim3(256), 0, s,
+    hipLaunchKernelGGL((bj_apply_kernel<float, N, FULL>), dim3(grid), dim3(256), 0, s,
                        (const float*)binv, (const float*)r, (float*)z, (int)K);
   DDGX_HIP(hipGetLastError());
   return DDGX_OK;
 }
 
+template <int N>
+static int bj_apply_t(int64_t K, int dtype, const void* binv, const void* r, void* z,
+                      hipStream_t s, bool full) {
+  return full ? bj_apply_tf<N, true>(K, dtype, binv, r, z, s)
+              : bj_apply_tf<N, false>(K, dtype, binv, r, z, s);
+}
+
 int launch_bj_apply(int N, int64_t K, int dtype, const void* binv, const void* r, void* z,
-                    hipStream_t s) {
+                    hipStream_t s, bool full) {
   switch (N) {
-    case 1: return bj_apply_t<1>(K, dtype, binv, r, z, s);
-    case 2: return bj_apply_t<2>(K, dtype, binv, r, z, s);
-    case 3: return bj_apply_t<3>(K, dtype, binv, r, z, s);
-    case 4: return bj_apply_t<4>(K, dtype, binv, r, z, s);
-    case 5: return bj_apply_t<5>(K, dtype, binv, r, z, s);
-    case 6: return bj_apply_t<6>(K, dtype, binv, r, z, s);
+    case 1: return bj_apply_t<1>(K, dtype, binv, r, z, s, full);
+    case 2: return bj_apply_t<2>(K, dtype, binv, r, z, s, full);
+    case 3: return bj_apply_t<3>(K, dtype, binv, r, z, s, full);
+    case 4: return bj_apply_t<4>(K, dtype, binv, r, z, s, full);
+    case 5: return bj_apply_t<5>(K, dtype, binv, r, z, s, full);
+    case 6: return bj_apply_t<6>(K, dtype, binv, r, z, s, full);
   }
   set_error("polynomial order not built");
   return DDGX_ERR_UNSUPPORTED;
@@ -598,9 +606,9 @@ static int finalize(Work& w, int nparts, int mode, hipStream_t s) {
   return DDGX_OK;
 }
 
-template <typename T, int N>
-static int update_bj(const Op& op, Work& w, void* x, const void* binv, int q, hipStream_t s) {
-  hipLaunchKernelGGL((update_bj_kernel<T, N>), dim3(op.grid_vec), dim3(256), 0, s, (T*)x,
+template <typename T, int N, bool FULL>
+static int update_bj_f(const Op& op, Work& w, void* x, const void* binv, int q, hipStream_t s) {
+  hipLaunchKernelGGL((update_bj_kernel<T, N, FULL>), dim3(op.grid_vec), dim3(256), 0, s, (T*)x,
                      (T*)w.r, (const T*)w.p, (const T*)w.Ap, (const T*)binv, (T*)w.z,
                      (const CGScalars*)w.scal, (int)op.K, w.partials,
                      (const double*)(w.partials + 2 * w.npart), (int)op.grid_apply, q);
@@ -608,16 +616,23 @@ static int update_bj(const Op& op, Work& w, void* x, const void* binv, int q, hi
   return DDGX_OK;
 }
 
+template <typename T, int N>
+static int update_bj(const Op& op, Work& w, void* x, const void* binv, int q, hipStream_t s,
+                     bool full) {
+  return full ? update_bj_f<T, N, true>(op, w, x, binv, q, s)
+              : update_bj_f<T, N, false>(op, w, x, binv, q, s);
+}
+
 template <typename T>
 static int update_bj_dispatch(const Op& op, Work& w, void* x, const void* binv, int q,
-                              hipStream_t s) {
+                              hipStream_t s, bool full) {
   switch (op.N) {
-    case 1: return update_bj<T, 1>(op, w, x, binv, q, s);
-    case 2: return update_bj<T, 2>(op, w, x, binv, q, s);
-    case 3: return update_bj<T, 3>(op, w, x, binv, q, s);
-    case 4: return update_bj<T, 4>(op, w, x, binv, q, s);
-    case 5: return update_bj<T, 5>(op, w, x, binv, q, s);
-    case 6: return update_bj<T, 6>(op, w, x, binv, q, s);
+    case 1: return update_bj<T, 1>(op, w, x, binv, q, s, full);
+    case 2: return update_bj<T, 2>(op, w, x, binv, q, s, full);
+    case 3: return update_bj<T, 3>(op, w, x, binv, q, s, full);
+    case 4: return update_bj<T, 4>(op, w, x, binv, q, s, full);
+    case 5: return update_bj<T, 5>(op, w, x, binv, q, s, full);
+    case 6: return update_bj<T, 6>(op, w, x, binv, q, s, full);
   }
   set_error("polynomial order not built");
   return DDGX_ERR_UNSUPPORTED;
@@ -718,7 +733,7 @@ static int pcg_t(const Op& op, Work& w, double alpha, double shift, const PC* pc
     CHECK(dot2<T>(op, w, w.r, w.r, w.r, w.r, nullptr, s));
     CHECK(finalize(w, op.grid_vec, F_SUM, s));
   }
-  if (bj) CHECK(launch_bj_apply(op.N, op.K, op.dtype, pc->binv, w.r, w.z, s));
+  if (bj) CHECK(launch_bj_apply(op.N, op.K, op.dtype, pc->binv, w.r, w.z, s, pc->full));
   else if (has_pc) CHECK(precondition(nullptr));
   hipLaunchKernelGGL((copy_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (const T*)z,
                      (T*)w.p, n);
@@ -738,7 +753,7 @@ static int pcg_t(const Op& op, Work& w, double alpha, double shift, const PC* pc
     CHECK(launch_apply(op, w.p, w.Ap, nullptr, alpha, shift, M_DOT, w.partials + 2 * w.npart,
                        done, s));
     if (bj) {
-      CHECK(update_bj_dispatch<T>(op, w, x, pc->binv, q, s));
+      CHECK(update_bj_dispatch<T>(op, w, x, pc->binv, q, s, pc->full));
     } else {
       // with a float32 multigrid the update also writes the scaled float32
       // residual, and the conversion back is fused with the dot products

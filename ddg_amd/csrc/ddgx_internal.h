@@ -52,24 +52,33 @@ struct SymPack {
   }
 };
 
+// Inverse blocks of a general (curved-element) operator are not symmetric
+// (its nodal volume form weights M_ref by the per-node Jacobian on one side,
+// sipg.py:188-189): they are stored in full, row-major, Np*Np words per
+// element (FULL = true) instead of packed.
+template <int N, bool FULL>
+struct BlockWords {
+  static constexpr int value = FULL ? Shape<N>::Np * Shape<N>::Np : SymPack<N>::NPK;
+};
+
 // Inverse blocks of the EPB elements of a tile into LDS (contiguous loads).
-template <typename T, int N>
+template <typename T, int N, bool FULL = false>
 __device__ __forceinline__ void stage_blocks(const T* __restrict__ binv, T* s_b, int k0, int ne,
                                              int tid) {
-  constexpr int NPK = SymPack<N>::NPK;
-  const T* src = binv + (size_t)k0 * NPK;
-  for (int i = tid; i < ne * NPK; i += 256) s_b[i] = src[i];
+  constexpr int NW = BlockWords<N, FULL>::value;
+  const T* src = binv + (size_t)k0 * NW;
+  for (int i = tid; i < ne * NW; i += 256) s_b[i] = src[i];
 }
 
-// Row li of B_e r_e from the staged packed block and the staged vector.
-template <typename T, int N>
+// Row li of B_e r_e from the staged block and the staged vector.
+template <typename T, int N, bool FULL = false>
 __device__ __forceinline__ T block_row(const T* s_b, const T* s_r, int e, int li) {
-  constexpr int Np = Shape<N>::Np, NPK = SymPack<N>::NPK;
-  const T* b = s_b + e * NPK;
+  constexpr int Np = Shape<N>::Np, NW = BlockWords<N, FULL>::value;
+  const T* b = s_b + e * NW;
   const T* v = s_r + e * Np;
   T acc = 0;
 #pragma unroll
-  for (int j = 0; j < Np; ++j) acc += b[SymPack<N>::pk(li, j)] * v[j];
+  for (int j = 0; j < Np; ++j) acc += b[FULL ? li * Np + j : SymPack<N>::pk(li, j)] * v[j];
   return acc;
 }
 
@@ -232,7 +241,9 @@ struct PC {
   int type = 0;              // 0 block-Jacobi, 1 multigrid V-cycle
   Op* op = nullptr;          // BJ: the float64 operator the blocks came from
   int dtype = 0;             // precision of binv and of the vectors
-  void* binv = nullptr;      // K x Np(Np+1)/2 packed symmetric inverse blocks (dtype)
+  void* binv = nullptr;      // K x Np(Np+1)/2 packed symmetric inverse blocks (dtype),
+                             // or K x Np x Np full ones (full: a curved operator's)
+  bool full = false;
   double* blocks = nullptr;  // K x Np x Np float64 blocks (kept for inspection)
   MG* mg = nullptr;          // type 1
   int64_t K = 0;             // vector shape the PC applies to
@@ -243,7 +254,9 @@ struct PC {
 // solved directly with a dense float64 inverse.
 struct MGLevel {
   Op* op = nullptr;        // operator at this level (MG dtype)
-  void* binv = nullptr;    // packed block-Jacobi inverse blocks of this level (MG dtype)
+  void* binv = nullptr;    // block-Jacobi inverse blocks of this level (MG dtype): packed,
+                           // or full for a curved level (full = true)
+  bool full = false;
   double alpha = 0.0;      // Helmholtz coefficient of this level's operator
   int tkind = 0;           // transfer to the next-coarser level: 0 = p, 1 = h
   int np_lo = 0;           // Np of the next-coarser level (p transfer)
@@ -322,11 +335,13 @@ int launch_curved(const Op& op, const void* u, void* y, const void* udir, double
 int launch_curved_rhs(const Op& op, const void* f, const void* udir, const void* q, void* rhs,
                       hipStream_t s);
 int launch_block_probe(const Op& op, double* blocks, double alpha, hipStream_t s);
+int launch_block_probe_general(const Op& op, double* blocks, double alpha, hipStream_t s);
 int launch_invert_blocks(int N, int64_t K, int dtype, const double* blocks, void* binv,
-                         hipStream_t s);
+                         hipStream_t s, bool full = false);
 int launch_bj_apply(int N, int64_t K, int dtype, const void* binv, const void* r, void* z,
-                    hipStream_t s);
+                    hipStream_t s, bool full = false);
 size_t packed_block_words(int N);
+size_t block_words(int N, bool full);  // per element: packed or full
 
 // Batched solves: nseg requests of one problem class in every launch.
 int op_replicate(const Op& src, int nseg, hipStream_t s, Op** out);

@@ -37,7 +37,7 @@ constexpr int MG_MAXIN = 2560;     // staged input values per workgroup pass
 constexpr int MG_MAX_COARSE = 8192;
 
 // x = (accumulate ? x : 0) + omega * Binv r, element by element
-template <typename T, int N>
+template <typename T, int N, bool FULL>
 __global__ void __launch_bounds__(256)
 bj_smooth_kernel(const T* __restrict__ binv, const T* __restrict__ r, T* __restrict__ x, int K,
                  T omega, int accumulate, const int* __restrict__ skip) {
@@ -45,7 +45,7 @@ bj_smooth_kernel(const T* __restrict__ binv, const T* __restrict__ r, T* __restr
   constexpr int Np = S::Np, NL = S::NL, EPB = S::EPB;
   if (skip != nullptr && *skip) return;
   __shared__ T s_r[EPB * Np];
-  __shared__ T s_b[EPB * SymPack<N>::NPK];
+  __shared__ T s_b[EPB * BlockWords<N, FULL>::value];
   const int tid = threadIdx.x, e = tid / NL, li = tid % NL;
   const int ntiles = (K + EPB - 1) / EPB;
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -53,10 +53,10 @@ bj_smooth_kernel(const T* __restrict__ binv, const T* __restrict__ r, T* __restr
     const int ne = min(EPB, K - k0);
     __syncthreads();
     for (int i = tid; i < ne * Np; i += 256) s_r[i] = r[(size_t)k0 * Np + i];
-    stage_blocks<T, N>(binv, s_b, k0, ne, tid);
+    stage_blocks<T, N, FULL>(binv, s_b, k0, ne, tid);
     __syncthreads();
     if (li < Np && e < ne) {
-      const T acc = block_row<T, N>(s_b, s_r, e, li);
+      const T acc = block_row<T, N, FULL>(s_b, s_r, e, li);
       const size_t g = (size_t)(k0 + e) * Np + li;
       x[g] = accumulate ? x[g] + omega * acc : omega * acc;
     }
@@ -269,8 +269,12 @@ static int bj_smooth_t(const MGLevel& L, const void* r, void* x, double omega, i
   constexpr int PER_CU = (sizeof(T) == 8 && N >= 4) ? 32 : 8;
   int grid = PER_CU * device_cu_count();
   grid = ntiles < grid ? ntiles : grid;
-  hipLaunchKernelGGL((bj_smooth_kernel<T, N>), dim3(grid), dim3(256), 0, s,
-                     (const T*)L.binv, (const T*)r, (T*)x, K, (T)omega, acc, skip);
+  if (L.full)
+    hipLaunchKernelGGL((bj_smooth_kernel<T, N, true>), dim3(grid), dim3(256), 0, s,
+                       (const T*)L.binv, (const T*)r, (T*)x, K, (T)omega, acc, skip);
+  else
+    hipLaunchKernelGGL((bj_smooth_kernel<T, N, false>), dim3(grid), dim3(256), 0, s,
+                       (const T*)L.binv, (const T*)r, (T*)x, K, (T)omega, acc, skip);
   DDGX_HIP(hipGetLastError());
   return DDGX_OK;
 }
@@ -606,11 +610,14 @@ int mg_create(int nlev, Op* const* ops, Op* const* ops64, const int* tkind,
       L.op = ops[l];
       L.alpha = cfg.alpha;
       if (l + 1 == nlev) break;
-      const size_t nb = (size_t)ops[l]->K * packed_block_words(ops[l]->N);
+      L.full = ops[l]->cv != nullptr;  // a curved level's blocks are not symmetric
+      const size_t nb = (size_t)ops[l]->K * block_words(ops[l]->N, L.full);
       hipError_t he = hipMalloc(&L.binv, nb * dsz(dtype));
       if (he != hipSuccess) { rc = hip_fail(he, "hipMalloc(level)"); break; }
-      rc = launch_block_probe(*ops64[l], blocks, cfg.alpha, s);
-      if (rc == DDGX_OK) rc = launch_invert_blocks(ops[l]->N, ops[l]->K, dtype, blocks, L.binv, s);
+      rc = ops64[l]->cv != nullptr ? launch_block_probe_general(*ops64[l], blocks, cfg.alpha, s)
+                                   : launch_block_probe(*ops64[l], blocks, cfg.alpha, s);
+      if (rc == DDGX_OK)
+        rc = launch_invert_blocks(ops[l]->N, ops[l]->K, dtype, blocks, L.binv, s, L.full);
       if (rc != DDGX_OK) break;
       he = hipStreamSynchronize(s);  // blocks is reused by the next level
       if (he != hipSuccess) { rc = hip_fail(he, "block-Jacobi build"); break; }

@@ -160,9 +160,11 @@ int apply_pipe_grid(int N, int dtype, int halo_cap, int64_t ntiles) {
 // Fused block-Jacobi sweep of the multigrid smoother: the apply's smoothing
 // mode keeps the residual on chip (registers at order 1, LDS rows on the
 // matrix cores), and one launch replaces the residual apply and the sweep.
-bool smooth_fused(const Op& op) { return op.N >= 1 && op.N <= 6; }
+// the fused sweeps are modes of the affine kernels; a general (curved)
+// level smooths with the unfused block-Jacobi sweeps
+bool smooth_fused(const Op& op) { return op.cv == nullptr && op.N >= 1 && op.N <= 6; }
 
-bool smooth_prolongs(const Op& op) { return op.N == 1; }
+bool smooth_prolongs(const Op& op) { return op.cv == nullptr && op.N == 1; }
 
 template <typename T, int N>
 static int smooth_n(const Op& op, const void* binv, double omega, double alpha, const void* x_in,

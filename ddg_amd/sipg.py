@@ -15,8 +15,10 @@ every call (sipg.py:412-413, 849-850).
 Differences from the reference, by design:
 * Curved meshes (per-node metrics) and ``cub=`` (a ``MeshCubature`` from
   ``ddg_amd.cubature`` or the reference's) run on the general operator
-  (``ddgx_op_create_curved``); block-Jacobi and multigrid preconditioners
-  need affine meshes there (NotImplementedError).
+  (``ddgx_op_create_curved``).  The preconditioner builders take such a
+  mesh as the reference's do: block-Jacobi blocks of the nodal form by a
+  colour probe of the general apply, p-multigrid with the curved finest
+  level and straight-sided lower orders (multigrid.py:253-294).
 * ``iterative=None`` (direct LU) assembles the dense matrix from device
   applies and factors it with torch on the device; like the reference it
   is only practical for small meshes.
@@ -330,7 +332,8 @@ def extract_block_diag_matfree(apply_flat, mesh):
     are float64 tensors on the HIP device; an ``apply_flat`` that answers
     with numpy arrays gets numpy blocks back, otherwise device tensors.
     (The device preconditioner builders use a closed form instead of
-    probing: ``make_sipg_block_jacobi_preconditioner``.)"""
+    probing on affine meshes, and the same colour probe on curved ones:
+    ``make_sipg_block_jacobi_preconditioner``.)"""
     Np, K = int(mesh.Np), int(mesh.K)
     colour, nc = _color_elements(mesh)
     dev = torch.device("cuda", torch.cuda.current_device())

@@ -42,6 +42,10 @@ curved_trace_kernel(const T* __restrict__ tab, const T* __restrict__ geo,
                     const int* __restrict__ skip) {
   if (skip != nullptr && *skip) return;
   constexpr int Np = CShape<N>::Np, Nfp = CShape<N>::Nfp;
+  // Dr and Ds in LDS for the workgroup's lifetime (each thread reads a row)
+  __shared__ T s_d[2 * Np * Np];
+  for (int i = threadIdx.x; i < 2 * Np * Np; i += 256) s_d[i] = tab[i];
+  __syncthreads();
   const int64_t nf = K * 3 * Nfp;
   const int64_t stride = (int64_t)gridDim.x * 256;
   for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < nf; t += stride) {
@@ -51,8 +55,8 @@ curved_trace_kernel(const T* __restrict__ tab, const T* __restrict__ geo,
     T ui = 0, ur = 0, us = 0;
     if (u != nullptr) {
       const T* uk = u + k * Np;
-      const T* dr = tab + i * Np;
-      const T* ds = tab + Np * Np + i * Np;
+      const T* dr = s_d + i * Np;
+      const T* ds = s_d + Np * Np + i * Np;
 #pragma unroll
       for (int j = 0; j < Np; ++j) {
         const T uj = uk[j];
@@ -181,12 +185,18 @@ curved_elem_kernel(const T* __restrict__ tab, const T* __restrict__ geo,
   __shared__ double s_red[4];
   T* sh = reinterpret_cast<T*>(smem_raw);
   const int L = epb * Np;
+  // per element of the group: u, face cotangents (3), volume products (2),
+  // pulled-back cotangents (2), and the per-node metrics (5); then the
+  // element matrices (cubature); then Dr, Ds, M for the workgroup's lifetime
   T *s_u = sh, *s_gv = sh + L, *s_gx = sh + 2 * L, *s_gy = sh + 3 * L;
   T *s_a = sh + 4 * L, *s_b = sh + 5 * L, *s_vr = sh + 6 * L, *s_vs = sh + 7 * L;
-  T *s_S = sh + 8 * L, *s_Q = s_S + epb * NN;
-  const T* Dr = tab;
-  const T* Ds = tab + Np * Np;
-  const T* M = tab + 2 * Np * Np;
+  T* s_geo = sh + 8 * L;
+  T *s_S = sh + 13 * L, *s_Q = s_S + epb * NN;
+  T* s_tab = s_S + (CUB ? 2 * epb * NN : 0);
+  for (int i = threadIdx.x; i < 3 * Np * Np; i += 256) s_tab[i] = tab[i];
+  const T* Dr = s_tab;
+  const T* Ds = s_tab + Np * Np;
+  const T* M = s_tab + 2 * Np * Np;
   const int64_t nf = K * 3 * Nfp;
   const int tid = threadIdx.x;
   const int64_t ngroups = (K + epb - 1) / epb;
@@ -202,6 +212,8 @@ curved_elem_kernel(const T* __restrict__ tab, const T* __restrict__ geo,
       s_gx[idx] = 0;
       s_gy[idx] = 0;
     }
+    // the group's metrics are contiguous (element-major): coalesced loads
+    for (int idx = tid; idx < ne * 5 * Np; idx += 256) s_geo[idx] = geo[k0 * 5 * Np + idx];
     if constexpr (CUB) {
       // the group's element matrices are contiguous: coalesced loads
       for (int idx = tid; idx < ne * NN; idx += 256) s_S[idx] = estiff[k0 * NN + idx];
@@ -242,7 +254,7 @@ curved_elem_kernel(const T* __restrict__ tab, const T* __restrict__ geo,
           ur += Dr[i * Np + j] * ue[j];
           us += Ds[i * Np + j] * ue[j];
         }
-        const T* gk = geo + (k0 + e) * 5 * Np;
+        const T* gk = s_geo + e * 5 * Np;
         const T J = gk[4 * Np + i];
         s_a[idx] = J * (gk[i] * ur + gk[Np + i] * us);
         s_b[idx] = J * (gk[2 * Np + i] * ur + gk[3 * Np + i] * us);
@@ -261,7 +273,7 @@ curved_elem_kernel(const T* __restrict__ tab, const T* __restrict__ geo,
           wy += M[i * Np + j] * be[j];
         }
       }
-      const T* gk = geo + (k0 + e) * 5 * Np;
+      const T* gk = s_geo + e * 5 * Np;
       s_vr[idx] = gk[i] * wx + gk[2 * Np + i] * wy;
       s_vs[idx] = gk[Np + i] * wx + gk[3 * Np + i] * wy;
     }
@@ -292,7 +304,7 @@ curved_elem_kernel(const T* __restrict__ tab, const T* __restrict__ geo,
         T mu = 0;
 #pragma unroll
         for (int j = 0; j < Np; ++j) mu += M[i * Np + j] * ue[j];
-        acc += alpha * geo[(k0 + e) * 5 * Np + 4 * Np + i] * mu;
+        acc += alpha * s_geo[e * 5 * Np + 4 * Np + i] * mu;
       }
       acc += shift * s_u[idx];
       const int64_t gi = k0 * Np + idx;
@@ -428,7 +440,7 @@ int curved_create(Op& op, int use_cub, const double* reftab, const double* geo,
   }
   // elements per workgroup of the element kernel: its LDS (elem_words per
   // element) within 4096 words
-  const int words = 8 * Np + (use_cub ? Np * (Np + 1) : 0);
+  const int words = 13 * Np + (use_cub ? Np * (Np + 1) : 0);
   int epb = 4096 / words;
   epb = epb < 1 ? 1 : (epb > 64 ? 64 : epb);
   cv->epb = epb;
@@ -462,7 +474,10 @@ static int curved_t(const Op& op, const void* u, void* y, const void* udir, doub
                      (const T*)c.reftab, (const T*)c.fnorm, (const T*)c.fpar, fm, c.codes,
                      (const T*)c.trace, (const T*)udir, (T*)c.cown, (T*)c.cnbr, K, skip);
   DDGX_HIP(hipGetLastError());
-  const size_t lds = (size_t)c.epb * (8 * op.Np + (cub ? op.Np * (op.Np + 1) : 0)) * sizeof(T);
+  // the element matrices are staged whenever the operator has them (CUB
+  // kernel), the tables always
+  const size_t lds = ((size_t)c.epb * (13 * op.Np + (cub ? op.Np * (op.Np + 1) : 0)) +
+                      3 * (size_t)op.Np * op.Np) * sizeof(T);
   if (cub)
     hipLaunchKernelGGL((curved_elem_kernel<T, N, true>), dim3(c.grid_elem), dim3(256), lds, s,
                        (const T*)c.reftab, (const T*)c.geo, (const T*)c.estiff,

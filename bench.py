@@ -154,6 +154,23 @@ def curved_block(n, N, steps, warmup):
     return out
 
 
+def copy_bandwidth(steps, warmup, mib=256):
+    """Measured HBM bandwidth of a plain device-to-device copy (read + write
+    bytes per second): what the box's memory sustains, beside the nominal
+    8 TB/s the roofline fractions use."""
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.randn(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+
+    class _Copy:
+        def apply(self, x, y):
+            y.copy_(x)
+    t, _ = time_apply(_Copy(), a, b, max(20, steps // 4), warmup)
+    del a, b
+    return 2 * n * 8 / t / 1e9
+
+
 def time_apply(op, u, y, steps, warmup, **kw):
     """Average device time of one apply.  The applies are captured once into
     a HIP graph and replayed, so host launch overhead is not in the timed
@@ -603,6 +620,8 @@ def main():
     t32, _ = time_apply(op32, u32, torch.empty_like(u32), args.steps, args.warmup)
     print(f"[bench] apply f32: {t32 * 1e6:.2f} us", file=sys.stderr, flush=True)
     del op, op32
+    copy = copy_bandwidth(args.steps, args.warmup)
+    print(f"[bench] device copy: {copy:.0f} GB/s", file=sys.stderr, flush=True)
 
     # --- solves on the same mesh (hp-multigrid, tol 1e-10) ---
     from ddg_amd import sipg as D
@@ -672,6 +691,10 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": read_traffic("sipg_apply_f64_n4_k131072_traffic.txt")},
         "cpu_baseline": cpu,
+        "hbm_copy": {"GB/s": round(copy, 1), "frac_of_peak": round(copy / HBM_PEAK_GBS, 4),
+                     "apply_f64_frac_of_copy": round(achieved / copy, 4),
+                     "what": "device-to-device copy of 256 MiB (float64), read + write bytes "
+                             "over the average time of graph-replayed copies"},
         "apply_f32": {"us": round(t32 * 1e6, 3), "GDOF/s": round(K * Np / t32 / 1e9, 2),
                       "frac": round(apply_bytes(K, Np, 4) / t32 / 1e9 / HBM_PEAK_GBS, 4)},
         "cg": {"preconditioner": f"hp-multigrid levels K={mg64.sizes} N={mg64.degrees}",

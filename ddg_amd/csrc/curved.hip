@@ -167,9 +167,15 @@ __device__ inline double block_sum_256c(double v, double* s_red) {
   return t;
 }
 
-// LDS per element: 8 vectors of Np (u, gv, gx, gy, a, b, vr, vs), then with
-// cubature the element's stiffness and mass matrices (symmetric, packed upper
-// triangles of NPK words each).
+// LDS per element of a group (curved_elem_words): u, the face cotangents
+// gv, gx, gy, the pulled-back cotangents vr, vs, the per-node metrics (rx,
+// sx, ry, sy, and J for the nodal form), and with the nodal form its volume
+// products a, b, with cubature the element's stiffness (and, for a
+// Helmholtz apply, mass) matrix as a packed upper triangle of NPK words;
+// then Dr, Ds, M for the workgroup's lifetime.
+__host__ __device__ constexpr int curved_elem_words(int Np, bool cub, bool mass) {
+  return cub ? 10 * Np + (Np * (Np + 1) / 2) * (mass ? 2 : 1) : 13 * Np;
+}
 
 template <typename T, int N, bool CUB>
 __global__ void __launch_bounds__(256)
@@ -185,14 +191,17 @@ curved_elem_kernel(const T* __restrict__ tab, const T* __restrict__ geo,
   __shared__ double s_red[4];
   T* sh = reinterpret_cast<T*>(smem_raw);
   const int L = epb * Np;
-  // per element of the group: u, face cotangents (3), volume products (2),
-  // pulled-back cotangents (2), and the per-node metrics (5); then the
-  // element matrices (cubature); then Dr, Ds, M for the workgroup's lifetime
+  const bool mass = alpha != T(0);
+  constexpr int GW = CUB ? 4 : 5;  // metric planes staged (J: nodal form only)
   T *s_u = sh, *s_gv = sh + L, *s_gx = sh + 2 * L, *s_gy = sh + 3 * L;
-  T *s_a = sh + 4 * L, *s_b = sh + 5 * L, *s_vr = sh + 6 * L, *s_vs = sh + 7 * L;
-  T* s_geo = sh + 8 * L;
-  T *s_S = sh + 13 * L, *s_Q = s_S + epb * NN;
-  T* s_tab = s_S + (CUB ? 2 * epb * NN : 0);
+  T* nxt = sh + 4 * L;
+  T *s_a = nxt, *s_b = nxt + L;  // nodal form only
+  if constexpr (!CUB) nxt += 2 * L;
+  T *s_vr = nxt, *s_vs = nxt + L;
+  T* s_geo = nxt + 2 * L;
+  T* s_S = s_geo + GW * L;
+  T* s_Q = s_S + (CUB ? epb * NN : 0);
+  T* s_tab = s_Q + (CUB && mass ? epb * NN : 0);
   for (int i = threadIdx.x; i < 3 * Np * Np; i += 256) s_tab[i] = tab[i];
   const T* Dr = s_tab;
   const T* Ds = s_tab + Np * Np;
@@ -200,7 +209,6 @@ curved_elem_kernel(const T* __restrict__ tab, const T* __restrict__ geo,
   const int64_t nf = K * 3 * Nfp;
   const int tid = threadIdx.x;
   const int64_t ngroups = (K + epb - 1) / epb;
-  const bool mass = alpha != T(0);
   double dacc = 0.0;
   for (int64_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
     const int64_t k0 = g * epb;
@@ -212,8 +220,12 @@ curved_elem_kernel(const T* __restrict__ tab, const T* __restrict__ geo,
       s_gx[idx] = 0;
       s_gy[idx] = 0;
     }
-    // the group's metrics are contiguous (element-major): coalesced loads
-    for (int idx = tid; idx < ne * 5 * Np; idx += 256) s_geo[idx] = geo[k0 * 5 * Np + idx];
+    // the group's metrics (the first GW planes of each element's five):
+    // contiguous runs, coalesced loads
+    for (int idx = tid; idx < ne * GW * Np; idx += 256) {
+      const int e = idx / (GW * Np), r = idx - e * (GW * Np);
+      s_geo[idx] = geo[(k0 + e) * 5 * Np + r];
+    }
     if constexpr (CUB) {
       // the group's element matrices are contiguous: coalesced loads
       for (int idx = tid; idx < ne * NN; idx += 256) s_S[idx] = estiff[k0 * NN + idx];
@@ -254,7 +266,7 @@ curved_elem_kernel(const T* __restrict__ tab, const T* __restrict__ geo,
           ur += Dr[i * Np + j] * ue[j];
           us += Ds[i * Np + j] * ue[j];
         }
-        const T* gk = s_geo + e * 5 * Np;
+        const T* gk = s_geo + e * GW * Np;
         const T J = gk[4 * Np + i];
         s_a[idx] = J * (gk[i] * ur + gk[Np + i] * us);
         s_b[idx] = J * (gk[2 * Np + i] * ur + gk[3 * Np + i] * us);
@@ -273,7 +285,7 @@ curved_elem_kernel(const T* __restrict__ tab, const T* __restrict__ geo,
           wy += M[i * Np + j] * be[j];
         }
       }
-      const T* gk = s_geo + e * 5 * Np;
+      const T* gk = s_geo + e * GW * Np;
       s_vr[idx] = gk[i] * wx + gk[2 * Np + i] * wy;
       s_vs[idx] = gk[Np + i] * wx + gk[3 * Np + i] * wy;
     }
@@ -304,7 +316,7 @@ curved_elem_kernel(const T* __restrict__ tab, const T* __restrict__ geo,
         T mu = 0;
 #pragma unroll
         for (int j = 0; j < Np; ++j) mu += M[i * Np + j] * ue[j];
-        acc += alpha * s_geo[e * 5 * Np + 4 * Np + i] * mu;
+        acc += alpha * s_geo[e * GW * Np + 4 * Np + i] * mu;
       }
       acc += shift * s_u[idx];
       const int64_t gi = k0 * Np + idx;
@@ -440,7 +452,9 @@ int curved_create(Op& op, int use_cub, const double* reftab, const double* geo,
   }
   // elements per workgroup of the element kernel: its LDS (elem_words per
   // element) within 4096 words
-  const int words = 13 * Np + (use_cub ? Np * (Np + 1) : 0);
+  // elements per group of the element kernel: its LDS plan within 4096 words
+  // (the largest plan here; each launch sizes its groups to its own plan)
+  const int words = curved_elem_words(Np, use_cub != 0, use_cub != 0);
   int epb = 4096 / words;
   epb = epb < 1 ? 1 : (epb > 64 ? 64 : epb);
   cv->epb = epb;
@@ -476,19 +490,23 @@ static int curved_t(const Op& op, const void* u, void* y, const void* udir, doub
   DDGX_HIP(hipGetLastError());
   // the element matrices are staged whenever the operator has them (CUB
   // kernel), the tables always
-  const size_t lds = ((size_t)c.epb * (13 * op.Np + (cub ? op.Np * (op.Np + 1) : 0)) +
-                      3 * (size_t)op.Np * op.Np) * sizeof(T);
+  // group size for this launch's LDS plan (the grid, and so the number of dot
+  // partials, is the operator's: idle workgroups write a zero partial)
+  const int words = curved_elem_words(op.Np, cub, alpha != 0.0);
+  int epb = 4096 / words;
+  epb = epb < 1 ? 1 : (epb > 64 ? 64 : epb);
+  const size_t lds = ((size_t)epb * words + 3 * (size_t)op.Np * op.Np) * sizeof(T);
   if (cub)
     hipLaunchKernelGGL((curved_elem_kernel<T, N, true>), dim3(c.grid_elem), dim3(256), lds, s,
                        (const T*)c.reftab, (const T*)c.geo, (const T*)c.estiff,
                        (const T*)c.emass, c.codes, (const T*)c.cown, (const T*)c.cnbr,
-                       (const T*)u, (T*)y, (T)alpha, (T)shift, c.epb, K, partial, skip,
+                       (const T*)u, (T*)y, (T)alpha, (T)shift, epb, K, partial, skip,
                        (const T*)bvec);
   else
     hipLaunchKernelGGL((curved_elem_kernel<T, N, false>), dim3(c.grid_elem), dim3(256), lds, s,
                        (const T*)c.reftab, (const T*)c.geo, (const T*)nullptr,
                        (const T*)nullptr, c.codes, (const T*)c.cown, (const T*)c.cnbr,
-                       (const T*)u, (T*)y, (T)alpha, (T)shift, c.epb, K, partial, skip,
+                       (const T*)u, (T*)y, (T)alpha, (T)shift, epb, K, partial, skip,
                        (const T*)bvec);
   DDGX_HIP(hipGetLastError());
   return DDGX_OK;

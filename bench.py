@@ -405,6 +405,73 @@ def serve_block(n_requests, slots, method="pcg32", calib=200, load=0.8, sample=8
     return out
 
 
+def config3_block(n_grad=3, n_fd=4):
+    """Differentiation (config[3]): -div(kappa grad u) = f on 2x64x64, N=3,
+    per-element kappa log-normal (sigma 0.5) and per-element tau_scale,
+    float64, block-Jacobi PCG to 1e-14 (the test's settings); loss = half the
+    M-norm of the misfit to the solution of a perturbed kappa.  Times one
+    loss + gradient (forward solve, adjoint solve, parameter-gradient kernel)
+    and checks the gradient against centred differences (eps 1e-6) on
+    ``n_fd`` seeded directions."""
+    import torch
+    from ddg_amd import mesh as pm
+    from ddg_amd.adjoint import SolveSpec, mass_apply, sipg_solve
+    from ddg_amd.operator import SipgOperator
+    dev = torch.device("cuda")
+    VX, VY, E = pm.uniform_rectangle_mesh_2d(0.0, 1.0, 0.0, 1.0, 64, 64)
+    VX, VY = pm.jitter_vertices(VX, VY, 1.0 / 64, 0.1, 3)
+    m = pm.build_mesh_2d(VX, VY, E, 3)
+    bc = pm.dirichlet_x_neumann_y(m)
+    rng = np.random.default_rng(33)
+    kap = np.exp(0.5 * rng.standard_normal(m.K))
+    tau = 200.0 * rng.uniform(0.8, 1.2, m.K)
+    ud = np.exp(m.x) * np.cos(3 * m.y)
+    spec = SolveSpec(m, bc, u_dir=_flat(ud, dev), preconditioner="bj", tol=1e-14,
+                     maxiter=30000)
+    ft = _flat(8.0 * np.exp(m.x) * np.cos(3 * m.y), dev)
+    mop = SipgOperator(m, bc)
+    with torch.no_grad():
+        target = sipg_solve(torch.tensor(kap * np.exp(0.3 * rng.standard_normal(m.K)),
+                                         device=dev), torch.tensor(tau, device=dev), ft, spec)
+
+    def loss(k, t):
+        e = sipg_solve(k, t, ft, spec) - target
+        return 0.5 * torch.dot(e, mass_apply(e, mop))
+
+    times = []
+    for _ in range(n_grad):
+        kt = torch.tensor(kap, device=dev, requires_grad=True)
+        tt = torch.tensor(tau, device=dev, requires_grad=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        L = loss(kt, tt)
+        L.backward()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    g = torch.cat([kt.grad, tt.grad]).cpu().numpy()
+    gn = float(np.linalg.norm(g))
+    errs = []
+    with torch.no_grad():
+        for _ in range(n_fd):
+            d = rng.standard_normal(2 * m.K)
+            d /= np.linalg.norm(d)
+            dk, dt = torch.tensor(d[:m.K], device=dev), torch.tensor(d[m.K:], device=dev)
+            k0, t0_ = kt.detach(), tt.detach()
+            fd = (loss(k0 + 1e-6 * dk, t0_ + 1e-6 * dt)
+                  - loss(k0 - 1e-6 * dk, t0_ - 1e-6 * dt)) / 2e-6
+            errs.append(abs(fd.item() - float(g @ d)) / gn)
+    out = {"workload": "2x64x64, N=3, log-normal kappa and per-element tau, block-Jacobi PCG "
+                       "to 1e-14, float64",
+           "K": int(m.K), "dofs": int(m.K * m.Np),
+           "loss_and_gradient_ms": round(1e3 * float(np.median(times)), 2),
+           "what": "forward solve + adjoint solve (same operator and preconditioner) + "
+                   "ddgx_sipg_param_grad, median of repeats",
+           "fd_directions": n_fd, "fd_max_rel_err": max(errs), "grad_norm": gn,
+           "loss": float(L.item())}
+    print(f"[bench] config3: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def config0_block(n_solves, cpu):
     """The reference's CPU-runnable case: 2x16x16, N=3, all-wall Dirichlet,
     f = 2 pi^2 sin(pi x) sin(pi y), tau_scale = 200, block-Jacobi CG to tol
@@ -476,6 +543,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-config2", action="store_true")
     ap.add_argument("--no-curved", action="store_true", help="skip the curved-element apply")
+    ap.add_argument("--config3-only", action="store_true",
+                    help="run only the differentiation leg (config[3])")
     ap.add_argument("--curved-only", action="store_true",
                     help="only time the curved apply (for rocprofv3)")
     ap.add_argument("--no-plain", action="store_true",
@@ -552,6 +621,9 @@ def main():
             torch.mul(a, 2.0, out=b)
         torch.cuda.synchronize()
         print(f"[profile] {args.steps} copies of {args.profile_copy} MB", flush=True)
+        return
+    if args.config3_only:
+        print(json.dumps({"config3": config3_block()}), flush=True)
         return
     if args.curved_only:
         print(json.dumps({"curved": curved_block(args.n, 4, args.steps, args.warmup)}), flush=True)
@@ -658,6 +730,7 @@ def main():
             fixture="config2_n6" if fx else None, log="config2 fourier N=6")
 
     c0 = config0_block(10, cpu=not args.no_cpu)
+    c3 = config3_block()
     curved = None if args.no_curved else curved_block(args.n, 4, args.steps, args.warmup)
 
     serve = None
@@ -699,7 +772,7 @@ def main():
                       "frac": round(apply_bytes(K, Np, 4) / t32 / 1e9 / HBM_PEAK_GBS, 4)},
         "cg": {"preconditioner": f"hp-multigrid levels K={mg64.sizes} N={mg64.degrees}",
                "tol": args.tol, "inner_tol": args.inner_tol, "config1": cg1, "config2": cg2,
-               "config0": c0},
+               "config0": c0, "config3": c3},
         "serve": serve,
         "curved": curved,
     }

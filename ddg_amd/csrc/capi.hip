@@ -61,7 +61,9 @@ static int valu_table_size(int N) {
 
 static int frag_size(int N) {
   const int Np = (N + 1) * (N + 2) / 2, Nfp = N + 1;
-  const int NB = (Np + 15) / 16, KB = 4 * NB, FK = (9 * Nfp + 3) / 4;
+  // face k-steps: 12 when the face rows are formed in registers (one
+  // 16-row block), else the 9 Nfp rows four per k-step
+  const int NB = (Np + 15) / 16, KB = 4 * NB, FK = NB == 1 ? 12 : (9 * Nfp + 3) / 4;
   return 64 * (2 * NB * KB + NB * 2 * KB + NB * FK + NB * KB);
 }
 
@@ -190,17 +192,22 @@ int ddgx_op_create(int N, int64_t K, int dtype, const double* tables, const doub
   }
   // the smaller-LDS kernel variant when every halo fits its capacity
   const int hcap = max_halo <= HALO_SMALL ? HALO_SMALL : HALO_MAX;
+  // the halo lists hold element ids (the tile positions mapped through
+  // perm), so the apply loads a halo element's values without a dependent
+  // load of perm
   std::vector<int> hpos(ntiles * hcap);
   for (int64_t t = 0; t < ntiles; ++t)
-    for (int h = 0; h < hcap; ++h) hpos[t * hcap + h] = hall[t * HALO_MAX + h];
+    for (int h = 0; h < hcap; ++h) {
+      const int pos = hall[t * HALO_MAX + h];
+      hpos[t * hcap + h] = pos >= 0 ? perm[pos] : -1;
+    }
   // halo face coefficients a_f, b_f gathered per tile, hab[t][c][h], so the
   // apply reads them with contiguous loads
   std::vector<double> hab((size_t)ntiles * 6 * hcap, 0.0);
   for (int64_t t = 0; t < ntiles; ++t)
     for (int h = 0; h < hcap; ++h) {
-      const int pos = hpos[t * hcap + h];
-      if (pos < 0) continue;
-      const int64_t kid = perm[pos];
+      const int64_t kid = hpos[t * hcap + h];
+      if (kid < 0) continue;
       for (int c = 0; c < 6; ++c)
         hab[((size_t)t * 6 + c) * hcap + h] = rec[kid * DDGX_REC_WORDS + R_A + c];
     }

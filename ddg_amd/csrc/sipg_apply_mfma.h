@@ -74,7 +74,11 @@ struct MS {
   static constexpr int NB = (Np + 15) / 16;     // 16-row blocks of a nodal vector
   static constexpr int KB = 4 * NB;             // k-steps over one nodal vector
   static constexpr int NF = 9 * Nfp;            // face rows: P, aX, bX
-  static constexpr int FK = (NF + 3) / 4;       // k-steps over the face rows
+  // one 16-row block: the face rows are formed in registers, four face
+  // nodes per lane (face_rows_inreg), 3 quantities each -- 12 k-steps;
+  // two blocks: the rows are staged in LDS, 4 per k-step
+  static constexpr bool INREG = NB == 1;
+  static constexpr int FK = INREG ? 12 : (NF + 3) / 4;  // k-steps over the face rows
   static constexpr int FS = 4 * FK + 1;         // LDS row stride of the face rows (odd)
   static constexpr int TRW = 3 * Nfp * 2;       // trace words per element: u, dn per face node
   static constexpr int NFP3 = 3 * Nfp;          // face nodes of an element
@@ -224,6 +228,74 @@ __device__ __forceinline__ void grad_and_traces(const T (&A1)[2 * MS<N>::NB][MS<
     }
 }
 
+// Face rows of GEMM 2 in the accumulator layout (one 16-row block): lane
+// (g, e) forms P, aX, bX of its element at the face nodes its registers
+// hold -- register f's node p = g (p = g + 1 on face 2) for g < N -- and,
+// for g < 3, at one vertex's second face (g = 0: face 2, p = 0; g = 1:
+// face 0, p = N; g = 2: face 1, p = N), so the 3 Nfp face nodes are covered
+// once.  Own and neighbour traces both come from LDS.  Slot s (s = f for the
+// register faces, 3 for the vertex) gives the B operands of k-steps 3 s + q,
+// q = 0, 1, 2 for P, aX, bX (reference.mfma_face_row is the host copy that
+// lays out the lift's columns to match; a lane with no node in a slot still
+// forms the finite values of face node (0, 0), which meet zero columns).
+template <typename T, int N, bool UDIR>
+__device__ __forceinline__ void face_rows_inreg(T (&Fv)[12], const T* s_x, const T* re,
+                                                const int* s_fc, int el, int g,
+                                                const T* __restrict__ udir, int kg) {
+  constexpr int Nfp = N + 1, Np = Shape<N>::Np, SW = MS<N>::TRW;
+  // the vertex slot's face and node for lane groups 0..3, 4 bits each
+  constexpr unsigned VF = 0x2102u, VP = (unsigned)(N << 4) | (unsigned)(N << 8);
+  auto slot = [&](int s, int& f, int& p) {
+    if (s < 3) {
+      f = s;
+      p = g < N ? (s < 2 ? g : g + 1) : 0;
+    } else {
+      f = (int)((VF >> (4 * g)) & 15u);
+      p = (int)((VP >> (4 * g)) & 15u);
+    }
+  };
+  // Dirichlet data at the four face nodes, loaded up front (no load in a
+  // branch; only Dirichlet faces use them)
+  T uDv[4] = {};
+  if constexpr (UDIR) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      int f, p;
+      slot(s, f, p);
+      uDv[s] = udir[(size_t)kg * Np + fmask<N>(f, p)];
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    int f, p;
+    slot(s, f, p);
+    const int j = f * Nfp + p;
+    const T uM = s_x[el * SW + 2 * j], dnM = s_x[el * SW + 2 * j + 1];
+    const int c = s_fc[f * TILE + el];
+    const int ty = fc_type(c);
+    const int pn = fc_rev(c) ? Nfp - 1 - p : p;
+    // the neighbour's trace, read for every face (slot 0 for a boundary
+    // face) and weighted by the face type, so no read sits in a branch; its
+    // dn was taken with its own outward normal
+    const T* xp = s_x + fc_slot(c) * SW + 2 * (fc_face(c) * Nfp + pn);
+    const T uN = xp[0], dN = xp[1];
+    const T cI = ty == FT_INTERIOR ? T(1) : T(0);
+    const T cD = ty == FT_DIRICHLET ? T(1) : T(0);
+    T ju = cI * (uM - uN);  // [u]: u^- - u^+ (Neumann: 0)
+    if constexpr (UDIR) {
+      ju += cD * (uM - uDv[s]);
+    } else {
+      ju += cD * uM;
+    }
+    const T dnP = -cI * dN;
+    const T sj = re[R_SJW + f];
+    const T X = -sj * ju;
+    Fv[3 * s] = re[R_PEN + f] * ju - sj * (dnM + dnP);
+    Fv[3 * s + 1] = re[R_A + f] * X;
+    Fv[3 * s + 2] = re[R_B + f] * X;
+  }
+}
+
 // Tile handled by workgroup b of a G-workgroup launch.  Workgroups are
 // dispatched to the 8 XCDs round-robin (b % 8); each XCD gets one contiguous
 // run of tiles, so a tile's halo neighbours -- mostly the adjacent tiles of a
@@ -240,7 +312,8 @@ __device__ __forceinline__ int xcd_tile(int b, int G) {
 // need more registers per lane.
 template <typename T, int N>
 struct ApplyWaves {
-  static constexpr int value = N <= 4 ? (sizeof(T) == 8 ? 4 : 8) : (sizeof(T) == 8 ? 1 : 3);
+  static constexpr int value =
+      N <= 4 ? (sizeof(T) == 8 ? 4 : (N == 3 ? 7 : 8)) : (sizeof(T) == 8 ? 1 : 3);
 };
 
 // One 64-element tile per workgroup (grid = number of tiles).  HC is the
@@ -270,7 +343,8 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
   constexpr bool SMOOTH = MODE & M_SMOOTH;
   constexpr int RECS = DDGX_REC_WORDS + 1;  // LDS stride of a record (odd: no bank conflicts)
   constexpr int XT = (TILE + HC) * SW;      // element slots: tile then halo
-  constexpr int XF = TILE * Q::FS;          // face rows
+  // face rows staged in LDS (two blocks), or the smoothing epilogue's rows
+  constexpr int XF = Q::INREG ? (SMOOTH ? 2 * TILE * Np : 0) : TILE * Q::FS;
   constexpr int XW = XT > XF ? XT : XF;
   static_assert(!SMOOTH || 2 * TILE * Np <= XW, "smoothing rows fit the face-row buffer");
   // s_x holds GEMM 1's output for the tile and its halo during the face
@@ -285,7 +359,9 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
   if (skip != nullptr && *skip) return;
 
   const int tid = threadIdx.x;
-  const int lane = tid & 63, wv = tid >> 6;
+  // the wave index is uniform: held in a scalar register, so the staging
+  // addresses below are scalar bases plus the lane
+  const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int g = lane >> 4, e = lane & 15;
   const int el = 16 * wv + e;  // this lane's element (and halo entry) within the tile
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -293,39 +369,72 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
   const int ne = min(TILE, K - k0);
   const int k = k0 + el;  // this lane's tile position
   const bool valid = el < ne;
-  const int kg = valid ? (perm ? perm[k] : k) : 0;  // its element id
 
-  // ---- global loads of the tile: indices first, then nodal values --------
-  // halo: the tile's out-of-tile neighbours (tile positions), 16 per wave,
-  // listed from entry 0; a wave past the list skips the halo work
-  const int* hp = hpos + (size_t)tile * HC;
-  const bool hwave = 16 * wv < HC && hp[16 * wv] >= 0;  // uniform per wave
-  const int hk = hwave ? hp[el] : -1;
-  const int hg = hk >= 0 ? (perm ? perm[hk] : hk) : -1;
-  T U[NB][4], UH[NB][4];
+  // ---- global loads of the tile ------------------------------------------
+  // Every load of the tile is issued before the first wait on one, with
+  // clamped addresses and the out-of-range values selected away afterwards
+  // (no load sits in a lane-divergent branch): a workgroup pays one memory
+  // round trip for its own data and one more for the halo, whose element
+  // list is the only index a nodal load depends on.  The node index of each
+  // accumulator position is computed once here, so the loads can be
+  // scheduled back to back.
+  int kg = valid ? k : 0;  // this lane's element id
+  if (perm != nullptr) {   // uniform branch
+    const int pv = perm[valid ? k : k0];
+    kg = valid ? pv : 0;
+  }
+  int nn[NB][4];
+#pragma unroll
+  for (int b = 0; b < NB; ++b)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) nn[b][r] = RowMap<T, N>::node(b, g, r);
+  T U[NB][4];
   // the right-hand side of the float64 residual and smoothing modes, loaded
   // with u so the epilogue does not wait on a load issued after GEMM 2
   // (float32 reads it in the epilogue: it has no registers to spare)
   constexpr bool NEEDB = (RESID || SMOOTH) && sizeof(T) == 8;
   T BV[NB][4];  // unused (and dropped by the compiler) unless NEEDB
+  {
+    T uv[NB][4], bv[NB][4];
 #pragma unroll
-  for (int b = 0; b < NB; ++b)
+    for (int b = 0; b < NB; ++b)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int n = RowMap<T, N>::node_here(b, g, r);
-      U[b][r] = (valid && n < Np) ? u[(size_t)kg * Np + n] : T(0);
-      UH[b][r] = (hg >= 0 && n < Np) ? u[(size_t)hg * Np + n] : T(0);
-      if constexpr (NEEDB) BV[b][r] = (valid && n < Np) ? bvec[(size_t)kg * Np + n] : T(0);
-    }
-  for (int i = tid; i < DDGX_REC_WORDS * TILE; i += 256) {
-    const int w = i / TILE, ee = i % TILE;
-    s_rec[ee * RECS + w] = ee < ne ? rec[(size_t)w * K + k0 + ee] : T(0);
+      for (int r = 0; r < 4; ++r) {
+        const int n = nn[b][r];
+        const size_t gi = (size_t)kg * Np + (n < Np ? n : Np - 1);
+        uv[b][r] = u[gi];
+        if constexpr (NEEDB) bv[b][r] = bvec[gi];
+      }
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const bool on = valid && nn[b][r] < Np;
+        U[b][r] = on ? uv[b][r] : T(0);
+        if constexpr (NEEDB) BV[b][r] = on ? bv[b][r] : T(0);
+      }
   }
-  for (int i = tid; i < 3 * TILE; i += 256) {
-    const int f = i / TILE, ee = i % TILE;
-    s_fc[i] = ee < ne ? fcode[(size_t)f * K + k0 + ee] : (FT_NONE << 11);
+  // halo: the element ids of the tile's out-of-tile neighbours, 16 per
+  // wave, listed from entry 0 (-1 past the end); a wave past the list skips
+  // the halo work
+  const int hkv = hpos[(size_t)tile * HC + (el < HC ? el : HC - 1)];
+  // records, face codes and halo coefficients into registers, then LDS
+  constexpr int RPT = DDGX_REC_WORDS * TILE / 256;  // record words per thread
+  constexpr int HPT = (6 * HC + 255) / 256;         // halo coefficients per thread
+  static_assert(DDGX_REC_WORDS * TILE % 256 == 0 && 3 * TILE <= 256, "staging split");
+  T rc[RPT], hb[HPT];
+  // thread (wave wv, lane) stages record word wv + 4 j and face code wv
+  // (waves 0-2) of tile element lane, and halo coefficient 64 (wv + 4 j) + lane
+  static_assert(TILE == 64, "staging by wave and lane");
+  const int le = lane < ne ? lane : ne - 1;
+#pragma unroll
+  for (int j = 0; j < RPT; ++j) rc[j] = rec[(size_t)(wv + 4 * j) * K + k0 + le];
+  const int fcv = fcode[(size_t)(wv < 3 ? wv : 2) * K + k0 + le];
+#pragma unroll
+  for (int j = 0; j < HPT; ++j) {
+    const int i = 64 * (wv + 4 * j) + lane;
+    hb[j] = hab[(size_t)tile * 6 * HC + (i < 6 * HC ? i : 0)];
   }
-  for (int i = tid; i < 6 * HC; i += 256) s_hab[i] = hab[(size_t)tile * 6 * HC + i];
   // GEMM 1 fragments, from one of the table's replicas (the many workgroups
   // that start together spread these loads over L2 channels)
   const T* fr = frag + (size_t)(blockIdx.x % FRAG_COPIES) * Q::nfrag * 64;
@@ -334,6 +443,30 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
   for (int b = 0; b < 2 * NB; ++b)
 #pragma unroll
     for (int t = 0; t < KB; ++t) A1[b][t] = fr[(size_t)(Q::f1 + b * KB + t) * 64 + lane];
+  // the halo's nodal values, once its element list has arrived; they are
+  // masked where the halo's GEMM 1 uses them, so no wait on these loads
+  // comes before the staging writes below
+  const int hk = el < HC ? hkv : -1;
+  const bool hwave = __ballot(hk >= 0) != 0;  // uniform per wave
+  T UH[NB][4] = {};
+  if (hwave) {
+    const int hg = hk >= 0 ? hk : 0;
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = nn[b][r];
+        UH[b][r] = u[(size_t)hg * Np + (n < Np ? n : Np - 1)];
+      }
+  }
+#pragma unroll
+  for (int j = 0; j < RPT; ++j) s_rec[lane * RECS + wv + 4 * j] = lane < ne ? rc[j] : T(0);
+  if (wv < 3) s_fc[tid] = lane < ne ? fcv : (FT_NONE << 11);
+#pragma unroll
+  for (int j = 0; j < HPT; ++j) {
+    const int i = tid + 256 * j;
+    if (i < 6 * HC) s_hab[i] = hb[j];
+  }
   __syncthreads();
 
   // ---- GEMM 1: own elements and halo into LDS ----------------------------
@@ -345,6 +478,11 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
     if (hwave) {
       const FaceAB<T> abh{s_hab[el], s_hab[HC + el], s_hab[2 * HC + el],
                           s_hab[3 * HC + el], s_hab[4 * HC + el], s_hab[5 * HC + el]};
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (!(hk >= 0 && RowMap<T, N>::node_here(b, g, r) < Np)) UH[b][r] = T(0);
       T HR[NB][4], HS[NB][4];
       grad_and_traces<T, N>(A1, UH, HR, HS, abh, s_x + (TILE + el) * SW, g);
     }
@@ -367,69 +505,74 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
 #pragma unroll
     for (int t = 0; t < FK; ++t) A2[b][2 * KB + t] = fr[(size_t)(Q::fF + b * FK + t) * 64 + lane];
   }
-  __syncthreads();
+  __syncthreads();  // the tile's and the halo's traces are in LDS
 
   // ---- face phase: P, aX, bX per (element, face node) ---------------------
-  // thread -> one face node (f, p) of elements eg, eg + EG, ...
-  constexpr int EG = 256 / FPW, ITE = TILE / EG;
-  const int fp = tid % FPW, eg = tid / FPW;
-  const bool fpa = fp < NFP3;
-  const int ff = fpa ? fp / Nfp : 0, pp = fpa ? fp % Nfp : 0;
-  const int nM = fmask<N>(ff, pp);
-  T fv[ITE][3];
+  T Fv[Q::INREG ? 12 : 1];
+  if constexpr (Q::INREG) {
+    face_rows_inreg<T, N, UDIR>(Fv, s_x, re, s_fc, el, g, udir, kg);
+  } else {
+    // thread -> one face node (f, p) of elements eg, eg + EG, ...
+    constexpr int EG = 256 / FPW, ITE = TILE / EG;
+    const int fp = tid % FPW, eg = tid / FPW;
+    const bool fpa = fp < NFP3;
+    const int ff = fpa ? fp / Nfp : 0, pp = fpa ? fp % Nfp : 0;
+    const int nM = fmask<N>(ff, pp);
+    T fv[ITE][3];
 #pragma unroll
-  for (int it = 0; it < ITE; ++it) {
-    const int e2 = eg + EG * it;
-    fv[it][0] = fv[it][1] = fv[it][2] = T(0);
-    if (fpa && e2 < ne) {
-      const T* rr = s_rec + e2 * RECS;
-      const T* xm = s_x + e2 * SW;
-      const T aM = rr[R_A + ff], bM = rr[R_B + ff];
-      const T uM = xm[fp * 2], dnM = xm[fp * 2 + 1];
-      const int c = s_fc[ff * TILE + e2];
-      const int ty = fc_type(c);
-      T uP, dnP;
-      if (ty == FT_INTERIOR) {
-        const int sl = fc_slot(c), fn = fc_face(c);
-        const int pn = fc_rev(c) ? (Nfp - 1 - pp) : pp;
-        // the neighbour's trace; its dn was taken with its own outward normal
-        const T* xp = s_x + sl * SW;
-        uP = xp[(fn * Nfp + pn) * 2];
-        dnP = -xp[(fn * Nfp + pn) * 2 + 1];
-      } else if (ty == FT_DIRICHLET) {
-        if constexpr (UDIR) {
-          const int kk = k0 + e2;
-          uP = udir[(size_t)(perm ? perm[kk] : kk) * Np + nM];
+    for (int it = 0; it < ITE; ++it) {
+      const int e2 = eg + EG * it;
+      fv[it][0] = fv[it][1] = fv[it][2] = T(0);
+      if (fpa && e2 < ne) {
+        const T* rr = s_rec + e2 * RECS;
+        const T* xm = s_x + e2 * SW;
+        const T aM = rr[R_A + ff], bM = rr[R_B + ff];
+        const T uM = xm[fp * 2], dnM = xm[fp * 2 + 1];
+        const int c = s_fc[ff * TILE + e2];
+        const int ty = fc_type(c);
+        T uP, dnP;
+        if (ty == FT_INTERIOR) {
+          const int sl = fc_slot(c), fn = fc_face(c);
+          const int pn = fc_rev(c) ? (Nfp - 1 - pp) : pp;
+          // the neighbour's trace; its dn was taken with its own outward normal
+          const T* xp = s_x + sl * SW;
+          uP = xp[(fn * Nfp + pn) * 2];
+          dnP = -xp[(fn * Nfp + pn) * 2 + 1];
+        } else if (ty == FT_DIRICHLET) {
+          if constexpr (UDIR) {
+            const int kk = k0 + e2;
+            uP = udir[(size_t)(perm ? perm[kk] : kk) * Np + nM];
+          } else {
+            uP = T(0);
+          }
+          dnP = T(0);
         } else {
-          uP = T(0);
+          uP = uM;
+          dnP = T(0);
         }
-        dnP = T(0);
-      } else {
-        uP = uM;
-        dnP = T(0);
+        const T ju = uM - uP;
+        const T X = -rr[R_SJW + ff] * ju;
+        fv[it][0] = rr[R_PEN + ff] * ju - rr[R_SJW + ff] * (dnM + dnP);
+        fv[it][1] = aM * X;
+        fv[it][2] = bM * X;
       }
-      const T ju = uM - uP;
-      const T X = -rr[R_SJW + ff] * ju;
-      fv[it][0] = rr[R_PEN + ff] * ju - rr[R_SJW + ff] * (dnM + dnP);
-      fv[it][1] = aM * X;
-      fv[it][2] = bM * X;
     }
-  }
-  __syncthreads();  // nodal values consumed: s_x becomes the face rows
+    __syncthreads();  // nodal values consumed: s_x becomes the face rows
 #pragma unroll
-  for (int it = 0; it < ITE; ++it) {
-    const int e2 = eg + EG * it;
-    if (fpa) {
-      s_x[e2 * Q::FS + fp] = fv[it][0];
-      s_x[e2 * Q::FS + NFP3 + fp] = fv[it][1];
-      s_x[e2 * Q::FS + 2 * NFP3 + fp] = fv[it][2];
+    for (int it = 0; it < ITE; ++it) {
+      const int e2 = eg + EG * it;
+      if (fpa) {
+        s_x[e2 * Q::FS + fp] = fv[it][0];
+        s_x[e2 * Q::FS + NFP3 + fp] = fv[it][1];
+        s_x[e2 * Q::FS + 2 * NFP3 + fp] = fv[it][2];
+      }
     }
+    constexpr int PADC = 4 * FK - Q::NF;
+    if constexpr (PADC > 0) {
+      for (int i = tid; i < TILE * PADC; i += 256) s_x[(i / PADC) * Q::FS + Q::NF + i % PADC] = T(0);
+    }
+    __syncthreads();
   }
-  constexpr int PADC = 4 * FK - Q::NF;
-  if constexpr (PADC > 0) {
-    for (int i = tid; i < TILE * PADC; i += 256) s_x[(i / PADC) * Q::FS + Q::NF + i % PADC] = T(0);
-  }
-  __syncthreads();
 
   // ---- GEMM 2: volume + face lift (+ mass) -------------------------------
   T Jm = T(0);
@@ -447,7 +590,14 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
       acs = MF::mma(A2[bo][KB + t], US[t / 4][t % 4], acs);
     }
 #pragma unroll
-    for (int t = 0; t < FK; ++t) acf = MF::mma(A2[bo][2 * KB + t], s_x[el * Q::FS + 4 * t + g], acf);
+    for (int t = 0; t < FK; ++t) {
+      T fb;
+      if constexpr (Q::INREG)
+        fb = Fv[t];
+      else
+        fb = s_x[el * Q::FS + 4 * t + g];
+      acf = MF::mma(A2[bo][2 * KB + t], fb, acf);
+    }
     if constexpr (HELM) {
 #pragma unroll
       for (int t = 0; t < KB; ++t)

@@ -75,15 +75,14 @@ sipg_apply_pipe_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
   using v4 = typename MF::v4;
   using PI = PipeIn<T, N, HC>;
   constexpr int Np = Q::Np, Nfp = Q::Nfp, NB = Q::NB, KB = Q::KB, FK = Q::FK;
-  constexpr int NFP3 = Q::NFP3, FPW = Q::FPW, SW = Q::TRW;
+  constexpr int SW = Q::TRW;
   static_assert(NB == 1, "one 16-row block per nodal vector");
   static_assert(DDGX_REC_WORDS * TILE % 256 == 0, "record words split evenly");
   constexpr bool HELM = MODE & M_HELM, UDIR = MODE & M_UDIR, DOT = MODE & M_DOT;
   constexpr bool RESID = MODE & M_RESID;
   constexpr int RECS = DDGX_REC_WORDS + 1;
-  constexpr int XT = (TILE + HC) * SW;
-  constexpr int XF = TILE * Q::FS;
-  constexpr int XW = XT > XF ? XT : XF;
+  static_assert(Q::INREG, "face rows formed in registers");
+  constexpr int XW = (TILE + HC) * SW;  // element slots: tile then halo
   __shared__ T s_fr[Q::nfrag * 64];
   __shared__ T s_x[XW];
   __shared__ T s_rec[TILE * RECS];
@@ -213,64 +212,9 @@ sipg_apply_pipe_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
     }
     lds_barrier();
 
-    // ---- face phase ------------------------------------------------------
-    constexpr int EG = 256 / FPW, ITE = TILE / EG;
-    const int fp = tid % FPW, eg = tid / FPW;
-    const bool fpa = fp < NFP3;
-    const int ff = fpa ? fp / Nfp : 0, pp = fpa ? fp % Nfp : 0;
-    const int nM = fmask<N>(ff, pp);
-    T fv[ITE][3];
-#pragma unroll
-    for (int it = 0; it < ITE; ++it) {
-      const int e2 = eg + EG * it;
-      fv[it][0] = fv[it][1] = fv[it][2] = T(0);
-      if (fpa && e2 < ne) {
-        const T* rr = s_rec + e2 * RECS;
-        const T* xm = s_x + e2 * SW;
-        const T aM = rr[R_A + ff], bM = rr[R_B + ff];
-        const T uM = xm[fp * 2], dnM = xm[fp * 2 + 1];
-        const int c = s_fc[ff * TILE + e2];
-        const int ty = fc_type(c);
-        T uP, dnP;
-        if (ty == FT_INTERIOR) {
-          const int sl = fc_slot(c), fn = fc_face(c);
-          const int pn = fc_rev(c) ? (Nfp - 1 - pp) : pp;
-          const T* xp = s_x + sl * SW;
-          uP = xp[(fn * Nfp + pn) * 2];
-          dnP = -xp[(fn * Nfp + pn) * 2 + 1];
-        } else if (ty == FT_DIRICHLET) {
-          if constexpr (UDIR)
-            uP = udir[(size_t)(k0 + e2) * Np + nM];
-          else
-            uP = T(0);
-          dnP = T(0);
-        } else {
-          uP = uM;
-          dnP = T(0);
-        }
-        const T ju = uM - uP;
-        const T X = -rr[R_SJW + ff] * ju;
-        fv[it][0] = rr[R_PEN + ff] * ju - rr[R_SJW + ff] * (dnM + dnP);
-        fv[it][1] = aM * X;
-        fv[it][2] = bM * X;
-      }
-    }
-    lds_barrier();  // traces consumed: s_x becomes the face rows
-#pragma unroll
-    for (int it = 0; it < ITE; ++it) {
-      const int e2 = eg + EG * it;
-      if (fpa) {
-        s_x[e2 * Q::FS + fp] = fv[it][0];
-        s_x[e2 * Q::FS + NFP3 + fp] = fv[it][1];
-        s_x[e2 * Q::FS + 2 * NFP3 + fp] = fv[it][2];
-      }
-    }
-    constexpr int PADC = 4 * FK - Q::NF;
-    if constexpr (PADC > 0) {
-      for (int i = tid; i < TILE * PADC; i += 256)
-        s_x[(i / PADC) * Q::FS + Q::NF + i % PADC] = T(0);
-    }
-    lds_barrier();
+    // ---- face phase: P, aX, bX in registers -------------------------------
+    T Fv[12];
+    face_rows_inreg<T, N, UDIR>(Fv, s_x, re, s_fc, el, g, udir, k0 + el);
 
     // ---- GEMM 2: volume + face lift (+ mass) -----------------------------
     T Jm = T(0);
@@ -286,9 +230,9 @@ sipg_apply_pipe_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
 #pragma unroll
     for (int t = 0; t < FK; ++t) {
       if (t % 2 == 0)
-        acf = MF::mma(s_fr[(Q::fF + t) * 64 + lane], s_x[el * Q::FS + 4 * t + g], acf);
+        acf = MF::mma(s_fr[(Q::fF + t) * 64 + lane], Fv[t], acf);
       else
-        acg = MF::mma(s_fr[(Q::fF + t) * 64 + lane], s_x[el * Q::FS + 4 * t + g], acg);
+        acg = MF::mma(s_fr[(Q::fF + t) * 64 + lane], Fv[t], acg);
     }
     if constexpr (HELM) {
 #pragma unroll

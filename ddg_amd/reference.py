@@ -239,6 +239,35 @@ def mfma_node_map(ref: "RefElement", dtype: str) -> np.ndarray:
     return out
 
 
+def mfma_face_ksteps(Np: int, Nfp: int) -> int:
+    """k-steps of the face lift: 12 with the face rows formed in registers
+    (one 16-row block), else the 9 Nfp rows four per k-step."""
+    return 12 if (Np + 15) // 16 == 1 else (9 * Nfp + 3) // 4
+
+
+def mfma_face_row(N: int, Np: int, t, g):
+    """Face row (index into [P | aX | bX] x 3 Nfp face nodes, 9 Nfp meaning
+    padding) that lane group g supplies at k-step t of the face lift; the
+    kernels' face_rows_inreg for one block, 4 t + g for two.
+
+    One block: slot s = t // 3, quantity q = t % 3.  Slots 0-2 are face s
+    at node p = g (p = g + 1 on face 2) for g < N; slot 3 is one vertex's
+    second face: g = 0 face 2 p = 0, g = 1 face 0 p = N, g = 2 face 1 p = N.
+    """
+    Nfp = N + 1
+    NF = 9 * Nfp
+    t = np.asarray(t)
+    g = np.asarray(g)
+    if (Np + 15) // 16 != 1:
+        r = 4 * t + g
+        return np.where(r < NF, r, NF)
+    s, q = t // 3, t % 3
+    f = np.where(s < 3, s, np.choose(np.minimum(g, 3), [2, 0, 1, 0]))
+    p = np.where(s < 3, np.where(s < 2, g, g + 1), np.where((g == 1) | (g == 2), N, 0))
+    on = np.where(s < 3, g < N, g < 3)
+    return np.where(on, q * 3 * Nfp + f * Nfp + p, NF)
+
+
 def mfma_fragments(ref: "RefElement", dtype: str) -> np.ndarray:
     """A-operand fragments of the matrix-core apply, one 64-lane fragment per
     (output block, k-step), in the order GEMM1 | GEMM2 volume | face | mass.
@@ -248,13 +277,13 @@ def mfma_fragments(ref: "RefElement", dtype: str) -> np.ndarray:
     so row(b, m) is the node mfma_node_map puts there; for products whose B
     operand is a nodal vector held in accumulator layout, col(t, g) is the
     node of block t // 4, lane group g, register t % 4; for the face rows it
-    is 4 t + g.  Entries outside the matrix (padding) are zero.
+    is mfma_face_row(t, g).  Entries outside the matrix (padding) are zero.
     """
     Np, Nfp, Fm = ref.Np, ref.Nfp, ref.Fmask
     NB = (Np + 15) // 16
     KB = 4 * NB
     NF = 9 * Nfp
-    FK = (NF + 3) // 4
+    FK = mfma_face_ksteps(Np, Nfp)
     lanes = np.arange(64)
     m, g = lanes % 16, lanes // 16
     nmap = mfma_node_map(ref, dtype)
@@ -291,7 +320,7 @@ def mfma_fragments(ref: "RefElement", dtype: str) -> np.ndarray:
             out.append(frag_nodal(ArT if t < KB else AsT, b, t % KB))
     for b in range(NB):                                  # face lift
         for t in range(FK):
-            row, col = row_node(b), 4 * t + g
+            row, col = row_node(b), mfma_face_row(ref.N, Np, t, g)
             ok = (row < Np) & (col < NF)
             out.append(np.where(ok, G[np.minimum(row, Np - 1), np.minimum(col, NF - 1)], 0.0))
     for b in range(NB):                                  # mass

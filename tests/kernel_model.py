@@ -11,6 +11,8 @@ from __future__ import annotations
 
 import numpy as np
 
+from ddg_amd.reference import mfma_face_ksteps, mfma_face_row
+
 
 def unpack_tables(tab, N):
     """Inverse of RefElement.kernel_tables (matrices stored by columns,
@@ -117,7 +119,7 @@ def model_apply_mfma(N, tab, frag, rec, nbr, u, dtype="f64", node_map=None):
     NB = (Np + 15) // 16
     KB = 4 * NB
     NF = 9 * Nfp
-    FK = (NF + 3) // 4
+    FK = mfma_face_ksteps(Np, Nfp)
     nfr = [2 * NB * KB, NB * 2 * KB, NB * FK, NB * KB]
     offs = np.cumsum([0] + nfr)
     F = frag.reshape(-1, 64)
@@ -128,7 +130,7 @@ def model_apply_mfma(N, tab, frag, rec, nbr, u, dtype="f64", node_map=None):
     y = np.zeros_like(u)
     # face rows from the exact model (P, aX, bX per face node)
     ur_all, us_all = u @ Dr.T, u @ Ds.T
-    Fr = np.zeros((K, 4 * FK))
+    Fr = np.zeros((K, NF + 1))  # the last column is the padding row (zero)
     for f in range(3):
         dnM = rec[:, [4 + f]] * ur_all[:, Fm[f]] + rec[:, [7 + f]] * us_all[:, Fm[f]]
         code = nbr[:, f]
@@ -178,7 +180,7 @@ def model_apply_mfma(N, tab, frag, rec, nbr, u, dtype="f64", node_map=None):
             for t in range(KB):
                 acc = mma(F[offs[1] + bo * 2 * KB + KB + t], WS[t // 4][t % 4], acc, dtype)
             for t in range(FK):
-                acc = mma(F[offs[2] + bo * FK + t], Fr[ek, 4 * t + g], acc, dtype)
+                acc = mma(F[offs[2] + bo * FK + t], Fr[ek, mfma_face_row(N, Np, t, g)], acc, dtype)
             for r in range(4):
                 n = node(bo, r)
                 ok = valid & (n < Np)

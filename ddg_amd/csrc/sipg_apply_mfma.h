@@ -308,12 +308,14 @@ __device__ __forceinline__ int xcd_tile(int b, int G) {
 
 // Minimum waves per SIMD the register allocator is held to: N <= 4 fits 128
 // registers in float64 (4 waves) and 64 in float32 (8 waves, which the
-// float32 LDS footprint also allows); orders with two 16-row blocks (N >= 5)
-// need more registers per lane.
+// float32 LDS footprint also allows; 7 at N = 3, which spills at 64).
+// Orders with two 16-row blocks (N >= 5) hold 2 x 48 A fragments and fit
+// 256 registers in float64 (2 waves: twice the workgroups of one wave at
+// 512, N = 6 91 -> 67 us at 2x256x256; 3 waves spill) and 128 in float32.
 template <typename T, int N>
 struct ApplyWaves {
   static constexpr int value =
-      N <= 4 ? (sizeof(T) == 8 ? 4 : (N == 3 ? 7 : 8)) : (sizeof(T) == 8 ? 1 : 3);
+      N <= 4 ? (sizeof(T) == 8 ? 4 : (N == 3 ? 7 : 8)) : (sizeof(T) == 8 ? 2 : 4);
 };
 
 // One 64-element tile per workgroup (grid = number of tiles).  HC is the

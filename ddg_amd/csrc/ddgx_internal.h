@@ -52,6 +52,11 @@ struct SymPack {
   }
 };
 
+// Entry (i, j) of a packed symmetric n x n block (SymPack at run-time n)
+__host__ __device__ inline int sym_pk(int n, int i, int j) {
+  return i <= j ? i * n - i * (i - 1) / 2 + (j - i) : j * n - j * (j - 1) / 2 + (i - j);
+}
+
 // Inverse blocks of a general (curved-element) operator are not symmetric
 // (its nodal volume form weights M_ref by the per-node Jacobian on one side,
 // sipg.py:188-189): they are stored in full, row-major, Np*Np words per
@@ -118,8 +123,13 @@ constexpr int CODE_NONE = -2;
 // Kernel mode bits for the apply (M_SMOOTH: one block-Jacobi sweep
 // x_out = x + omega Binv (b - A x), launched through launch_smooth;
 // M_PROLONG: with x = u + the h-prolongation of a coarser vector, formed on
-// the fly, order 1 only)
-enum : int { M_HELM = 1, M_UDIR = 2, M_DOT = 8, M_RESID = 16, M_SMOOTH = 32, M_PROLONG = 64 };
+// the fly, order 1 only; M_RESTRICT, with M_RESID: the residual b - A x is
+// not stored but restricted to the next-coarser multigrid level, with that
+// level's first smoothing sweep, launched through launch_resid_restrict)
+enum : int {
+  M_HELM = 1, M_UDIR = 2, M_DOT = 8, M_RESID = 16, M_SMOOTH = 32, M_PROLONG = 64,
+  M_RESTRICT = 128
+};
 
 // Device scalars of one CG solve
 struct CGScalars {
@@ -318,6 +328,17 @@ int launch_smooth(const Op& op, const void* binv, double omega, double alpha, co
 // Whether launch_smooth can take the h-prolongation of a coarser level as part
 // of its input (prolong_from / prolong_mat) on this operator
 bool smooth_prolongs(const Op& op);
+// The multigrid residual and its restriction in one launch:
+//   b_c = P^T (b - (A + alpha M) x)   per coarse unit (tkind 0: the element,
+//         P is Np x np_lo; tkind 1: the four children 4k..4k+3 of coarse
+//         element k, P is 4 Np x Np), and with x_c != null also the coarser
+//         level's first sweep x_c = omega Binv_c b_c (Binv_c packed, np_lo).
+// The fine residual is not stored.  Returns DDGX_ERR_UNSUPPORTED where the
+// fused kernel is not built (the caller then runs the residual apply and the
+// restriction kernel): an h-transfer needs order 1 in the identity tile order.
+int launch_resid_restrict(const Op& op, int tkind, int np_lo, const void* P, const void* x,
+                          const void* b, double alpha, void* b_c, const void* binv_c,
+                          double omega, void* x_c, const int* skip, hipStream_t s);
 int mg_vcycle(const MG& mg, MGWork& w, const void* b, void* x, const int* skip, hipStream_t s);
 int mg_create(int nlev, Op* const* ops, Op* const* ops64, const int* tkind,
               const double* const* P, const ddgx_mg_cfg& cfg, hipStream_t s, MG** out);

@@ -63,11 +63,6 @@ bj_smooth_kernel(const T* __restrict__ binv, const T* __restrict__ r, T* __restr
   }
 }
 
-// Entry (i, j) of a packed symmetric n x n block (SymPack at run-time n)
-__device__ __forceinline__ int sym_pk(int n, int i, int j) {
-  return i <= j ? i * n - i * (i - 1) / 2 + (j - i) : j * n - j * (j - 1) / 2 + (i - j);
-}
-
 // lo[k][j] = sum_h P[h][j] hi[k][h]; with x_c != null also the coarse level's
 // first smoothing sweep x_c[k] = omega Binv_c[k] lo[k] (the arithmetic of
 // bj_smooth_kernel without accumulation), so that level skips its launch
@@ -367,11 +362,20 @@ static int vcycle(const MG& mg, MGWork& w, int l, const void* b, void* x, const 
       MG_CHECK(launch_smooth(op, L.binv, mg.omega, mg.alpha, bufs[at], bufs[1 - at], b, skip, s));
       at = 1 - at;
     }
-    MG_CHECK(resid());
     // a smoothed coarser level gets its first sweep from the restriction
     const bool fuse_first = l + 2 < mg.nlev && smooth_fused(*C.op);
     void* c_first = (mg.nu - 1) % 2 ? w.r[l + 1] : Cx;
-    MG_CHECK(transfer<T>(L, C, false, Lr, Cb, skip, s, fuse_first ? c_first : nullptr, mg.omega));
+    // the residual and its restriction in one launch where that kernel is
+    // built (the fine residual is then never stored), else two
+    const int rr = launch_resid_restrict(op, L.tkind, C.op->Np, L.P, x, b, mg.alpha, Cb, C.binv,
+                                         mg.omega, fuse_first ? c_first : nullptr, skip, s);
+    if (rr == DDGX_ERR_UNSUPPORTED) {
+      MG_CHECK(resid());
+      MG_CHECK(transfer<T>(L, C, false, Lr, Cb, skip, s, fuse_first ? c_first : nullptr,
+                           mg.omega));
+    } else {
+      MG_CHECK(rr);
+    }
     MG_CHECK(vcycle<T>(mg, w, l + 1, Cb, Cx, skip, s, fuse_first));
     // the h-prolongation into an order-1 level is formed inside the first
     // post-smoothing sweep (which never stores the corrected iterate)

@@ -28,7 +28,8 @@ template <typename T, int N, int MODE>
 static int launch_t(const Op& op, const void* u, void* y, const void* udir, double alpha,
                     double shift, double* partial, const int* skip, hipStream_t s,
                     const void* bvec, const void* binv = nullptr, double omega = 0.0,
-                    const void* pxc = nullptr, const void* pmat = nullptr) {
+                    const void* pxc = nullptr, const void* pmat = nullptr, int np_lo = 0,
+                    void* xco = nullptr) {
   // one workgroup per 64-element tile; in dot mode each writes one partial
   const int* perm = op.perm_id ? nullptr : op.perm;
   // the pipelined kernel serves the plain and Helmholtz applies; the dot,
@@ -56,19 +57,20 @@ static int launch_t(const Op& op, const void* u, void* y, const void* udir, doub
     hipLaunchKernelGGL((sipg_apply_valu_kernel<T, N, MODE>), dim3(op.grid_apply), dim3(TILE), 0,
                        s, (const T*)op.tab, (const T*)op.rec, op.nbrp, perm, (const T*)u, (T*)y,
                        (const T*)udir, (T)alpha, (T)shift, (int)op.K, partial, skip,
-                       (const T*)bvec, (const T*)binv, (T)omega, (const T*)pxc, (const T*)pmat);
+                       (const T*)bvec, (const T*)binv, (T)omega, (const T*)pxc, (const T*)pmat,
+                       (T*)xco);
   else if (op.halo_cap == HALO_SMALL)
     hipLaunchKernelGGL((sipg_apply_mfma_kernel<T, N, MODE, HALO_SMALL>), dim3(op.grid_apply),
                        dim3(256), 0, s, (const T*)op.frag, (const T*)op.rec, op.fcode, perm,
                        op.hpos, (const T*)op.hab, (const T*)u, (T*)y, (const T*)udir, (T)alpha,
                        (T)shift, (int)op.K, partial, skip, (const T*)bvec, (const T*)binv,
-                       (T)omega);
+                       (T)omega, (const T*)pmat, np_lo, (T*)xco);
   else
     hipLaunchKernelGGL((sipg_apply_mfma_kernel<T, N, MODE, HALO_MAX>), dim3(op.grid_apply),
                        dim3(256), 0, s, (const T*)op.frag, (const T*)op.rec, op.fcode, perm,
                        op.hpos, (const T*)op.hab, (const T*)u, (T*)y, (const T*)udir, (T)alpha,
                        (T)shift, (int)op.K, partial, skip, (const T*)bvec, (const T*)binv,
-                       (T)omega);
+                       (T)omega, (const T*)pmat, np_lo, (T*)xco);
   DDGX_HIP(hipGetLastError());
   return DDGX_OK;
 }
@@ -220,6 +222,49 @@ int launch_smooth(const Op& op, const void* binv, double omega, double alpha, co
   }
 #undef SMOOTH_N
   set_error("fused smoothing sweep: polynomial order not built");
+  return DDGX_ERR_UNSUPPORTED;
+}
+
+// Residual + restriction (+ the coarser level's first sweep) in one launch:
+// the order-1 kernel takes h-transfers (identity tile order, whole families
+// in a tile), the matrix-core kernels p-transfers.
+static bool resid_restrict_fused(const Op& op, int tkind, int np_lo) {
+  if (op.cv != nullptr || op.N < 1 || op.N > 6) return false;
+  if (op.N == 1) return tkind == 1 && op.perm_id && op.K % 4 == 0 && np_lo == op.Np;
+  return tkind == 0 && np_lo >= 1 && np_lo <= op.Np;
+}
+
+template <typename T, int N>
+static int resid_restrict_n(const Op& op, const void* P, int np_lo, const void* x, const void* b,
+                            double alpha, void* b_c, const void* binv_c, double omega, void* x_c,
+                            const int* skip, hipStream_t s) {
+  constexpr int M = M_RESID | M_RESTRICT;
+  if (alpha != 0.0)
+    return launch_t<T, N, M | M_HELM>(op, x, b_c, nullptr, alpha, 0.0, nullptr, skip, s, b,
+                                       binv_c, omega, nullptr, P, np_lo, x_c);
+  return launch_t<T, N, M>(op, x, b_c, nullptr, alpha, 0.0, nullptr, skip, s, b, binv_c, omega,
+                           nullptr, P, np_lo, x_c);
+}
+
+int launch_resid_restrict(const Op& op, int tkind, int np_lo, const void* P, const void* x,
+                          const void* b, double alpha, void* b_c, const void* binv_c,
+                          double omega, void* x_c, const int* skip, hipStream_t s) {
+  if (!resid_restrict_fused(op, tkind, np_lo)) return DDGX_ERR_UNSUPPORTED;
+#define RR_N(T)                                                                          \
+  switch (op.N) {                                                                        \
+    case 1: return resid_restrict_n<T, 1>(op, P, np_lo, x, b, alpha, b_c, binv_c, omega, x_c, skip, s); \
+    case 2: return resid_restrict_n<T, 2>(op, P, np_lo, x, b, alpha, b_c, binv_c, omega, x_c, skip, s); \
+    case 3: return resid_restrict_n<T, 3>(op, P, np_lo, x, b, alpha, b_c, binv_c, omega, x_c, skip, s); \
+    case 4: return resid_restrict_n<T, 4>(op, P, np_lo, x, b, alpha, b_c, binv_c, omega, x_c, skip, s); \
+    case 5: return resid_restrict_n<T, 5>(op, P, np_lo, x, b, alpha, b_c, binv_c, omega, x_c, skip, s); \
+    case 6: return resid_restrict_n<T, 6>(op, P, np_lo, x, b, alpha, b_c, binv_c, omega, x_c, skip, s); \
+  }
+  if (op.dtype == DDGX_F64) {
+    RR_N(double)
+  } else {
+    RR_N(float)
+  }
+#undef RR_N
   return DDGX_ERR_UNSUPPORTED;
 }
 

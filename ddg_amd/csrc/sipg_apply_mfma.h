@@ -328,7 +328,8 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
                        const T* __restrict__ u, T* __restrict__ y, const T* __restrict__ udir,
                        T alpha, T shift, int K, double* __restrict__ partial,
                        const int* __restrict__ skip, const T* __restrict__ bvec,
-                       const T* __restrict__ binv, T omega) {
+                       const T* __restrict__ binv, T omega, const T* __restrict__ pmat,
+                       int np_lo, T* __restrict__ xco) {
   using Q = MS<N>;
   using MF = MfmaT<T>;
   using v4 = typename MF::v4;
@@ -343,12 +344,19 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
   constexpr bool RESID = MODE & M_RESID;
   // smoothing sweep: y = u + omega Binv (bvec - A u) (the epilogue below)
   constexpr bool SMOOTH = MODE & M_SMOOTH;
+  // with RESID: the residual restricted to the coarser p-level instead of
+  // stored (the epilogue below): y receives P^T (bvec - A u) per element
+  // (pmat: Np x np_lo), xco, when given, the coarse level's first sweep
+  // omega Binv_c (that) with binv the coarse level's packed blocks
+  constexpr bool RESTRICT = MODE & M_RESTRICT;
+  static_assert(!RESTRICT || (RESID && !SMOOTH && !DOT), "restriction of a residual");
+  constexpr bool ROWS = SMOOTH || RESTRICT;  // epilogue on element rows in LDS
   constexpr int RECS = DDGX_REC_WORDS + 1;  // LDS stride of a record (odd: no bank conflicts)
   constexpr int XT = (TILE + HC) * SW;      // element slots: tile then halo
   // face rows staged in LDS (two blocks), or the smoothing epilogue's rows
-  constexpr int XF = Q::INREG ? (SMOOTH ? 2 * TILE * Np : 0) : TILE * Q::FS;
+  constexpr int XF = Q::INREG ? (ROWS ? 2 * TILE * Np : 0) : TILE * Q::FS;
   constexpr int XW = XT > XF ? XT : XF;
-  static_assert(!SMOOTH || 2 * TILE * Np <= XW, "smoothing rows fit the face-row buffer");
+  static_assert(!ROWS || 2 * TILE * Np <= XW, "epilogue rows fit the face-row buffer");
   // s_x holds GEMM 1's output for the tile and its halo during the face
   // phase and, after a barrier, the face rows F that feed GEMM 2 (and, in
   // the smoothing epilogue, the residual and iterate rows).
@@ -484,7 +492,11 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
       for (int b = 0; b < NB; ++b)
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          if (!(hk >= 0 && RowMap<T, N>::node_here(b, g, r) < Np)) UH[b][r] = T(0);
+        {
+          // both terms evaluated (no branch around the node computation)
+          const bool on = (hk >= 0) & (RowMap<T, N>::node_here(b, g, r) < Np);
+          UH[b][r] = on ? UH[b][r] : T(0);
+        }
       T HR[NB][4], HS[NB][4];
       grad_and_traces<T, N>(A1, UH, HR, HS, abh, s_x + (TILE + el) * SW, g);
     }
@@ -610,7 +622,7 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
       const int n = RowMap<T, N>::node_here(bo, g, r);
       T v = acc[r] + acs[r] + acf[r];
       if constexpr (HELM) v += shift * U[bo][r];
-      if constexpr (SMOOTH) {
+      if constexpr (ROWS) {
         vv[bo][r] = v;
       } else if (valid && n < Np) {
         const size_t gi = (size_t)kg * Np + n;
@@ -653,6 +665,44 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
 #pragma unroll
         for (int j = 0; j < Np; ++j) a += bk[SymPack<N>::pk(li, j)] * s_r[e2 * Np + j];
         y[k2 * Np + li] = s_u[e2 * Np + li] + omega * a;
+      }
+    }
+  }
+
+  if constexpr (RESTRICT) {
+    // the residual rows into LDS, then one thread per (element, coarse node)
+    T* s_r = s_x;
+    T* s_lo = s_x + TILE * Np;
+    __syncthreads();  // every wave is done reading the traces
+#pragma unroll
+    for (int bo = 0; bo < NB; ++bo)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = RowMap<T, N>::node_here(bo, g, r);
+        if (valid && n < Np)
+          s_r[el * Np + n] = (NEEDB ? BV[bo][r] : bvec[(size_t)kg * Np + n]) - vv[bo][r];
+      }
+    __syncthreads();
+    for (int it = tid; it < ne * np_lo; it += 256) {
+      const int e2 = it / np_lo, j = it % np_lo;
+      T lo = 0;
+#pragma unroll
+      for (int h = 0; h < Np; ++h) lo += pmat[h * np_lo + j] * s_r[e2 * Np + h];
+      const int kk = k0 + e2;
+      y[(size_t)(perm ? perm[kk] : kk) * np_lo + j] = lo;
+      s_lo[it] = lo;
+    }
+    if (xco != nullptr) {
+      __syncthreads();
+      const int npk = np_lo * (np_lo + 1) / 2;
+      for (int it = tid; it < ne * np_lo; it += 256) {
+        const int e2 = it / np_lo, j = it % np_lo;
+        const int kk = k0 + e2;
+        const size_t k2 = (size_t)(perm ? perm[kk] : kk);
+        const T* bk = binv + k2 * npk;
+        T a = 0;
+        for (int i = 0; i < np_lo; ++i) a += bk[sym_pk(np_lo, j, i)] * s_lo[e2 * np_lo + i];
+        xco[k2 * np_lo + j] = omega * a;
       }
     }
   }

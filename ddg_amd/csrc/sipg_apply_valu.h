@@ -39,7 +39,7 @@ sipg_apply_valu_kernel(const T* __restrict__ tab, const T* __restrict__ rec,
                        T alpha, T shift, int K, double* __restrict__ partial,
                        const int* __restrict__ skip, const T* __restrict__ bvec,
                        const T* __restrict__ binv, T omega, const T* __restrict__ pxc,
-                       const T* __restrict__ pmat) {
+                       const T* __restrict__ pmat, T* __restrict__ xco) {
   using S = Shape<N>;
   constexpr int Np = S::Np, Nfp = S::Nfp, RS = S::RS, RSF = S::RSF;
   constexpr bool HELM = MODE & M_HELM, UDIR = MODE & M_UDIR, DOT = MODE & M_DOT;
@@ -50,6 +50,15 @@ sipg_apply_valu_kernel(const T* __restrict__ tab, const T* __restrict__ rec,
   // 4k+c is child c of coarse element k; pmat holds the 4 child matrices,
   // rows c*Np + i), formed on the fly instead of stored
   constexpr bool PROLONG = MODE & M_PROLONG;
+  // with RESID: the residual is restricted to the coarser h-level instead of
+  // stored -- y receives the coarse right-hand side P^T (b - A u) of every
+  // coarse element whose four children 4k..4k+3 the tile holds (pmat: the
+  // 4 Np x Np child matrices), and xco, when given, the coarse level's first
+  // sweep omega Binv_c (that) with binv the coarse level's packed blocks.
+  // Identity tile order only (the children of a coarse element are then
+  // consecutive tile positions of one tile).
+  constexpr bool RESTRICT = MODE & M_RESTRICT;
+  static_assert(!RESTRICT || (RESID && !SMOOTH && !DOT && !PROLONG), "restriction of a residual");
   auto uval = [&](size_t kid, int j) -> T {
     T v = u[kid * Np + j];
     if constexpr (PROLONG) {
@@ -265,6 +274,34 @@ sipg_apply_valu_kernel(const T* __restrict__ tab, const T* __restrict__ rec,
     for (int i = 0; i < Np; ++i) s_u[el * Np + i] = v[i];
   }
   __syncthreads();
+  if constexpr (RESTRICT) {
+    // r = b - A u in place; then one thread per (coarse element, node)
+    const T* bb = bvec + (size_t)k0 * Np;
+    for (int i = el; i < ne * Np; i += TILE) s_u[i] = bb[i] - s_u[i];
+    __syncthreads();
+    const int cl = el / Np, j = el % Np;
+    const bool on = cl < ne / 4;
+    const size_t kc = (size_t)(k0 / 4 + cl);
+    T lo = 0;
+    if (on) {
+#pragma unroll
+      for (int h = 0; h < 4 * Np; ++h) lo += pmat[h * Np + j] * s_u[4 * cl * Np + h];
+      y[kc * Np + j] = lo;
+    }
+    if (xco != nullptr) {
+      constexpr int NPK = SymPack<N>::NPK;
+      __syncthreads();  // every thread has read the residual rows
+      if (on) s_u[el] = lo;
+      __syncthreads();
+      if (on) {
+        T a = 0;
+#pragma unroll
+        for (int i = 0; i < Np; ++i) a += binv[kc * NPK + SymPack<N>::pk(j, i)] * s_u[cl * Np + i];
+        xco[kc * Np + j] = omega * a;
+      }
+    }
+    return;
+  }
   if (perm == nullptr) {
     T* dst = y + (size_t)k0 * Np;
     if constexpr (RESID && !SMOOTH) {

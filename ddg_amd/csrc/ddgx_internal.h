@@ -125,10 +125,13 @@ constexpr int CODE_NONE = -2;
 // M_PROLONG: with x = u + the h-prolongation of a coarser vector, formed on
 // the fly, order 1 only; M_RESTRICT, with M_RESID: the residual b - A x is
 // not stored but restricted to the next-coarser multigrid level, with that
-// level's first smoothing sweep, launched through launch_resid_restrict)
+// level's first smoothing sweep, launched through launch_resid_restrict;
+// M_PROLONG_OUT, with M_SMOOTH: the sweep's result is also prolongated into
+// the next-finer level's iterate, p-transfers, launched through
+// launch_smooth's prolong_to)
 enum : int {
   M_HELM = 1, M_UDIR = 2, M_DOT = 8, M_RESID = 16, M_SMOOTH = 32, M_PROLONG = 64,
-  M_RESTRICT = 128
+  M_RESTRICT = 128, M_PROLONG_OUT = 256
 };
 
 // Device scalars of one CG solve
@@ -322,9 +325,15 @@ bool smooth_fused(const Op& op);
 // grid (workgroups resident at once, at most one per tile).
 bool apply_pipelined(int N, int dtype, bool perm_id);
 int apply_pipe_grid(int N, int dtype, int halo_cap, int64_t ntiles);
+// With prolong_to != null (p-transfers, affine levels): the sweep's result is
+// also added, through prolong_to_mat (np_to x Np, row-major), into the
+// next-finer level's iterate prolong_to -- the prolongation x_f += P x of the
+// V-cycle, element by element, in the coarser level's last sweep.
 int launch_smooth(const Op& op, const void* binv, double omega, double alpha, const void* x_in,
                   void* x_out, const void* b, const int* skip, hipStream_t s,
-                  const void* prolong_from = nullptr, const void* prolong_mat = nullptr);
+                  const void* prolong_from = nullptr, const void* prolong_mat = nullptr,
+                  void* prolong_to = nullptr, const void* prolong_to_mat = nullptr,
+                  int np_to = 0);
 // Whether launch_smooth can take the h-prolongation of a coarser level as part
 // of its input (prolong_from / prolong_mat) on this operator
 bool smooth_prolongs(const Op& op);

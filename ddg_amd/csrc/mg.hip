@@ -334,11 +334,22 @@ static int transfer(const MGLevel& L, const MGLevel& C, bool prolong, const void
   return DDGX_OK;
 }
 
+// The parent's p-prolongation handed to a coarser level's cycle: its last
+// fused sweep also adds P x (P: np x Np of this level) into the parent's
+// iterate, and reports that it did.
+struct ProlongTo {
+  void* x;
+  const void* P;
+  int np;
+  bool done;
+};
+
 // V-cycle on level l; first_done: the level's first sweep (omega Binv b) is
-// already in its first-sweep buffer (written by the parent's restriction)
+// already in its first-sweep buffer (written by the parent's restriction);
+// pt: the parent's p-prolongation, done here when the last sweep is fused
 template <typename T>
 static int vcycle(const MG& mg, MGWork& w, int l, const void* b, void* x, const int* skip,
-                  hipStream_t s, bool first_done = false) {
+                  hipStream_t s, bool first_done = false, ProlongTo* pt = nullptr) {
   if (l == mg.nlev - 1) return coarse_solve<T>(mg, w, b, x, skip, s);
   const MGLevel& L = mg.lv[l];
   const MGLevel& C = mg.lv[l + 1];
@@ -376,16 +387,23 @@ static int vcycle(const MG& mg, MGWork& w, int l, const void* b, void* x, const 
     } else {
       MG_CHECK(rr);
     }
-    MG_CHECK(vcycle<T>(mg, w, l + 1, Cb, Cx, skip, s, fuse_first));
+    // a p-prolongation is added to x by the coarser level's last fused sweep
+    ProlongTo pc{x, L.P, op.Np, false};
+    MG_CHECK(vcycle<T>(mg, w, l + 1, Cb, Cx, skip, s, fuse_first, L.tkind == 0 ? &pc : nullptr));
     // the h-prolongation into an order-1 level is formed inside the first
     // post-smoothing sweep (which never stores the corrected iterate)
     const bool fuse_prolong = mg.nu >= 2 && L.tkind == 1 && smooth_prolongs(op);
-    if (!fuse_prolong) MG_CHECK(transfer<T>(L, C, true, Cx, x, skip, s));
+    if (!fuse_prolong && !pc.done) MG_CHECK(transfer<T>(L, C, true, Cx, x, skip, s));
     for (int i = 0; i + 1 < mg.nu; i += 2) {
       const bool pf = fuse_prolong && i == 0;
       MG_CHECK(launch_smooth(op, L.binv, mg.omega, mg.alpha, x, Lr, b, skip, s,
                              pf ? Cx : nullptr, pf ? L.P : nullptr));
-      MG_CHECK(launch_smooth(op, L.binv, mg.omega, mg.alpha, Lr, x, b, skip, s));
+      // the last sweep of an even count takes the parent's p-prolongation
+      const bool po = pt != nullptr && i + 2 == mg.nu;
+      MG_CHECK(launch_smooth(op, L.binv, mg.omega, mg.alpha, Lr, x, b, skip, s, nullptr,
+                             nullptr, po ? pt->x : nullptr, po ? pt->P : nullptr,
+                             po ? pt->np : 0));
+      if (po) pt->done = true;
     }
     if (mg.nu % 2) {
       MG_CHECK(resid());

@@ -28,7 +28,7 @@ template <typename T, int N, int MODE>
 static int launch_t(const Op& op, const void* u, void* y, const void* udir, double alpha,
                     double shift, double* partial, const int* skip, hipStream_t s,
                     const void* bvec, const void* binv = nullptr, double omega = 0.0,
-                    const void* pxc = nullptr, const void* pmat = nullptr, int np_lo = 0,
+                    const void* pxc = nullptr, const void* pmat = nullptr, int npt = 0,
                     void* xco = nullptr) {
   // one workgroup per 64-element tile; in dot mode each writes one partial
   const int* perm = op.perm_id ? nullptr : op.perm;
@@ -58,19 +58,19 @@ static int launch_t(const Op& op, const void* u, void* y, const void* udir, doub
                        s, (const T*)op.tab, (const T*)op.rec, op.nbrp, perm, (const T*)u, (T*)y,
                        (const T*)udir, (T)alpha, (T)shift, (int)op.K, partial, skip,
                        (const T*)bvec, (const T*)binv, (T)omega, (const T*)pxc, (const T*)pmat,
-                       (T*)xco);
+                       (T*)xco, npt);
   else if (op.halo_cap == HALO_SMALL)
     hipLaunchKernelGGL((sipg_apply_mfma_kernel<T, N, MODE, HALO_SMALL>), dim3(op.grid_apply),
                        dim3(256), 0, s, (const T*)op.frag, (const T*)op.rec, op.fcode, perm,
                        op.hpos, (const T*)op.hab, (const T*)u, (T*)y, (const T*)udir, (T)alpha,
                        (T)shift, (int)op.K, partial, skip, (const T*)bvec, (const T*)binv,
-                       (T)omega, (const T*)pmat, np_lo, (T*)xco);
+                       (T)omega, (const T*)pmat, npt, (T*)xco);
   else
     hipLaunchKernelGGL((sipg_apply_mfma_kernel<T, N, MODE, HALO_MAX>), dim3(op.grid_apply),
                        dim3(256), 0, s, (const T*)op.frag, (const T*)op.rec, op.fcode, perm,
                        op.hpos, (const T*)op.hab, (const T*)u, (T*)y, (const T*)udir, (T)alpha,
                        (T)shift, (int)op.K, partial, skip, (const T*)bvec, (const T*)binv,
-                       (T)omega, (const T*)pmat, np_lo, (T*)xco);
+                       (T)omega, (const T*)pmat, npt, (T*)xco);
   DDGX_HIP(hipGetLastError());
   return DDGX_OK;
 }
@@ -171,7 +171,15 @@ bool smooth_prolongs(const Op& op) { return op.cv == nullptr && op.N == 1; }
 template <typename T, int N>
 static int smooth_n(const Op& op, const void* binv, double omega, double alpha, const void* x_in,
                     void* x_out, const void* b, const int* skip, hipStream_t s, const void* pxc,
-                    const void* pmat) {
+                    const void* pmat, void* pto, const void* ptm, int np_to) {
+  if (pto != nullptr) {
+    constexpr int M = M_SMOOTH | M_PROLONG_OUT;
+    if (alpha != 0.0)
+      return launch_t<T, N, M | M_HELM>(op, x_in, x_out, nullptr, alpha, 0.0, nullptr, skip, s,
+                                         b, binv, omega, nullptr, ptm, np_to, pto);
+    return launch_t<T, N, M>(op, x_in, x_out, nullptr, alpha, 0.0, nullptr, skip, s, b, binv,
+                             omega, nullptr, ptm, np_to, pto);
+  }
   if constexpr (N == 1) {
     if (pxc != nullptr) {
       if (alpha != 0.0)
@@ -191,10 +199,16 @@ static int smooth_n(const Op& op, const void* binv, double omega, double alpha, 
 
 int launch_smooth(const Op& op, const void* binv, double omega, double alpha, const void* x_in,
                   void* x_out, const void* b, const int* skip, hipStream_t s,
-                  const void* prolong_from, const void* prolong_mat) {
+                  const void* prolong_from, const void* prolong_mat, void* prolong_to,
+                  const void* prolong_to_mat, int np_to) {
   if (prolong_from != nullptr && (!smooth_prolongs(op) || prolong_mat == nullptr)) {
     set_error("fused smoothing sweep: prolongation on the fly is built for order 1 only");
     return DDGX_ERR_UNSUPPORTED;
+  }
+  if (prolong_to != nullptr &&
+      (prolong_from != nullptr || prolong_to_mat == nullptr || np_to < op.Np || np_to > 28)) {
+    set_error("fused smoothing sweep: bad prolongation into the finer level");
+    return DDGX_ERR_INVALID;
   }
   if (!smooth_fused(op)) {
     set_error("fused smoothing sweep: polynomial order not built");
@@ -206,15 +220,18 @@ int launch_smooth(const Op& op, const void* binv, double omega, double alpha, co
   }
 #define SMOOTH_N(T)                                                                   \
   switch (op.N) {                                                                     \
-    case 1: return smooth_n<T, 1>(op, binv, omega, alpha, x_in, x_out, b, skip, s, PF, PM); \
-    case 2: return smooth_n<T, 2>(op, binv, omega, alpha, x_in, x_out, b, skip, s, PF, PM); \
-    case 3: return smooth_n<T, 3>(op, binv, omega, alpha, x_in, x_out, b, skip, s, PF, PM); \
-    case 4: return smooth_n<T, 4>(op, binv, omega, alpha, x_in, x_out, b, skip, s, PF, PM); \
-    case 5: return smooth_n<T, 5>(op, binv, omega, alpha, x_in, x_out, b, skip, s, PF, PM); \
-    case 6: return smooth_n<T, 6>(op, binv, omega, alpha, x_in, x_out, b, skip, s, PF, PM); \
+    case 1: return smooth_n<T, 1>(op, binv, omega, alpha, x_in, x_out, b, skip, s, PF, PM, PT, PTM, NPT); \
+    case 2: return smooth_n<T, 2>(op, binv, omega, alpha, x_in, x_out, b, skip, s, PF, PM, PT, PTM, NPT); \
+    case 3: return smooth_n<T, 3>(op, binv, omega, alpha, x_in, x_out, b, skip, s, PF, PM, PT, PTM, NPT); \
+    case 4: return smooth_n<T, 4>(op, binv, omega, alpha, x_in, x_out, b, skip, s, PF, PM, PT, PTM, NPT); \
+    case 5: return smooth_n<T, 5>(op, binv, omega, alpha, x_in, x_out, b, skip, s, PF, PM, PT, PTM, NPT); \
+    case 6: return smooth_n<T, 6>(op, binv, omega, alpha, x_in, x_out, b, skip, s, PF, PM, PT, PTM, NPT); \
   }
   const void* PF = prolong_from;
   const void* PM = prolong_mat;
+  void* PT = prolong_to;
+  const void* PTM = prolong_to_mat;
+  const int NPT = np_to;
   if (op.dtype == DDGX_F64) {
     SMOOTH_N(double)
   } else {

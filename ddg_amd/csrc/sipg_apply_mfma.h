@@ -329,7 +329,7 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
                        T alpha, T shift, int K, double* __restrict__ partial,
                        const int* __restrict__ skip, const T* __restrict__ bvec,
                        const T* __restrict__ binv, T omega, const T* __restrict__ pmat,
-                       int np_lo, T* __restrict__ xco) {
+                       int npt, T* __restrict__ xco) {
   using Q = MS<N>;
   using MF = MfmaT<T>;
   using v4 = typename MF::v4;
@@ -350,6 +350,12 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
   // omega Binv_c (that) with binv the coarse level's packed blocks
   constexpr bool RESTRICT = MODE & M_RESTRICT;
   static_assert(!RESTRICT || (RESID && !SMOOTH && !DOT), "restriction of a residual");
+  // with SMOOTH: the sweep's result is also prolongated into the next-finer
+  // p-level's iterate, xco[k] += pmat x_new[k] (pmat: npt x Np)
+  constexpr bool PROLONG_OUT = MODE & M_PROLONG_OUT;
+  static_assert(!PROLONG_OUT || SMOOTH, "prolongation of a sweep's result");
+  // the transfer's other level: np_lo = npt coarse nodes (RESTRICT)
+  const int np_lo = npt;
   constexpr bool ROWS = SMOOTH || RESTRICT;  // epilogue on element rows in LDS
   constexpr int RECS = DDGX_REC_WORDS + 1;  // LDS stride of a record (odd: no bank conflicts)
   constexpr int XT = (TILE + HC) * SW;      // element slots: tile then halo
@@ -664,7 +670,23 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
         T a = 0;
 #pragma unroll
         for (int j = 0; j < Np; ++j) a += bk[SymPack<N>::pk(li, j)] * s_r[e2 * Np + j];
-        y[k2 * Np + li] = s_u[e2 * Np + li] + omega * a;
+        const T xn = s_u[e2 * Np + li] + omega * a;
+        y[k2 * Np + li] = xn;
+        // only this lane reads s_u[e2 * Np + li]: the new value takes its place
+        if constexpr (PROLONG_OUT) s_u[e2 * Np + li] = xn;
+      }
+    }
+    if constexpr (PROLONG_OUT) {
+      // one thread per (element, fine node): x_f += P x_new, element-local
+      __syncthreads();
+      for (int it = tid; it < ne * npt; it += 256) {
+        const int e2 = it / npt, i = it % npt;
+        const int kk = k0 + e2;
+        const size_t k2 = (size_t)(perm ? perm[kk] : kk);
+        T acc = 0;
+#pragma unroll
+        for (int l = 0; l < Np; ++l) acc += pmat[i * Np + l] * s_u[e2 * Np + l];
+        xco[k2 * npt + i] += acc;
       }
     }
   }

@@ -39,7 +39,7 @@ sipg_apply_valu_kernel(const T* __restrict__ tab, const T* __restrict__ rec,
                        T alpha, T shift, int K, double* __restrict__ partial,
                        const int* __restrict__ skip, const T* __restrict__ bvec,
                        const T* __restrict__ binv, T omega, const T* __restrict__ pxc,
-                       const T* __restrict__ pmat, T* __restrict__ xco) {
+                       const T* __restrict__ pmat, T* __restrict__ xco, int npt) {
   using S = Shape<N>;
   constexpr int Np = S::Np, Nfp = S::Nfp, RS = S::RS, RSF = S::RSF;
   constexpr bool HELM = MODE & M_HELM, UDIR = MODE & M_UDIR, DOT = MODE & M_DOT;
@@ -59,6 +59,10 @@ sipg_apply_valu_kernel(const T* __restrict__ tab, const T* __restrict__ rec,
   // consecutive tile positions of one tile).
   constexpr bool RESTRICT = MODE & M_RESTRICT;
   static_assert(!RESTRICT || (RESID && !SMOOTH && !DOT && !PROLONG), "restriction of a residual");
+  // with SMOOTH: the sweep's result is also prolongated into the next-finer
+  // p-level's iterate, xco[k] += pmat x_new[k] (pmat: npt x Np)
+  constexpr bool PROLONG_OUT = MODE & M_PROLONG_OUT;
+  static_assert(!PROLONG_OUT || (SMOOTH && !PROLONG), "prolongation of a sweep's result");
   auto uval = [&](size_t kid, int j) -> T {
     T v = u[kid * Np + j];
     if constexpr (PROLONG) {
@@ -263,6 +267,14 @@ sipg_apply_valu_kernel(const T* __restrict__ tab, const T* __restrict__ rec,
 #pragma unroll
         for (int j = 0; j < Np; ++j) acc += bk[SymPack<N>::pk(i, j)] * r[j];
         v[i] = uk[i] + omega * acc;
+      }
+      if constexpr (PROLONG_OUT) {
+        for (int i = 0; i < npt; ++i) {
+          T acc = 0;
+#pragma unroll
+          for (int l = 0; l < Np; ++l) acc += pmat[i * Np + l] * v[l];
+          xco[kg * npt + i] += acc;
+        }
       }
     }
   }

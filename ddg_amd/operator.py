@@ -300,7 +300,11 @@ class SipgOperator:
         tab = np.concatenate([ref.kernel_tables(),
                               mfma_fragments(ref, "f64" if dtype == torch.float64 else "f32")])
         L = _lib.lib()
-        assert tab.size == L.ddgx_table_size(self.N)
+        if tab.size != L.ddgx_table_size(self.N):
+            raise RuntimeError(
+                f"libddgx.so expects {L.ddgx_table_size(self.N)} table words at order {self.N}, "
+                f"the host built {tab.size}: the library and ddg_amd/reference.py disagree "
+                "(rebuild with python -m ddg_amd.build)")
         if isinstance(order, str):
             if order != "auto":
                 raise ValueError("order must be 'auto', None or a permutation of range(K)")

@@ -11,10 +11,11 @@
 // would be mostly padding; here every element's contractions are
 // unrolled scalar FMAs with the tables read through the scalar cache
 // (uniform addresses).  A workgroup is one wave and one 64-element tile
-// (same tiles, XCD mapping and dot partials as the matrix-core kernel):
-// the tile's nodal values are staged through LDS with contiguous loads and
-// stores, a neighbour inside the tile is read from LDS, one outside it from
-// global memory (cache hits mostly: tiles are compact patches of the mesh).
+// (same tiles, XCD mapping and dot partials as the matrix-core kernel).
+// A thread reads its element's values, record and face codes, then its three
+// neighbours' values (cache hits mostly: tiles are compact patches of the
+// mesh), each batch issued before its first wait; the results go through LDS
+// so the tile's rows are stored contiguously.
 
 #pragma once
 
@@ -94,28 +95,61 @@ sipg_apply_valu_kernel(const T* __restrict__ tab, const T* __restrict__ rec,
   const int k = k0 + el;
   const bool valid = el < ne;
 
-  // the tile's nodal values into LDS (one contiguous range when the tile
-  // order is the element order)
-  if constexpr (PROLONG) {
-    for (int i = el; i < ne * Np; i += TILE)
-      s_u[i] = uval((size_t)(perm ? perm[k0 + i / Np] : k0 + i / Np), i % Np);
-  } else if (perm == nullptr) {
-    const T* src = u + (size_t)k0 * Np;
-    for (int i = el; i < ne * Np; i += TILE) s_u[i] = src[i];
-  } else {
-    for (int i = el; i < ne * Np; i += TILE) s_u[i] = u[(size_t)perm[k0 + i / Np] * Np + i % Np];
+  // Loads in two batches, each issued before its first wait: this element's
+  // values, record and face codes (and the smoothing or residual data), then,
+  // once the codes are in, the three neighbours' values and face
+  // coefficients.  Addresses are clamped (a lane past the tile reads tile
+  // position k0, a boundary face its own element), so no load sits in a
+  // lane-divergent branch; a neighbour in the tile is read from memory like
+  // any other (a cache hit).
+  const int kc = valid ? k : k0;  // clamped tile position
+  const size_t kid = (size_t)(perm ? perm[kc] : kc);
+  T uk[Np];
+#pragma unroll
+  for (int j = 0; j < Np; ++j) uk[j] = uval(kid, j);
+  T rc[DDGX_REC_WORDS];
+#pragma unroll
+  for (int w = 0; w < DDGX_REC_WORDS; ++w) rc[w] = rec[(size_t)w * K + kc];
+  int cf[3];
+#pragma unroll
+  for (int f = 0; f < 3; ++f) cf[f] = nbrp[(size_t)f * K + kc];
+  constexpr int NPK = SymPack<N>::NPK;
+  T bvv[Np], bkk[NPK];  // used (and kept) by the residual and smoothing modes only
+  if constexpr (SMOOTH || RESID) {
+#pragma unroll
+    for (int i = 0; i < Np; ++i) bvv[i] = bvec[kid * Np + i];
   }
-  __syncthreads();
+  if constexpr (SMOOTH) {
+#pragma unroll
+    for (int q = 0; q < NPK; ++q) bkk[q] = binv[kid * NPK + q];
+  }
+  // the neighbours: tile positions, element ids (all three before any of
+  // their loads, so no load waits on another's branch), then values and
+  // face coefficients
+  T unb[3][Np], aPf[3], bPf[3];
+  int npf[3], fnf[3];
+  size_t ngf[3];
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    npf[f] = cf[f] >= 0 ? nbr_elem(cf[f]) : kc;
+    fnf[f] = cf[f] >= 0 ? nbr_face(cf[f]) : 0;
+    ngf[f] = (size_t)npf[f];
+  }
+  if (perm != nullptr) {  // uniform branch
+#pragma unroll
+    for (int f = 0; f < 3; ++f) ngf[f] = (size_t)perm[npf[f]];
+  }
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+#pragma unroll
+    for (int j = 0; j < Np; ++j) unb[f][j] = uval(ngf[f], j);
+    aPf[f] = rec[(size_t)(R_A + fnf[f]) * K + npf[f]];
+    bPf[f] = rec[(size_t)(R_B + fnf[f]) * K + npf[f]];
+  }
 
   double dacc = 0.0;
   T v[Np];
   if (valid) {
-    T uk[Np];
-#pragma unroll
-    for (int j = 0; j < Np; ++j) uk[j] = s_u[el * Np + j];
-    T rc[DDGX_REC_WORDS];
-#pragma unroll
-    for (int w = 0; w < DDGX_REC_WORDS; ++w) rc[w] = rec[(size_t)w * K + k];
     T ur[Np], us[Np];
 #pragma unroll
     for (int n = 0; n < Np; ++n) {
@@ -157,26 +191,17 @@ sipg_apply_valu_kernel(const T* __restrict__ tab, const T* __restrict__ rec,
     // faces
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
-      const int c = nbrp[(size_t)f * K + k];
+      const int c = cf[f];
       const T aM = rc[R_A + f], bM = rc[R_B + f];
       const T sjw = rc[R_SJW + f], pen = rc[R_PEN + f];
       T uP[Nfp], dnP[Nfp];
       if (c >= 0) {
         // interior: the neighbour's trace and normal derivative (its own
-        // face coefficients, so the sign flips to the own normal); a
-        // neighbour in the tile is read from LDS
-        const int np = nbr_elem(c), fn = nbr_face(c);
+        // face coefficients, so the sign flips to the own normal)
+        const int fn = nbr_face(c);
         const bool rv = nbr_rev(c);
-        T un[Np];
-        if (np >= k0 && np < k0 + ne) {
-#pragma unroll
-          for (int j = 0; j < Np; ++j) un[j] = s_u[(np - k0) * Np + j];
-        } else {
-          const size_t ng = (size_t)(perm ? perm[np] : np);
-#pragma unroll
-          for (int j = 0; j < Np; ++j) un[j] = uval(ng, j);
-        }
-        const T aP = rec[(size_t)(R_A + fn) * K + np], bP = rec[(size_t)(R_B + fn) * K + np];
+        const T* un = unb[f];
+        const T aP = aPf[f], bP = bPf[f];
         T urn[Np], usn[Np];
 #pragma unroll
         for (int n = 0; n < Np; ++n) {
@@ -206,11 +231,10 @@ sipg_apply_valu_kernel(const T* __restrict__ tab, const T* __restrict__ rec,
           dnP[p] = -(aP * dr + bP * ds);
         }
       } else if (c == CODE_DIRICHLET) {
-        const int kg = perm ? perm[k] : k;
 #pragma unroll
         for (int p = 0; p < Nfp; ++p) {
           if constexpr (UDIR)
-            uP[p] = udir[(size_t)kg * Np + fmask<N>(f, p)];
+            uP[p] = udir[kid * Np + fmask<N>(f, p)];
           else
             uP[p] = T(0);
           dnP[p] = T(0);
@@ -254,18 +278,15 @@ sipg_apply_valu_kernel(const T* __restrict__ tab, const T* __restrict__ rec,
       for (int i = 0; i < Np; ++i) dacc += (double)uk[i] * (double)v[i];
     }
     if constexpr (SMOOTH) {
-      constexpr int NPK = SymPack<N>::NPK;
-      const size_t kg = (size_t)(perm ? perm[k] : k);
-      T r[Np], bk[NPK];
+      const size_t kg = kid;
+      T r[Np];
 #pragma unroll
-      for (int i = 0; i < Np; ++i) r[i] = bvec[kg * Np + i] - v[i];
-#pragma unroll
-      for (int q = 0; q < NPK; ++q) bk[q] = binv[kg * NPK + q];
+      for (int i = 0; i < Np; ++i) r[i] = bvv[i] - v[i];
 #pragma unroll
       for (int i = 0; i < Np; ++i) {
         T acc = 0;
 #pragma unroll
-        for (int j = 0; j < Np; ++j) acc += bk[SymPack<N>::pk(i, j)] * r[j];
+        for (int j = 0; j < Np; ++j) acc += bkk[SymPack<N>::pk(i, j)] * r[j];
         v[i] = uk[i] + omega * acc;
       }
       if constexpr (PROLONG_OUT) {
@@ -278,58 +299,42 @@ sipg_apply_valu_kernel(const T* __restrict__ tab, const T* __restrict__ rec,
       }
     }
   }
-  // results through LDS (every thread has read its own and its in-tile
-  // neighbours' values before any is overwritten)
-  __syncthreads();
+  // results (the residual b - A u in the residual modes) through LDS, so the
+  // tile's rows are stored contiguously
   if (valid) {
 #pragma unroll
-    for (int i = 0; i < Np; ++i) s_u[el * Np + i] = v[i];
+    for (int i = 0; i < Np; ++i) s_u[el * Np + i] = (RESID && !SMOOTH) ? bvv[i] - v[i] : v[i];
   }
   __syncthreads();
   if constexpr (RESTRICT) {
-    // r = b - A u in place; then one thread per (coarse element, node)
-    const T* bb = bvec + (size_t)k0 * Np;
-    for (int i = el; i < ne * Np; i += TILE) s_u[i] = bb[i] - s_u[i];
-    __syncthreads();
+    // one thread per (coarse element, node) restricts the tile's residual
     const int cl = el / Np, j = el % Np;
     const bool on = cl < ne / 4;
-    const size_t kc = (size_t)(k0 / 4 + cl);
+    const size_t kcs = (size_t)(k0 / 4 + cl);
     T lo = 0;
     if (on) {
 #pragma unroll
       for (int h = 0; h < 4 * Np; ++h) lo += pmat[h * Np + j] * s_u[4 * cl * Np + h];
-      y[kc * Np + j] = lo;
+      y[kcs * Np + j] = lo;
     }
     if (xco != nullptr) {
-      constexpr int NPK = SymPack<N>::NPK;
       __syncthreads();  // every thread has read the residual rows
       if (on) s_u[el] = lo;
       __syncthreads();
       if (on) {
         T a = 0;
 #pragma unroll
-        for (int i = 0; i < Np; ++i) a += binv[kc * NPK + SymPack<N>::pk(j, i)] * s_u[cl * Np + i];
-        xco[kc * Np + j] = omega * a;
+        for (int i = 0; i < Np; ++i) a += binv[kcs * NPK + SymPack<N>::pk(j, i)] * s_u[cl * Np + i];
+        xco[kcs * Np + j] = omega * a;
       }
     }
     return;
   }
   if (perm == nullptr) {
     T* dst = y + (size_t)k0 * Np;
-    if constexpr (RESID && !SMOOTH) {
-      const T* bb = bvec + (size_t)k0 * Np;
-      for (int i = el; i < ne * Np; i += TILE) dst[i] = bb[i] - s_u[i];
-    } else {
-      for (int i = el; i < ne * Np; i += TILE) dst[i] = s_u[i];
-    }
+    for (int i = el; i < ne * Np; i += TILE) dst[i] = s_u[i];
   } else {
-    for (int i = el; i < ne * Np; i += TILE) {
-      const size_t gi = (size_t)perm[k0 + i / Np] * Np + i % Np;
-      if constexpr (RESID && !SMOOTH)
-        y[gi] = bvec[gi] - s_u[i];
-      else
-        y[gi] = s_u[i];
-    }
+    for (int i = el; i < ne * Np; i += TILE) y[(size_t)perm[k0 + i / Np] * Np + i % Np] = s_u[i];
   }
   if constexpr (DOT) {
     for (int off = 32; off > 0; off >>= 1) dacc += __shfl_down(dacc, off, 64);

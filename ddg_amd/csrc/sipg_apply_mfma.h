@@ -641,6 +641,20 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
     }
   }
 
+  // the float32 right-hand side of the row epilogues, every value loaded
+  // (clamped) before the first is used: one memory round trip, not one per
+  // register
+  T BR[NB][4];
+  if constexpr (ROWS && !NEEDB) {
+#pragma unroll
+    for (int bo = 0; bo < NB; ++bo)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = RowMap<T, N>::node_here(bo, g, r);
+        BR[bo][r] = bvec[(size_t)kg * Np + (n < Np ? n : Np - 1)];
+      }
+  }
+
   if constexpr (SMOOTH) {
     // the residual r = b - A u and u, by element rows, into LDS; then one
     // block-Jacobi row per lane (Np of NL lanes per element): the same
@@ -655,7 +669,7 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
       for (int r = 0; r < 4; ++r) {
         const int n = RowMap<T, N>::node_here(bo, g, r);
         if (valid && n < Np) {
-          s_r[el * Np + n] = (NEEDB ? BV[bo][r] : bvec[(size_t)kg * Np + n]) - vv[bo][r];
+          s_r[el * Np + n] = (NEEDB ? BV[bo][r] : BR[bo][r]) - vv[bo][r];
           s_u[el * Np + n] = U[bo][r];
         }
       }
@@ -701,8 +715,7 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int n = RowMap<T, N>::node_here(bo, g, r);
-        if (valid && n < Np)
-          s_r[el * Np + n] = (NEEDB ? BV[bo][r] : bvec[(size_t)kg * Np + n]) - vv[bo][r];
+        if (valid && n < Np) s_r[el * Np + n] = (NEEDB ? BV[bo][r] : BR[bo][r]) - vv[bo][r];
       }
     __syncthreads();
     for (int it = tid; it < ne * np_lo; it += 256) {

@@ -91,44 +91,44 @@ curved_side_kernel(const T* __restrict__ tab, const T* __restrict__ fnorm,
   const T* m1d = tab + 3 * Np * Np;
   const int ls = threadIdx.x / Nfp, p = threadIdx.x - (threadIdx.x / Nfp) * Nfp;
   for (int64_t s0 = (int64_t)blockIdx.x * SPB; s0 < nside; s0 += (int64_t)gridDim.x * SPB) {
-    const int64_t t = s0 + ls;
-    const bool act = ls < SPB && t < nside;
-    int c = CODE_NONE;
-    T w = 0, wtau = 0, sj0 = 0, nx = 0, ny = 0;
-    __syncthreads();
+    const int64_t t0 = s0 + ls;
+    const bool act = ls < SPB && t0 < nside;
+    // every load of the side from clamped indices, in two batches (the
+    // side's own data, then the neighbour's trace at the node its code
+    // names), each issued before its first use
+    const int64_t t = act ? t0 : 0;
+    const int64_t k = t / 3;
+    const int f = (int)(t - k * 3);
+    const int64_t at = t * Nfp + p;
+    const int c = codes[t];
+    const T w = fpar[t * 3], wtau = fpar[t * 3 + 1], sj0 = fpar[t * 3 + 2];
+    const T nx = fnorm[at * 2], ny = fnorm[at * 2 + 1];
+    const T uM = tr[at];
+    const T dM = nx * tr[nf + at] + ny * tr[2 * nf + at];
+    T m[Nfp];  // the side's face-mass column p
+#pragma unroll
+    for (int q = 0; q < Nfp; ++q)
+      m[q] = fmass != nullptr ? fmass[t * Nfp * Nfp + q * Nfp + p]
+                              : sj0 * m1d[f * Nfp * Nfp + q * Nfp + p];
+    const int64_t o = c >= 0 ? ((int64_t)nbr_elem(c) * 3 + nbr_face(c)) * Nfp +
+                                   (nbr_rev(c) ? Nfp - 1 - p : p)
+                             : at;
+    const T uN = tr[o];
+    const T dN = nx * tr[nf + o] + ny * tr[2 * nf + o];
+    // Dirichlet: ghost u_dir (or 0), ghost gradient = own
+    const T uD = udir != nullptr ? udir[k * Np + fmask<N>(f, p)] : T(0);
+    const bool on = act && c != CODE_NONE && w != T(0);
+    const T uP = c >= 0 ? uN : uD;
+    const T dP = c >= 0 ? dN : dM;
+    const T ju = on ? uM - uP : T(0);
+    const T av = on ? T(0.5) * (dM + dP) : T(0);
+    __syncthreads();  // the previous group's LDS reads are done
     if (act) {
-      const int64_t k = t / 3;
-      const int f = (int)(t - k * 3);
-      const int64_t at = t * Nfp + p;
-      c = codes[t];
-      w = fpar[t * 3];
-      wtau = fpar[t * 3 + 1];
-      sj0 = fpar[t * 3 + 2];
-      nx = fnorm[at * 2];
-      ny = fnorm[at * 2 + 1];
-      T ju = 0, av = 0;
-      if (c != CODE_NONE && w != T(0)) {
-        const T uM = tr[at];
-        const T dM = nx * tr[nf + at] + ny * tr[2 * nf + at];
-        T uP, dP;
-        if (c >= 0) {
-          const int64_t o = ((int64_t)nbr_elem(c) * 3 + nbr_face(c)) * Nfp +
-                            (nbr_rev(c) ? Nfp - 1 - p : p);
-          uP = tr[o];
-          dP = nx * tr[nf + o] + ny * tr[2 * nf + o];
-        } else {  // Dirichlet: ghost u_dir (or 0), ghost gradient = own
-          uP = udir != nullptr ? udir[k * Np + fmask<N>(f, p)] : T(0);
-          dP = dM;
-        }
-        ju = uM - uP;
-        av = T(0.5) * (dM + dP);
-      }
       s_ju[ls * Nfp + p] = ju;
       s_av[ls * Nfp + p] = av;
     }
     __syncthreads();
     if (!act) continue;
-    const int64_t at = t * Nfp + p;
     if (c == CODE_NONE || w == T(0)) {
       for (int q = 0; q < 3; ++q) {
         cown[q * nf + at] = 0;
@@ -136,14 +136,11 @@ curved_side_kernel(const T* __restrict__ tab, const T* __restrict__ fnorm,
       }
       continue;
     }
-    const int f = (int)(t % 3);
     T mj = 0, ma = 0;
 #pragma unroll
     for (int q = 0; q < Nfp; ++q) {
-      const T m = fmass != nullptr ? fmass[t * Nfp * Nfp + q * Nfp + p]
-                                   : sj0 * m1d[f * Nfp * Nfp + q * Nfp + p];
-      mj += m * s_ju[ls * Nfp + q];
-      ma += m * s_av[ls * Nfp + q];
+      mj += m[q] * s_ju[ls * Nfp + q];
+      ma += m[q] * s_av[ls * Nfp + q];
     }
     const bool inner = c >= 0;
     const T gjv = wtau * mj - w * ma;  // d/d[v]
@@ -175,6 +172,12 @@ __device__ inline double block_sum_256c(double v, double* s_red) {
 // then Dr, Ds, M for the workgroup's lifetime.
 __host__ __device__ constexpr int curved_elem_words(int Np, bool cub, bool mass) {
   return cub ? 10 * Np + (Np * (Np + 1) / 2) * (mass ? 2 : 1) : 13 * Np;
+}
+
+// Elements per group of the element kernel: its LDS plan within 4096 words.
+__host__ __device__ constexpr int curved_epb(int Np, bool cub, bool mass) {
+  const int e = 4096 / curved_elem_words(Np, cub, mass);
+  return e < 1 ? 1 : (e > 64 ? 64 : e);
 }
 
 template <typename T, int N, bool CUB>
@@ -209,49 +212,110 @@ curved_elem_kernel(const T* __restrict__ tab, const T* __restrict__ geo,
   const int64_t nf = K * 3 * Nfp;
   const int tid = threadIdx.x;
   const int64_t ngroups = (K + epb - 1) / epb;
+  // The group's staging loads are issued (from clamped indices) before the
+  // first LDS store, in compile-time-sized batches for the largest group
+  // this kernel is launched with, and the three faces' cotangents in one
+  // batch after their codes: a group pays a few memory round trips, not
+  // one per staging pass and face.
+  constexpr int EMAX = curved_epb(Np, CUB, false);
+  constexpr int IU = (EMAX * Np + 255) / 256, IG = (EMAX * GW * Np + 255) / 256;
+  constexpr int IS = CUB ? (EMAX * NN + 255) / 256 : 1;
+  static_assert(EMAX * Nfp <= 256, "one face node per thread and face");
   double dacc = 0.0;
   for (int64_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
     const int64_t k0 = g * epb;
     const int ne = (int)((K - k0) < epb ? (K - k0) : epb);
-    __syncthreads();
-    for (int idx = tid; idx < ne * Np; idx += 256) {
-      s_u[idx] = u != nullptr ? u[k0 * Np + idx] : T(0);
-      s_gv[idx] = 0;
-      s_gx[idx] = 0;
-      s_gy[idx] = 0;
+    T ru[IU], rg[IG], rs[IS], rq[IS];
+#pragma unroll
+    for (int j = 0; j < IU; ++j) {
+      const int idx = tid + 256 * j;
+      ru[j] = u != nullptr ? u[k0 * Np + (idx < ne * Np ? idx : ne * Np - 1)] : T(0);
     }
     // the group's metrics (the first GW planes of each element's five):
     // contiguous runs, coalesced loads
-    for (int idx = tid; idx < ne * GW * Np; idx += 256) {
-      const int e = idx / (GW * Np), r = idx - e * (GW * Np);
-      s_geo[idx] = geo[(k0 + e) * 5 * Np + r];
+#pragma unroll
+    for (int j = 0; j < IG; ++j) {
+      const int idx = tid + 256 * j;
+      const int ic = idx < ne * GW * Np ? idx : ne * GW * Np - 1;
+      const int e = ic / (GW * Np), r = ic - e * (GW * Np);
+      rg[j] = geo[(k0 + e) * 5 * Np + r];
     }
     if constexpr (CUB) {
       // the group's element matrices are contiguous: coalesced loads
-      for (int idx = tid; idx < ne * NN; idx += 256) s_S[idx] = estiff[k0 * NN + idx];
-      if (mass)
-        for (int idx = tid; idx < ne * NN; idx += 256) s_Q[idx] = emass[k0 * NN + idx];
+#pragma unroll
+      for (int j = 0; j < IS; ++j) {
+        const int idx = tid + 256 * j, ic = idx < ne * NN ? idx : ne * NN - 1;
+        rs[j] = estiff[k0 * NN + ic];
+        rq[j] = mass ? emass[k0 * NN + ic] : T(0);
+      }
+    }
+    // face cotangents of this thread's face node (e, p) on each face: its
+    // own side's, plus the neighbour side's for an interior face
+    const bool fon = tid < ne * Nfp;
+    const int fe = fon ? tid / Nfp : 0, fpn = fon ? tid - (tid / Nfp) * Nfp : 0;
+    T cv[3][3];
+    int64_t co[3];
+    int cc[3];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      const int64_t side = (k0 + fe) * 3 + f;
+      const int64_t t = side * Nfp + fpn;
+      cv[f][0] = cown[t];
+      cv[f][1] = cown[nf + t];
+      cv[f][2] = cown[2 * nf + t];
+      cc[f] = codes[side];
+    }
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      const int c = cc[f];
+      co[f] = c >= 0 ? ((int64_t)nbr_elem(c) * 3 + nbr_face(c)) * Nfp +
+                           (nbr_rev(c) ? Nfp - 1 - fpn : fpn)
+                     : 0;
+    }
+    T cn[3][3];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      cn[f][0] = cnbr[co[f]];
+      cn[f][1] = cnbr[nf + co[f]];
+      cn[f][2] = cnbr[2 * nf + co[f]];
+    }
+    __syncthreads();  // the previous group's LDS reads are done
+#pragma unroll
+    for (int j = 0; j < IU; ++j) {
+      const int idx = tid + 256 * j;
+      if (idx < ne * Np) {
+        s_u[idx] = ru[j];
+        s_gv[idx] = 0;
+        s_gx[idx] = 0;
+        s_gy[idx] = 0;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < IG; ++j) {
+      const int idx = tid + 256 * j;
+      if (idx < ne * GW * Np) s_geo[idx] = rg[j];
+    }
+    if constexpr (CUB) {
+#pragma unroll
+      for (int j = 0; j < IS; ++j) {
+        const int idx = tid + 256 * j;
+        if (idx < ne * NN) {
+          s_S[idx] = rs[j];
+          if (mass) s_Q[idx] = rq[j];
+        }
+      }
     }
     __syncthreads();
-    // face cotangents, one face at a time (a vertex node lies on two faces)
+    // face cotangents into the element rows, one face at a time (a vertex
+    // node lies on two faces)
+#pragma unroll
     for (int f = 0; f < 3; ++f) {
-      for (int idx = tid; idx < ne * Nfp; idx += 256) {
-        const int e = idx / Nfp, p = idx - (idx / Nfp) * Nfp;
-        const int64_t side = (k0 + e) * 3 + f;
-        const int64_t t = side * Nfp + p;
-        T v0 = cown[t], v1 = cown[nf + t], v2 = cown[2 * nf + t];
-        const int c = codes[side];
-        if (c >= 0) {
-          const int64_t o = ((int64_t)nbr_elem(c) * 3 + nbr_face(c)) * Nfp +
-                            (nbr_rev(c) ? Nfp - 1 - p : p);
-          v0 += cnbr[o];
-          v1 += cnbr[nf + o];
-          v2 += cnbr[2 * nf + o];
-        }
-        const int i = e * Np + fmask<N>(f, p);
-        s_gv[i] += v0;
-        s_gx[i] += v1;
-        s_gy[i] += v2;
+      if (fon) {
+        const bool in = cc[f] >= 0;
+        const int i = fe * Np + fmask<N>(f, fpn);
+        s_gv[i] += cv[f][0] + (in ? cn[f][0] : T(0));
+        s_gx[i] += cv[f][1] + (in ? cn[f][1] : T(0));
+        s_gy[i] += cv[f][2] + (in ? cn[f][2] : T(0));
       }
       __syncthreads();
     }
@@ -454,9 +518,7 @@ int curved_create(Op& op, int use_cub, const double* reftab, const double* geo,
   // element) within 4096 words
   // elements per group of the element kernel: its LDS plan within 4096 words
   // (the largest plan here; each launch sizes its groups to its own plan)
-  const int words = curved_elem_words(Np, use_cub != 0, use_cub != 0);
-  int epb = 4096 / words;
-  epb = epb < 1 ? 1 : (epb > 64 ? 64 : epb);
+  const int epb = curved_epb(Np, use_cub != 0, use_cub != 0);
   cv->epb = epb;
   const int64_t cap = 4 * (int64_t)device_cu_count();
   auto grid = [&](int64_t items) {
@@ -493,8 +555,7 @@ static int curved_t(const Op& op, const void* u, void* y, const void* udir, doub
   // group size for this launch's LDS plan (the grid, and so the number of dot
   // partials, is the operator's: idle workgroups write a zero partial)
   const int words = curved_elem_words(op.Np, cub, alpha != 0.0);
-  int epb = 4096 / words;
-  epb = epb < 1 ? 1 : (epb > 64 ? 64 : epb);
+  const int epb = curved_epb(op.Np, cub, alpha != 0.0);
   const size_t lds = ((size_t)epb * words + 3 * (size_t)op.Np * op.Np) * sizeof(T);
   if (cub)
     hipLaunchKernelGGL((curved_elem_kernel<T, N, true>), dim3(c.grid_elem), dim3(256), lds, s,

@@ -67,12 +67,26 @@ struct BlockWords {
 };
 
 // Inverse blocks of the EPB elements of a tile into LDS (contiguous loads).
+// Every load is issued (from a clamped index) before the first LDS store,
+// so a tile pays one memory round trip for its blocks, not one per pass.
 template <typename T, int N, bool FULL = false>
 __device__ __forceinline__ void stage_blocks(const T* __restrict__ binv, T* s_b, int k0, int ne,
                                              int tid) {
   constexpr int NW = BlockWords<N, FULL>::value;
+  constexpr int IT = (Shape<N>::EPB * NW + 255) / 256;  // loads per thread
   const T* src = binv + (size_t)k0 * NW;
-  for (int i = tid; i < ne * NW; i += 256) s_b[i] = src[i];
+  const int n = ne * NW;
+  T v[IT];
+#pragma unroll
+  for (int j = 0; j < IT; ++j) {
+    const int i = tid + 256 * j;
+    v[j] = src[i < n ? i : n - 1];
+  }
+#pragma unroll
+  for (int j = 0; j < IT; ++j) {
+    const int i = tid + 256 * j;
+    if (i < n) s_b[i] = v[j];
+  }
 }
 
 // Row li of B_e r_e from the staged block and the staged vector.

@@ -126,8 +126,6 @@ int op_replicate(const Op& src, int nseg, hipStream_t s, Op** out) {
     if ((rc = codes_planes(o->perm, src.perm, 1, K1, (int)K1, nseg, s))) break;
     if ((rc = codes_planes(o->hpos, src.hpos, 1, T1 * src.halo_cap, (int)K1, nseg, s))) break;
     o->grid_apply = (int)(T1 * nseg);
-    o->grid_pipe = src.grid_pipe > 0 ? apply_pipe_grid(src.N, src.dtype, src.halo_cap, T1 * nseg)
-                                     : 0;
     const int64_t gv = (KB * src.Np + 255) / 256;
     const int64_t cap = 4 * (int64_t)device_cu_count();
     o->grid_vec = (int)(gv < cap ? gv : cap);

@@ -192,6 +192,26 @@ int ddgx_op_create(int N, int64_t K, int dtype, const double* tables, const doub
   }
   // the smaller-LDS kernel variant when every halo fits its capacity
   const int hcap = max_halo <= HALO_SMALL ? HALO_SMALL : HALO_MAX;
+  // the neighbour read of every face as an LDS address (fc_nbase, fc_nstep):
+  // interior faces the neighbour's trace, Neumann faces their own, Dirichlet
+  // faces the zero slot after the halo (slot TILE + hcap); a slot holds u and
+  // dn at the 3 Nfp face nodes
+  {
+    const int Nfp = N + 1, SW = 6 * Nfp;
+    for (int64_t i = 0; i < K; ++i) {
+      const int own = (int)(i % TILE);
+      for (int f = 0; f < 3; ++f) {
+        int& c = fcode[(size_t)f * K + i];
+        const int ty = fc_type(c);
+        if (ty == FT_INTERIOR)
+          c = fc_addr_pack(c, fc_slot(c), fc_face(c), fc_rev(c), SW, Nfp);
+        else if (ty == FT_DIRICHLET)
+          c = fc_addr_pack(c, TILE + hcap, 0, 0, SW, Nfp);
+        else
+          c = fc_addr_pack(c, own, f, 0, SW, Nfp);
+      }
+    }
+  }
   // the halo lists hold element ids (the tile positions mapped through
   // perm), so the apply loads a halo element's values without a dependent
   // load of perm
@@ -264,7 +284,6 @@ int ddgx_op_create(int N, int64_t K, int dtype, const double* tables, const doub
         !put(&o.hpos, hpos, "hpos"))
       break;
     o.grid_apply = (int)ntiles;
-    o.grid_pipe = apply_pipelined(N, dtype, perm_id) ? apply_pipe_grid(N, dtype, hcap, ntiles) : 0;
     const int64_t nvec = K * o.Np;
     int64_t gv = (nvec + 255) / 256;
     const int64_t cap = 4 * (int64_t)device_cu_count();

@@ -162,11 +162,20 @@ struct CGScalars {
 // k (0..K-1, 64 per tile) holds element perm[k].  Vectors (u, y, ...) keep
 // the caller's element order and are addressed through perm.
 constexpr int TILE = 64;      // elements per matrix-core workgroup
+constexpr int LDS_BYTES_PER_CU = 160 * 1024;  // gfx950
 constexpr int HALO_MAX = 64;  // out-of-tile neighbour elements per tile (largest capacity)
 constexpr int HALO_SMALL = 32;  // capacity of the smaller-LDS variant of the apply
 // Packed per-face code of the apply: bits 0-7 LDS slot of the neighbour
 // trace (0..63 in the tile, 64+h halo), bits 8-9 neighbour face, bit 10
 // reversed node order, bits 11-12 face type.
+//
+// Bits 13-25 and 26-28 give the same neighbour read as an address: the LDS
+// word of the trace the face reads as its neighbour's at face node p is
+// fc_nbase + p * fc_nstep (2 words per face node: u, dn).  An interior face
+// reads the neighbour's trace; a Neumann face reads its own, so the jump
+// u^- - u^+ and the flux sum dn^- - dn_nbr vanish exactly; a Dirichlet face
+// reads the zero slot after the halo (u^+ = 0, no neighbour flux).  The
+// apply's face rows then need no test of the face type.
 enum : int { FT_INTERIOR = 0, FT_DIRICHLET = 1, FT_NONE = 2 };
 __host__ __device__ inline int fc_slot(int c) { return c & 0xff; }
 __host__ __device__ inline int fc_face(int c) { return (c >> 8) & 3; }
@@ -174,6 +183,16 @@ __host__ __device__ inline int fc_rev(int c) { return (c >> 10) & 1; }
 __host__ __device__ inline int fc_type(int c) { return (c >> 11) & 3; }
 __host__ __device__ inline int fc_pack(int slot, int face, int rev, int type) {
   return slot | (face << 8) | (rev << 10) | (type << 11);
+}
+__host__ __device__ inline int fc_nbase(int c) { return (c >> 13) & 0x1fff; }
+__host__ __device__ inline int fc_nstep(int c) { return (int)((unsigned)c << 3) >> 29; }
+// c with the address fields of a read of slot `slot`, face `face` (node
+// order reversed when rev), for SW LDS words per element slot and Nfp nodes
+// per face
+__host__ __device__ inline int fc_addr_pack(int c, int slot, int face, int rev, int SW, int Nfp) {
+  const int base = slot * SW + 2 * face * Nfp + (rev ? 2 * (Nfp - 1) : 0);
+  const int step = rev ? -2 : 2;
+  return (c & 0x1fff) | (base << 13) | ((step & 7) << 26);
 }
 
 // General operator for curved (non-affine) elements: per-node metrics,
@@ -216,8 +235,6 @@ struct Op {
   void* hab = nullptr;   // ntiles x 6 x halo_cap halo face coefficients a_f, b_f (dtype)
   void* fsj = nullptr;   // K x 4 Neumann face Jacobians, tile order (dtype)
   int grid_apply = 0;    // workgroups of one apply: one per 64-element tile (= dot partials)
-  int grid_pipe = 0;     // grid of the pipelined kernel (sipg_apply_pipe.h) for the plain
-                         // and Helmholtz applies, or 0 when this operator does not use it
   int max_halo = 0;      // largest halo of a tile
   int halo_cap = 0;      // HALO_SMALL or HALO_MAX: halo entries stored per tile
   int grid_vec = 0;      // grid for vector kernels / reductions
@@ -335,10 +352,6 @@ int launch_apply(const Op& op, const void* u, void* y, const void* udir, double 
 // One fused block-Jacobi sweep x_out = x_in + omega Binv (b - (A + alpha M) x_in)
 // (x_out must not alias x_in; Binv packed, element order, the operator's dtype).
 bool smooth_fused(const Op& op);
-// Whether the pipelined apply serves this order/dtype/tile order, and its
-// grid (workgroups resident at once, at most one per tile).
-bool apply_pipelined(int N, int dtype, bool perm_id);
-int apply_pipe_grid(int N, int dtype, int halo_cap, int64_t ntiles);
 // With prolong_to != null (p-transfers, affine levels): the sweep's result is
 // also added, through prolong_to_mat (np_to x Np, row-major), into the
 // next-finer level's iterate prolong_to -- the prolongation x_f += P x of the

@@ -20,7 +20,6 @@
 #include "ddgx_internal.h"
 #include "sipg_apply_mfma.h"
 #include "sipg_apply_valu.h"
-#include "sipg_apply_pipe.h"
 
 namespace ddgx {
 
@@ -32,27 +31,6 @@ static int launch_t(const Op& op, const void* u, void* y, const void* udir, doub
                     void* xco = nullptr) {
   // one workgroup per 64-element tile; in dot mode each writes one partial
   const int* perm = op.perm_id ? nullptr : op.perm;
-  // the pipelined kernel serves the plain and Helmholtz applies; the dot,
-  // residual and smoothing modes carry more per-lane state and spill under
-  // its 3-wave register budget (scratch traffic waits in order on the
-  // prefetch), so they keep the one-tile kernel
-  if constexpr (!UseValuApply<T, N>::value && MS<N>::NB == 1 &&
-                !(MODE & (M_DOT | M_RESID | M_SMOOTH))) {
-    if (op.grid_pipe > 0) {
-      if (op.halo_cap == HALO_SMALL)
-        hipLaunchKernelGGL((sipg_apply_pipe_kernel<T, N, MODE, HALO_SMALL>), dim3(op.grid_pipe),
-                           dim3(256), 0, s, (const T*)op.frag, (const T*)op.rec, op.fcode, op.hpos,
-                           (const T*)op.hab, (const T*)u, (T*)y, (const T*)udir, (T)alpha,
-                           (T)shift, (int)op.K, partial, skip, (const T*)bvec);
-      else
-        hipLaunchKernelGGL((sipg_apply_pipe_kernel<T, N, MODE, HALO_MAX>), dim3(op.grid_pipe),
-                           dim3(256), 0, s, (const T*)op.frag, (const T*)op.rec, op.fcode, op.hpos,
-                           (const T*)op.hab, (const T*)u, (T*)y, (const T*)udir, (T)alpha,
-                           (T)shift, (int)op.K, partial, skip, (const T*)bvec);
-      DDGX_HIP(hipGetLastError());
-      return DDGX_OK;
-    }
-  }
   if constexpr (UseValuApply<T, N>::value)
     hipLaunchKernelGGL((sipg_apply_valu_kernel<T, N, MODE>), dim3(op.grid_apply), dim3(TILE), 0,
                        s, (const T*)op.tab, (const T*)op.rec, op.nbrp, perm, (const T*)u, (T*)y,
@@ -119,44 +97,6 @@ int launch_apply(const Op& op, const void* u, void* y, const void* udir, double 
 #undef DISPATCH_N
   set_error("polynomial order " + std::to_string(op.N) + " not built (1..6)");
   return DDGX_ERR_UNSUPPORTED;
-}
-
-bool apply_pipelined(int N, int dtype, bool perm_id) {
-  return perm_id && dtype == DDGX_F64 && N >= 2 && N <= 4;
-}
-
-template <typename T, int N, int HC>
-static int pipe_occupancy() {
-  int best = 1 << 30;
-  auto one = [&](const void* f) {
-    int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 256, 0) == hipSuccess && nb > 0)
-      best = nb < best ? nb : best;
-  };
-  one((const void*)sipg_apply_pipe_kernel<T, N, 0, HC>);
-  one((const void*)sipg_apply_pipe_kernel<T, N, M_HELM, HC>);
-  one((const void*)sipg_apply_pipe_kernel<T, N, M_UDIR, HC>);
-  one((const void*)sipg_apply_pipe_kernel<T, N, M_HELM | M_UDIR, HC>);
-  return best == (1 << 30) ? 1 : best;
-}
-
-int apply_pipe_grid(int N, int dtype, int halo_cap, int64_t ntiles) {
-  int occ = 1;
-  const bool small = halo_cap == HALO_SMALL;
-#define OCC(T, NN)                                                                   \
-  occ = small ? pipe_occupancy<T, NN, HALO_SMALL>() : pipe_occupancy<T, NN, HALO_MAX>()
-  if (dtype == DDGX_F64) {
-    if (N == 2) OCC(double, 2);
-    else if (N == 3) OCC(double, 3);
-    else OCC(double, 4);
-  } else {
-    if (N == 2) OCC(float, 2);
-    else if (N == 3) OCC(float, 3);
-    else OCC(float, 4);
-  }
-#undef OCC
-  const int64_t g = (int64_t)occ * device_cu_count();
-  return (int)(g < ntiles ? g : ntiles);
 }
 
 // Fused block-Jacobi sweep of the multigrid smoother: the apply's smoothing

@@ -272,6 +272,21 @@ __device__ __forceinline__ void face_rows_inreg(T (&Fv)[12], const T* s_x, const
     const int j = f * Nfp + p;
     const T uM = s_x[el * SW + 2 * j], dnM = s_x[el * SW + 2 * j + 1];
     const int c = s_fc[f * TILE + el];
+    if constexpr (!UDIR) {
+      // the neighbour's trace at the face code's address: an interior
+      // face's neighbour (its dn taken with its own outward normal), a
+      // Neumann face's own trace (jump and flux sum exactly 0), a
+      // Dirichlet face's zero slot (u^+ = 0, no neighbour flux)
+      const T* xp = s_x + (fc_nbase(c) + p * fc_nstep(c));
+      const T uN = xp[0], dN = xp[1];
+      const T ju = uM - uN;
+      const T sj = re[R_SJW + f];
+      const T X = -sj * ju;
+      Fv[3 * s] = re[R_PEN + f] * ju - sj * (dnM - dN);
+      Fv[3 * s + 1] = re[R_A + f] * X;
+      Fv[3 * s + 2] = re[R_B + f] * X;
+      continue;
+    }
     const int ty = fc_type(c);
     const int pn = fc_rev(c) ? Nfp - 1 - p : p;
     // the neighbour's trace, read for every face (slot 0 for a boundary
@@ -358,7 +373,8 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
   const int np_lo = npt;
   constexpr bool ROWS = SMOOTH || RESTRICT;  // epilogue on element rows in LDS
   constexpr int RECS = DDGX_REC_WORDS + 1;  // LDS stride of a record (odd: no bank conflicts)
-  constexpr int XT = (TILE + HC) * SW;      // element slots: tile then halo
+  // element slots: tile, halo, then the zero slot Dirichlet faces read
+  constexpr int XT = (TILE + HC + 1) * SW;
   // face rows staged in LDS (two blocks), or the smoothing epilogue's rows
   constexpr int XF = Q::INREG ? (ROWS ? 2 * TILE * Np : 0) : TILE * Q::FS;
   constexpr int XW = XT > XF ? XT : XF;
@@ -371,6 +387,27 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
   __shared__ T s_hab[6 * HC];
   __shared__ int s_fc[3 * TILE];
   __shared__ double s_red[4];
+  // Fragments through LDS (one 16-row block): the workgroup loads each
+  // fragment once -- 256 threads, a quarter of the words each -- instead of
+  // every wave loading all of them, a quarter of the L2-to-CU traffic.
+  // GEMM 1's fragments have a region of their own.  GEMM 2's (and the mass
+  // fragments of the Helmholtz modes, which follow them in the table) have
+  // one too where the LDS still holds ApplyWaves workgroups per compute unit
+  // (A2OWN: float64 N = 2, float32 N = 2, 3); otherwise they are staged into
+  // s_x once the face rows have been formed and the traces there are no
+  // longer read, at the cost of two more barriers.
+  constexpr bool FLDS = Q::INREG;
+  constexpr int NA1 = 2 * NB * KB;                              // GEMM 1
+  constexpr int NA2 = NB * (2 * KB + FK) + (HELM ? NB * KB : 0);  // GEMM 2 (+ mass)
+  constexpr int LDS_OWN = (int)sizeof(T) * (XW + TILE * RECS + 6 * HC + (NA1 + NA2) * 64) +
+                          (int)sizeof(int) * 3 * TILE + (int)sizeof(double) * 4;
+  constexpr bool A2OWN = FLDS && (LDS_BYTES_PER_CU / LDS_OWN) >= ApplyWaves<T, N>::value;
+  constexpr int A1PT = FLDS ? NA1 / 4 : 1, A2PT = FLDS ? NA2 / 4 : 1;
+  static_assert(!FLDS || (NA1 % 4 == 0 && NA2 % 4 == 0 && NA2 * 64 <= XW),
+                "fragment staging split and fit");
+  static_assert(Q::fM == Q::f2 + NB * (2 * KB + FK), "mass fragments follow GEMM 2's");
+  __shared__ T s_a1[FLDS ? (A2OWN ? NA1 + NA2 : NA1) * 64 : 1];
+  T* s_a2 = A2OWN ? s_a1 + NA1 * 64 : s_x;
 
   if (skip != nullptr && *skip) return;
 
@@ -404,32 +441,25 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
   for (int b = 0; b < NB; ++b)
 #pragma unroll
     for (int r = 0; r < 4; ++r) nn[b][r] = RowMap<T, N>::node(b, g, r);
+  // Nodal values are loaded from clamped addresses and used unmasked: a
+  // padding position (n = Np) holds a finite copy of node Np - 1 and meets
+  // zero columns in every fragment table, and a lane past the tile's end
+  // computes a column of its own whose results are never stored.
   T U[NB][4];
   // the right-hand side of the float64 residual and smoothing modes, loaded
   // with u so the epilogue does not wait on a load issued after GEMM 2
   // (float32 reads it in the epilogue: it has no registers to spare)
   constexpr bool NEEDB = (RESID || SMOOTH) && sizeof(T) == 8;
   T BV[NB][4];  // unused (and dropped by the compiler) unless NEEDB
-  {
-    T uv[NB][4], bv[NB][4];
 #pragma unroll
-    for (int b = 0; b < NB; ++b)
+  for (int b = 0; b < NB; ++b)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int n = nn[b][r];
-        const size_t gi = (size_t)kg * Np + (n < Np ? n : Np - 1);
-        uv[b][r] = u[gi];
-        if constexpr (NEEDB) bv[b][r] = bvec[gi];
-      }
-#pragma unroll
-    for (int b = 0; b < NB; ++b)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const bool on = valid && nn[b][r] < Np;
-        U[b][r] = on ? uv[b][r] : T(0);
-        if constexpr (NEEDB) BV[b][r] = on ? bv[b][r] : T(0);
-      }
-  }
+    for (int r = 0; r < 4; ++r) {
+      const int n = nn[b][r];
+      const size_t gi = (size_t)kg * Np + (n < Np ? n : Np - 1);
+      U[b][r] = u[gi];
+      if constexpr (NEEDB) BV[b][r] = bvec[gi];
+    }
   // halo: the element ids of the tile's out-of-tile neighbours, 16 per
   // wave, listed from entry 0 (-1 past the end); a wave past the list skips
   // the halo work
@@ -451,17 +481,26 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
     const int i = 64 * (wv + 4 * j) + lane;
     hb[j] = hab[(size_t)tile * 6 * HC + (i < 6 * HC ? i : 0)];
   }
-  // GEMM 1 fragments, from one of the table's replicas (the many workgroups
-  // that start together spread these loads over L2 channels)
+  // fragments, from one of the table's replicas (the many workgroups that
+  // start together spread these loads over L2 channels): with FLDS a quarter
+  // of GEMM 1's and GEMM 2's words per thread, else GEMM 1's per lane
   const T* fr = frag + (size_t)(blockIdx.x % FRAG_COPIES) * Q::nfrag * 64;
   T A1[2 * NB][KB];
+  T a1v[A1PT], a2v[A2PT];
+  if constexpr (FLDS) {
 #pragma unroll
-  for (int b = 0; b < 2 * NB; ++b)
+    for (int j = 0; j < A1PT; ++j) a1v[j] = fr[(size_t)Q::f1 * 64 + tid + 256 * j];
 #pragma unroll
-    for (int t = 0; t < KB; ++t) A1[b][t] = fr[(size_t)(Q::f1 + b * KB + t) * 64 + lane];
-  // the halo's nodal values, once its element list has arrived; they are
-  // masked where the halo's GEMM 1 uses them, so no wait on these loads
-  // comes before the staging writes below
+    for (int j = 0; j < A2PT; ++j) a2v[j] = fr[(size_t)Q::f2 * 64 + tid + 256 * j];
+  } else {
+#pragma unroll
+    for (int b = 0; b < 2 * NB; ++b)
+#pragma unroll
+      for (int t = 0; t < KB; ++t) A1[b][t] = fr[(size_t)(Q::f1 + b * KB + t) * 64 + lane];
+  }
+  // the halo's nodal values, once its element list has arrived (a lane past
+  // the list loads element 0's); no wait on these loads comes before the
+  // staging writes below
   const int hk = el < HC ? hkv : -1;
   const bool hwave = __ballot(hk >= 0) != 0;  // uniform per wave
   T UH[NB][4] = {};
@@ -475,15 +514,32 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
         UH[b][r] = u[(size_t)hg * Np + (n < Np ? n : Np - 1)];
       }
   }
+  // records and codes past the tile's end are the clamped copies of its
+  // last element's, read only by the lanes past the end
 #pragma unroll
-  for (int j = 0; j < RPT; ++j) s_rec[lane * RECS + wv + 4 * j] = lane < ne ? rc[j] : T(0);
-  if (wv < 3) s_fc[tid] = lane < ne ? fcv : (FT_NONE << 11);
+  for (int j = 0; j < RPT; ++j) s_rec[lane * RECS + wv + 4 * j] = rc[j];
+  if (wv < 3) s_fc[tid] = fcv;
 #pragma unroll
   for (int j = 0; j < HPT; ++j) {
     const int i = tid + 256 * j;
     if (i < 6 * HC) s_hab[i] = hb[j];
   }
+  if (tid < SW) s_x[(TILE + HC) * SW + tid] = T(0);  // the zero slot
+  if constexpr (FLDS) {
+#pragma unroll
+    for (int j = 0; j < A1PT; ++j) s_a1[tid + 256 * j] = a1v[j];
+    if constexpr (A2OWN) {
+#pragma unroll
+      for (int j = 0; j < A2PT; ++j) s_a2[tid + 256 * j] = a2v[j];
+    }
+  }
   __syncthreads();
+  if constexpr (FLDS) {
+#pragma unroll
+    for (int b = 0; b < 2 * NB; ++b)
+#pragma unroll
+      for (int t = 0; t < KB; ++t) A1[b][t] = s_a1[(b * KB + t) * 64 + lane];
+  }
 
   // ---- GEMM 1: own elements and halo into LDS ----------------------------
   const T* re = s_rec + el * RECS;
@@ -492,17 +548,10 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
     const FaceAB<T> ab{re[R_A], re[R_A + 1], re[R_A + 2], re[R_B], re[R_B + 1], re[R_B + 2]};
     grad_and_traces<T, N>(A1, U, UR, US, ab, s_x + el * SW, g);
     if (hwave) {
+      // a lane past the halo list computes the gradient of element 0 (its
+      // clamped load) into a slot no face code names
       const FaceAB<T> abh{s_hab[el], s_hab[HC + el], s_hab[2 * HC + el],
                           s_hab[3 * HC + el], s_hab[4 * HC + el], s_hab[5 * HC + el]};
-#pragma unroll
-      for (int b = 0; b < NB; ++b)
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-        {
-          // both terms evaluated (no branch around the node computation)
-          const bool on = (hk >= 0) & (RowMap<T, N>::node_here(b, g, r) < Np);
-          UH[b][r] = on ? UH[b][r] : T(0);
-        }
       T HR[NB][4], HS[NB][4];
       grad_and_traces<T, N>(A1, UH, HR, HS, abh, s_x + (TILE + el) * SW, g);
     }
@@ -516,14 +565,17 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
       UR[b][r] = re[R_VRR] * a + re[R_VRS] * c;
       US[b][r] = re[R_VRS] * a + re[R_VSS] * c;
     }
-  // GEMM 2 fragments, issued here so their latency overlaps the face phase
+  // GEMM 2 fragments (two blocks), issued here so their latency overlaps
+  // the face phase
   T A2[NB][2 * KB + FK];
+  if constexpr (!FLDS) {
 #pragma unroll
-  for (int b = 0; b < NB; ++b) {
+    for (int b = 0; b < NB; ++b) {
 #pragma unroll
-    for (int t = 0; t < 2 * KB; ++t) A2[b][t] = fr[(size_t)(Q::f2 + b * 2 * KB + t) * 64 + lane];
+      for (int t = 0; t < 2 * KB; ++t) A2[b][t] = fr[(size_t)(Q::f2 + b * 2 * KB + t) * 64 + lane];
 #pragma unroll
-    for (int t = 0; t < FK; ++t) A2[b][2 * KB + t] = fr[(size_t)(Q::fF + b * FK + t) * 64 + lane];
+      for (int t = 0; t < FK; ++t) A2[b][2 * KB + t] = fr[(size_t)(Q::fF + b * FK + t) * 64 + lane];
+    }
   }
   __syncthreads();  // the tile's and the halo's traces are in LDS
 
@@ -531,6 +583,17 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
   T Fv[Q::INREG ? 12 : 1];
   if constexpr (Q::INREG) {
     face_rows_inreg<T, N, UDIR>(Fv, s_x, re, s_fc, el, g, udir, kg);
+    // GEMM 2's fragments into s_x once every wave has read its traces
+    if constexpr (FLDS && !A2OWN) {
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < A2PT; ++j) s_x[tid + 256 * j] = a2v[j];
+      __syncthreads();
+    }
+    if constexpr (FLDS) {
+#pragma unroll
+      for (int t = 0; t < 2 * KB + FK; ++t) A2[0][t] = s_a2[t * 64 + lane];
+    }
   } else {
     // thread -> one face node (f, p) of elements eg, eg + EG, ...
     constexpr int EG = 256 / FPW, ITE = TILE / EG;
@@ -621,7 +684,9 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
     if constexpr (HELM) {
 #pragma unroll
       for (int t = 0; t < KB; ++t)
-        acc = MF::mma(fr[(size_t)(Q::fM + bo * KB + t) * 64 + lane], Jm * U[t / 4][t % 4], acc);
+        acc = MF::mma(FLDS ? s_a2[(NB * (2 * KB + FK) + bo * KB + t) * 64 + lane]
+                           : fr[(size_t)(Q::fM + bo * KB + t) * 64 + lane],
+                      Jm * U[t / 4][t % 4], acc);
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {

@@ -114,8 +114,7 @@ def test_large_mesh_properties(dev):
     w = op.apply(2.0 * ut - 3.0 * vt)
     assert float((w - (2.0 * Au - 3.0 * Av)).abs().max()) <= 1e-11 * float(Au.abs().max())
     # Helmholtz with Dirichlet data (this row-major mesh runs in a Morton
-    # tile order, on the one-tile kernel; test_pipelined_forest_parity
-    # covers the pipelined kernel)
+    # tile order; test_forest_parity covers the identity order)
     ud = rng.standard_normal((m.K, m.Np))
     yh = op.apply(ut, u_dir=torch.as_tensor(ud.reshape(-1), device=dev), alpha=2.5, shift=0.5)
     refh = model_apply(4, ref_element(4).kernel_tables(), rec, nbr, u, udir=ud, alpha=2.5,
@@ -124,19 +123,19 @@ def test_large_mesh_properties(dev):
 
 
 @pytest.mark.parametrize("N", [2, 3, 4])
-def test_pipelined_forest_parity(dev, N):
-    """Forest-ordered 2x128x128 and 2x256x256 meshes (identity tile order, so
-    float64 orders 2-4 run the pipelined kernel with several tiles per
-    workgroup): plain, Helmholtz and Dirichlet-data applies against the numpy
-    model of the closed form, at the full face-major node layout."""
+def test_forest_parity(dev, N):
+    """Forest-ordered 2x128x128 and 2x256x256 meshes (identity tile order:
+    vectors addressed without a permutation): plain, Helmholtz and
+    Dirichlet-data applies against the numpy model of the closed form, at
+    the full face-major node layout."""
     for n, lv in ((128, 6), (256, 7)):
         VX, VY, EToV = pmesh.forest_rectangle_hierarchy(2, lv)[-1]
         VX, VY = pmesh.jitter_vertices(VX, VY, 1.0 / n, 0.1, seed=N)
         m = pmesh.build_mesh_2d(VX, VY, EToV, N)
         bc = pmesh.dirichlet_x_neumann_y(m)
-        _say(f"apply pipelined forest N={N} K={m.K}")
+        _say(f"apply forest N={N} K={m.K}")
         op = SipgOperator(m, bc)
-        assert op.order is None  # identity order: the pipelined kernel
+        assert op.order is None  # identity tile order
         rng = np.random.default_rng(100 + N)
         u = rng.standard_normal((m.K, m.Np))
         ud = rng.standard_normal((m.K, m.Np))

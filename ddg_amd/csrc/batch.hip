@@ -84,8 +84,9 @@ int op_replicate(const Op& src, int nseg, hipStream_t s, Op** out) {
     set_error("batched solves need a multiple of 64 elements per problem (whole tiles)");
     return DDGX_ERR_INVALID;
   }
-  if (nseg < 1 || src.K * nseg >= (int64_t(1) << 28)) {
-    set_error("batch size out of range");
+  if (nseg < 1 || src.K * nseg >= (int64_t(1) << 28) ||
+      src.K * nseg * src.Np * 8 > MAX_VECTOR_BYTES) {
+    set_error("batch size out of range (K * Np * 8 bytes of a batched vector below 4 GiB)");
     return DDGX_ERR_INVALID;
   }
   const int64_t K1 = src.K, KB = K1 * nseg, T1 = K1 / TILE;

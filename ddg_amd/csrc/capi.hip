@@ -125,6 +125,10 @@ int ddgx_op_create(int N, int64_t K, int dtype, const double* tables, const doub
     set_error("ddgx_op_create: element count out of range [1, 2^28)");
     return DDGX_ERR_INVALID;
   }
+  if (K * ((N + 1) * (N + 2) / 2) * 8 > MAX_VECTOR_BYTES) {
+    set_error("ddgx_op_create: a nodal vector of K * Np float64 values must stay below 4 GiB");
+    return DDGX_ERR_INVALID;
+  }
   // validate neighbour codes on the host: they index device memory
   for (int64_t k = 0; k < K; ++k)
     for (int f = 0; f < 3; ++f) {

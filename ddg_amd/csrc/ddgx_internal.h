@@ -162,6 +162,22 @@ struct CGScalars {
 // k (0..K-1, 64 per tile) holds element perm[k].  Vectors (u, y, ...) keep
 // the caller's element order and are addressed through perm.
 constexpr int TILE = 64;      // elements per matrix-core workgroup
+// Largest nodal vector (in float64 bytes) an operator takes: the apply
+// addresses vectors with 32-bit byte offsets from a uniform base
+// (ld_off / st_off), so K * Np * 8 stays below 2^32.
+constexpr int64_t MAX_VECTOR_BYTES = (int64_t(1) << 32) - 1;
+
+// Element of a device array at a 32-bit byte offset from a base held in
+// scalar registers: one global load or store with a scalar base and a
+// 32-bit vector offset, instead of a 64-bit address per lane.
+template <typename T>
+__device__ __forceinline__ T ld_off(const T* base, unsigned boff) {
+  return *(const T*)((const char*)base + boff);
+}
+template <typename T>
+__device__ __forceinline__ void st_off(T* base, unsigned boff, T v) {
+  *(T*)((char*)base + boff) = v;
+}
 constexpr int LDS_BYTES_PER_CU = 160 * 1024;  // gfx950
 constexpr int HALO_MAX = 64;  // out-of-tile neighbour elements per tile (largest capacity)
 constexpr int HALO_SMALL = 32;  // capacity of the smaller-LDS variant of the apply

@@ -456,9 +456,9 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int n = nn[b][r];
-      const size_t gi = (size_t)kg * Np + (n < Np ? n : Np - 1);
-      U[b][r] = u[gi];
-      if constexpr (NEEDB) BV[b][r] = bvec[gi];
+      const unsigned go = (unsigned)(kg * Np + (n < Np ? n : Np - 1)) * (unsigned)sizeof(T);
+      U[b][r] = ld_off(u, go);
+      if constexpr (NEEDB) BV[b][r] = ld_off(bvec, go);
     }
   // halo: the element ids of the tile's out-of-tile neighbours, 16 per
   // wave, listed from entry 0 (-1 past the end); a wave past the list skips
@@ -474,8 +474,9 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
   static_assert(TILE == 64, "staging by wave and lane");
   const int le = lane < ne ? lane : ne - 1;
 #pragma unroll
-  for (int j = 0; j < RPT; ++j) rc[j] = rec[(size_t)(wv + 4 * j) * K + k0 + le];
-  const int fcv = fcode[(size_t)(wv < 3 ? wv : 2) * K + k0 + le];
+  for (int j = 0; j < RPT; ++j)
+    rc[j] = ld_off(rec + (size_t)(wv + 4 * j) * K, (unsigned)(k0 + le) * (unsigned)sizeof(T));
+  const int fcv = ld_off(fcode + (size_t)(wv < 3 ? wv : 2) * K, (unsigned)(k0 + le) * 4u);
 #pragma unroll
   for (int j = 0; j < HPT; ++j) {
     const int i = 64 * (wv + 4 * j) + lane;
@@ -511,7 +512,7 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int n = nn[b][r];
-        UH[b][r] = u[(size_t)hg * Np + (n < Np ? n : Np - 1)];
+        UH[b][r] = ld_off(u, (unsigned)(hg * Np + (n < Np ? n : Np - 1)) * (unsigned)sizeof(T));
       }
   }
   // records and codes past the tile's end are the clamped copies of its
@@ -696,11 +697,11 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
       if constexpr (ROWS) {
         vv[bo][r] = v;
       } else if (valid && n < Np) {
-        const size_t gi = (size_t)kg * Np + n;
+        const unsigned go = (unsigned)(kg * Np + n) * (unsigned)sizeof(T);
         if constexpr (RESID)
-          y[gi] = (NEEDB ? BV[bo][r] : bvec[gi]) - v;
+          st_off(y, go, (NEEDB ? BV[bo][r] : ld_off(bvec, go)) - v);
         else
-          y[gi] = v;
+          st_off(y, go, v);
         if constexpr (DOT) dacc += (double)U[bo][r] * (double)v;
       }
     }
@@ -716,7 +717,7 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int n = RowMap<T, N>::node_here(bo, g, r);
-        BR[bo][r] = bvec[(size_t)kg * Np + (n < Np ? n : Np - 1)];
+        BR[bo][r] = ld_off(bvec, (unsigned)(kg * Np + (n < Np ? n : Np - 1)) * (unsigned)sizeof(T));
       }
   }
 

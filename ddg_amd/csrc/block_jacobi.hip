@@ -227,12 +227,13 @@ invert_blocks_kernel(const double* __restrict__ blocks, T* __restrict__ binv, in
         out[idx] = (T)A[i][Np + j];
       }
     } else {
-      // upper triangle of the symmetric part, packed (the inverse of a
-      // symmetric block, symmetric up to rounding)
+      // the symmetric part (the inverse of a symmetric block, symmetric up to
+      // rounding), packed by cyclic diagonals (diag_pk): entry (i, j), i <= j,
+      // is written once
       T* out = binv + (size_t)k * SymPack<N>::NPK;
       for (int idx = t; idx < Np * Np; idx += 64) {
         const int i = idx / Np, j = idx % Np;
-        if (i <= j) out[SymPack<N>::pk(i, j)] = (T)(0.5 * (A[i][Np + j] + A[j][Np + i]));
+        if (i <= j) out[diag_pk(Np, i, j)] = (T)(0.5 * (A[i][Np + j] + A[j][Np + i]));
       }
     }
   }

@@ -40,9 +40,10 @@ struct Shape {
   static constexpr int size = oLS + 3 * Nfp * RS;
 };
 
-// Block-Jacobi inverse blocks are symmetric and stored packed: entry (i, j),
-// i <= j, of element k at binv[k * NPK + pk(i, j)], rows of the upper
-// triangle one after another.
+// A symmetric Np x Np matrix packed as its upper triangle: entry (i, j),
+// i <= j, at pk(i, j), rows one after another (the curved operator's volume
+// stiffness and mass).  NPK words per matrix, also the size of a
+// block-Jacobi inverse block, which is packed by diagonals (diag_pk).
 template <int N>
 struct SymPack {
   static constexpr int Np = Shape<N>::Np;
@@ -55,6 +56,41 @@ struct SymPack {
 // Entry (i, j) of a packed symmetric n x n block (SymPack at run-time n)
 __host__ __device__ inline int sym_pk(int n, int i, int j) {
   return i <= j ? i * n - i * (i - 1) / 2 + (j - i) : j * n - j * (j - 1) / 2 + (i - j);
+}
+
+// Block-Jacobi inverse blocks (symmetric) are stored by cyclic diagonals:
+// diagonal q = (j - i) mod n holds the entries (i, i + q mod n), i = 0..n-1,
+// at q * n + i, for q = 0 .. n/2; the diagonal n - q is the same entries read
+// from the other end (entry (i, j) with (j - i) mod n > n/2 is entry (j, i)),
+// and for even n the middle diagonal q = n/2 holds its n/2 distinct entries.
+// n (n + 1) / 2 words per block, as for the packed upper triangle.  A row of
+// the block product read diagonal by diagonal (row i takes diagonal d at
+// column i + d mod n, block_row_diag) has the n lanes of an element read n
+// consecutive words per step: one or two cache lines, where reading a
+// packed upper triangle column by column spreads them over the whole block.
+__host__ __device__ constexpr int diag_pk(int n, int i, int j) {
+  int d = j - i;
+  if (d < 0) d += n;
+  int q = d, row = i;
+  if (2 * d > n) {
+    q = n - d;
+    row = j;
+  }
+  if (2 * q == n && row >= n / 2) row -= n / 2;
+  return q * n + row;
+}
+
+// Row li of B x for one element's block b (diag_pk layout) and its vector v,
+// summed over the cyclic diagonals d = 0..n-1 (column li + d mod n)
+template <typename T, int n, typename V>
+__device__ __forceinline__ T block_row_diag(const T* b, const V& v, int li) {
+  T acc = 0;
+#pragma unroll
+  for (int d = 0; d < n; ++d) {
+    const int j = li + d < n ? li + d : li + d - n;
+    acc += b[diag_pk(n, li, j)] * v(j);
+  }
+  return acc;
 }
 
 // Inverse blocks of a general (curved-element) operator are not symmetric
@@ -95,9 +131,10 @@ __device__ __forceinline__ T block_row(const T* s_b, const T* s_r, int e, int li
   constexpr int Np = Shape<N>::Np, NW = BlockWords<N, FULL>::value;
   const T* b = s_b + e * NW;
   const T* v = s_r + e * Np;
+  if constexpr (!FULL) return block_row_diag<T, Np>(b, [&](int j) { return v[j]; }, li);
   T acc = 0;
 #pragma unroll
-  for (int j = 0; j < Np; ++j) acc += b[FULL ? li * Np + j : SymPack<N>::pk(li, j)] * v[j];
+  for (int j = 0; j < Np; ++j) acc += b[li * Np + j] * v[j];
   return acc;
 }
 

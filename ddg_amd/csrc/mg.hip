@@ -95,7 +95,10 @@ restrict_kernel(const T* __restrict__ P, const T* __restrict__ hi, T* __restrict
       if (e < ne) {
         const T* bk = binv_c + (size_t)(k0 + e) * npk;
         T a = 0;
-        for (int i = 0; i < nlo; ++i) a += bk[sym_pk(nlo, j, i)] * sOut[e * nlo + i];
+        for (int d = 0; d < nlo; ++d) {
+          const int i = j + d < nlo ? j + d : j + d - nlo;
+          a += bk[diag_pk(nlo, j, i)] * sOut[e * nlo + i];
+        }
         x_c[(size_t)(k0 + e) * nlo + j] = omega * a;
       }
     }

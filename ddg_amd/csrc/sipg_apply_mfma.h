@@ -747,9 +747,7 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
         const int kk = k0 + e2;
         const size_t k2 = (size_t)(perm ? perm[kk] : kk);
         const T* bk = binv + k2 * NPK;
-        T a = 0;
-#pragma unroll
-        for (int j = 0; j < Np; ++j) a += bk[SymPack<N>::pk(li, j)] * s_r[e2 * Np + j];
+        const T a = block_row_diag<T, Np>(bk, [&](int j) { return s_r[e2 * Np + j]; }, li);
         const T xn = s_u[e2 * Np + li] + omega * a;
         y[k2 * Np + li] = xn;
         // only this lane reads s_u[e2 * Np + li]: the new value takes its place
@@ -802,7 +800,10 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
         const size_t k2 = (size_t)(perm ? perm[kk] : kk);
         const T* bk = binv + k2 * npk;
         T a = 0;
-        for (int i = 0; i < np_lo; ++i) a += bk[sym_pk(np_lo, j, i)] * s_lo[e2 * np_lo + i];
+        for (int d = 0; d < np_lo; ++d) {
+          const int i = j + d < np_lo ? j + d : j + d - np_lo;
+          a += bk[diag_pk(np_lo, j, i)] * s_lo[e2 * np_lo + i];
+        }
         xco[k2 * np_lo + j] = omega * a;
       }
     }

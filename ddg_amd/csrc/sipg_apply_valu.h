@@ -286,7 +286,7 @@ sipg_apply_valu_kernel(const T* __restrict__ tab, const T* __restrict__ rec,
       for (int i = 0; i < Np; ++i) {
         T acc = 0;
 #pragma unroll
-        for (int j = 0; j < Np; ++j) acc += bkk[SymPack<N>::pk(i, j)] * r[j];
+        for (int j = 0; j < Np; ++j) acc += bkk[diag_pk(Np, i, j)] * r[j];
         v[i] = uk[i] + omega * acc;
       }
       if constexpr (PROLONG_OUT) {
@@ -324,7 +324,7 @@ sipg_apply_valu_kernel(const T* __restrict__ tab, const T* __restrict__ rec,
       if (on) {
         T a = 0;
 #pragma unroll
-        for (int i = 0; i < Np; ++i) a += binv[kcs * NPK + SymPack<N>::pk(j, i)] * s_u[cl * Np + i];
+        for (int i = 0; i < Np; ++i) a += binv[kcs * NPK + diag_pk(Np, j, i)] * s_u[cl * Np + i];
         xco[kcs * Np + j] = omega * a;
       }
     }

@@ -201,7 +201,7 @@ int ddgx_op_create(int N, int64_t K, int dtype, const double* tables, const doub
   // faces the zero slot after the halo (slot TILE + hcap); a slot holds u and
   // dn at the 3 Nfp face nodes
   {
-    const int Nfp = N + 1, SW = 6 * Nfp;
+    const int Nfp = N + 1, SW = trace_slot_words(N);
     for (int64_t i = 0; i < K; ++i) {
       const int own = (int)(i % TILE);
       for (int f = 0; f < 3; ++f) {

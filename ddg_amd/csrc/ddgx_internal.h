@@ -237,6 +237,9 @@ __host__ __device__ inline int fc_type(int c) { return (c >> 11) & 3; }
 __host__ __device__ inline int fc_pack(int slot, int face, int rev, int type) {
   return slot | (face << 8) | (rev << 10) | (type << 11);
 }
+// LDS words of an element's trace slot in the apply: u and dn at the 3 (N+1)
+// face nodes, plus one word of padding that makes the stride odd
+__host__ __device__ constexpr int trace_slot_words(int N) { return 6 * (N + 1) + 1; }
 __host__ __device__ inline int fc_nbase(int c) { return (c >> 13) & 0x1fff; }
 __host__ __device__ inline int fc_nstep(int c) { return (int)((unsigned)c << 3) >> 29; }
 // c with the address fields of a read of slot `slot`, face `face` (node

@@ -81,6 +81,9 @@ struct MS {
   static constexpr int FK = INREG ? 12 : (NF + 3) / 4;  // k-steps over the face rows
   static constexpr int FS = 4 * FK + 1;         // LDS row stride of the face rows (odd)
   static constexpr int TRW = 3 * Nfp * 2;       // trace words per element: u, dn per face node
+  // LDS stride of an element's trace slot: odd, so the 16 elements of a
+  // wave reading the same trace word fall on different banks
+  static constexpr int SW = trace_slot_words(N);
   static constexpr int NFP3 = 3 * Nfp;          // face nodes of an element
   static constexpr int FPW = NFP3 <= 8 ? 8 : NFP3 <= 16 ? 16 : 32;  // face-phase lanes per element
   static constexpr int f1 = 0;                  // GEMM1: 2*NB out blocks x KB
@@ -242,7 +245,7 @@ template <typename T, int N, bool UDIR>
 __device__ __forceinline__ void face_rows_inreg(T (&Fv)[12], const T* s_x, const T* re,
                                                 const int* s_fc, int el, int g,
                                                 const T* __restrict__ udir, int kg) {
-  constexpr int Nfp = N + 1, Np = Shape<N>::Np, SW = MS<N>::TRW;
+  constexpr int Nfp = N + 1, Np = Shape<N>::Np, SW = MS<N>::SW;
   // the vertex slot's face and node for lane groups 0..3, 4 bits each
   constexpr unsigned VF = 0x2102u, VP = (unsigned)(N << 4) | (unsigned)(N << 8);
   auto slot = [&](int s, int& f, int& p) {
@@ -349,12 +352,12 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
   using MF = MfmaT<T>;
   using v4 = typename MF::v4;
   constexpr int Np = Q::Np, Nfp = Q::Nfp, NB = Q::NB, KB = Q::KB, FK = Q::FK;
-  constexpr int NFP3 = Q::NFP3, FPW = Q::FPW, TRW = Q::TRW;
+  constexpr int NFP3 = Q::NFP3, FPW = Q::FPW;
   // LDS layout of GEMM 1's output: u and dn at the face nodes only (the
   // traces a neighbour reads), so four float64 / eight float32 workgroups
   // fit in a compute unit's LDS -- eight float32 tiles per CU cover the 2048
   // tiles of a 2x256x256 mesh in one round.
-  constexpr int SW = TRW;                   // LDS words per element slot
+  constexpr int SW = Q::SW;                 // LDS words per element slot
   constexpr bool HELM = MODE & M_HELM, UDIR = MODE & M_UDIR, DOT = MODE & M_DOT;
   constexpr bool RESID = MODE & M_RESID;
   // smoothing sweep: y = u + omega Binv (bvec - A u) (the epilogue below)
@@ -399,8 +402,9 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
   constexpr bool FLDS = Q::INREG;
   constexpr int NA1 = 2 * NB * KB;                              // GEMM 1
   constexpr int NA2 = NB * (2 * KB + FK) + (HELM ? NB * KB : 0);  // GEMM 2 (+ mass)
-  constexpr int LDS_OWN = (int)sizeof(T) * (XW + TILE * RECS + 6 * HC + (NA1 + NA2) * 64) +
-                          (int)sizeof(int) * 3 * TILE + (int)sizeof(double) * 4;
+  // (LDS is allocated in 512-byte granules)
+  constexpr int LDS_OWN = ((int)sizeof(T) * (XW + TILE * RECS + 6 * HC + (NA1 + NA2) * 64) +
+                           (int)sizeof(int) * 3 * TILE + (int)sizeof(double) * 4 + 511) / 512 * 512;
   constexpr bool A2OWN = FLDS && (LDS_BYTES_PER_CU / LDS_OWN) >= ApplyWaves<T, N>::value;
   constexpr int A1PT = FLDS ? NA1 / 4 : 1, A2PT = FLDS ? NA2 / 4 : 1;
   static_assert(!FLDS || (NA1 % 4 == 0 && NA2 % 4 == 0 && NA2 * 64 <= XW),

@@ -576,6 +576,9 @@ def main():
     ap.add_argument("--profile-copy", type=int, default=0, metavar="MB",
                     help="only run --steps elementwise copies y = 2 x of MB megabytes each "
                          "(calibrates the HBM byte counters)")
+    ap.add_argument("--vcycle", action="store_true",
+                    help="only time one application of the config-1 hp-multigrid V-cycle, "
+                         "float32 and float64")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}): the SIPG apply and "
@@ -612,6 +615,21 @@ def main():
                 frac = apply_bytes(m.K, m.Np, s) / t / 1e9 / HBM_PEAK_GBS
                 print(f"[sweep] N={N} {str(dt)[6:]} K={m.K} {t * 1e6:8.2f} us "
                       f"{m.K * m.Np / t / 1e9:7.1f} GDOF/s  {frac * 100:5.1f}% HBM", flush=True)
+        return
+    if args.vcycle:
+        m, _ = workload_mesh(args.n, order=args.order)
+        out = {}
+        for dt in (torch.float32, torch.float64):
+            mg, _ = hpmg_for(m, 4, dt)
+            r = torch.randn(m.K * m.Np, dtype=torch.float64, device="cuda").to(dt)
+
+            class _V:
+                def apply(self, x, y):
+                    mg(x, y)
+            t, _ = time_apply(_V(), r, torch.empty_like(r), args.steps, args.warmup)
+            out[str(dt)[6:]] = round(t * 1e6, 1)
+        print(json.dumps({"vcycle_us": out, "workload": "config-1 hp-multigrid, 2x256x256 N=4, "
+                          "one preconditioner application (graph replays)"}), flush=True)
         return
     if args.profile_copy:
         n = args.profile_copy * (1 << 20) // 8

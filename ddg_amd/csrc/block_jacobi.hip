@@ -7,8 +7,8 @@
 // zero -- exactly what a colour probe sees, since face neighbours never share
 // a colour -- so Np launches give every block's column j.  The blocks are
 // inverted in float64 by Gauss-Jordan elimination (SPD blocks, no pivoting),
-// one wavefront per element with the augmented block in LDS, and stored as
-// packed symmetric upper triangles in the precision of the vectors they
+// one wavefront per element with the augmented block in LDS, and stored
+// packed by cyclic diagonals (diag_pk) in the precision of the vectors they
 // will multiply.  Blocks of a general (curved-element) operator come from
 // an actual colour probe instead (launch_block_probe_general).
 

@@ -53,11 +53,6 @@ struct SymPack {
   }
 };
 
-// Entry (i, j) of a packed symmetric n x n block (SymPack at run-time n)
-__host__ __device__ inline int sym_pk(int n, int i, int j) {
-  return i <= j ? i * n - i * (i - 1) / 2 + (j - i) : j * n - j * (j - 1) / 2 + (i - j);
-}
-
 // Block-Jacobi inverse blocks (symmetric) are stored by cyclic diagonals:
 // diagonal q = (j - i) mod n holds the entries (i, i + q mod n), i = 0..n-1,
 // at q * n + i, for q = 0 .. n/2; the diagonal n - q is the same entries read

@@ -168,13 +168,16 @@ def test_apply_wide_halo_tiles(dev, N, dtype):
     50 out-of-tile neighbours (the 64-entry halo variant of the matrix-core
     kernel, the global-memory neighbour reads of the vector-ALU kernel);
     also the Morton order of the same mesh (32-entry variant, permuted
-    element order).  Both against the oracle."""
+    element order).  Both against the oracle, with Dirichlet data (the face
+    rows' type-weighted form) and without (neighbour reads through the face
+    codes' addresses: a Dirichlet face reads the zero slot after the halo)."""
     pm, om = unit_square(24, 24, N, jitter=0.1, seed=7)
     bc = pmesh.dirichlet_x_neumann_y(pm)
     rng = np.random.default_rng(70 + N)
     u = rng.standard_normal((pm.Np, pm.K))
     ud = rng.standard_normal((pm.Np, pm.K))
     ref = S.sipg_apply_2d(om, u, bc_types=bc, u_dir=ud) + 2.5 * S.mass_apply(om, u)
+    ref0 = S.sipg_apply_2d(om, u, bc_types=bc)
     tol = 1e-12 if dtype == torch.float64 else 2e-5
     for order in (None, "auto"):
         _say(f"apply wide halo N={N} {dtype} order={order}")
@@ -182,6 +185,10 @@ def test_apply_wide_halo_tiles(dev, N, dtype):
         y = _field(op.apply(_flat(u, dev, dtype), u_dir=_flat(ud, dev, dtype), alpha=2.5),
                    pm.Np, pm.K)
         assert rel(y, ref) < tol
+        y = _field(op.apply(_flat(u, dev, dtype)), pm.Np, pm.K)
+        assert rel(y, ref0) < tol
+        y = _field(op.apply(_flat(u, dev, dtype), alpha=2.5), pm.Np, pm.K)
+        assert rel(y, ref0 + 2.5 * S.mass_apply(om, u)) < tol
 
 
 def test_too_scattered_order_rejected(dev):

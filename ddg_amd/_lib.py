@@ -56,6 +56,11 @@ class DdgxError(RuntimeError):
     pass
 
 
+class DdgxBusyError(DdgxError):
+    """DDGX_ERR_BUSY: the call's shared buffers (a handle's default solve
+    state, a solver or batch context) are held by a concurrent call."""
+
+
 def lib():
     """Load libddgx.so once and declare its signatures."""
     global _lib
@@ -115,5 +120,5 @@ def lib():
 def check(rc: int, what: str = "") -> None:
     if rc != 0:
         msg = lib().ddgx_last_error().decode(errors="replace")
-        kind = {1: ValueError, 2: NotImplementedError}.get(rc, DdgxError)
+        kind = {1: ValueError, 2: NotImplementedError, 5: DdgxBusyError}.get(rc, DdgxError)
         raise kind(f"{what}: {msg}" if what else msg)

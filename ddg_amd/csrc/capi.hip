@@ -18,6 +18,7 @@ struct ddgx_pc_s {
 };
 struct ddgx_batch_s {
   ddgx::Batch* b = nullptr;
+  int busy = 0;  // a solve is running on this context (Claim)
 };
 struct ddgx_solver_s {
   ddgx::Op* op = nullptr;     // operator of the solve (float64 for refinement)
@@ -26,6 +27,7 @@ struct ddgx_solver_s {
   ddgx::Work* w = nullptr;    // Krylov state of op
   ddgx::Work* w32 = nullptr;  // Krylov state of op32
   ddgx::MGWork* mgw = nullptr;  // V-cycle buffers when pc is a multigrid
+  int busy = 0;               // a solve is running on this context (Claim)
 };
 
 namespace ddgx {
@@ -449,6 +451,17 @@ int ddgx_block_jacobi_blocks(ddgx_pc pc, void* blocks, void* stream) {
   return DDGX_OK;
 }
 
+// Refusal of a call whose shared buffers another call holds (Claim).
+static int busy(const char* who, const char* what) {
+  set_error(std::string(who) + ": " + what +
+            " is in use by a concurrent call (a solver context, ddgx_solver_create, gives "
+            "each concurrent solve its own)");
+  return DDGX_ERR_BUSY;
+}
+
+static const char* kOpState = "the operator's default solve state";
+static const char* kMgState = "the multigrid preconditioner's default cycle buffers";
+
 int ddgx_pc_apply(ddgx_pc pc, const void* r, void* z, void* stream) {
   if (pc == nullptr || r == nullptr || z == nullptr) {
     set_error("ddgx_pc_apply: null argument");
@@ -459,8 +472,13 @@ int ddgx_pc_apply(ddgx_pc pc, const void* r, void* z, void* stream) {
     return DDGX_ERR_INVALID;
   }
   const Op& o = *pc->pc.op;
-  if (pc->pc.type == 1)
+  if (pc->pc.type == 1) {
+    // held while the cycle is enqueued: refused during a solve that uses the
+    // same buffers (the cycle itself runs after this call returns)
+    Claim claim;
+    if (!claim.take(&pc->pc.mg->ws_busy)) return busy("ddgx_pc_apply", kMgState);
     return mg_vcycle(*pc->pc.mg, *pc->pc.mg->ws, r, z, nullptr, (hipStream_t)stream);
+  }
   return launch_bj_apply(o.N, o.K, pc->pc.dtype, pc->pc.binv, r, z, (hipStream_t)stream,
                          pc->pc.full);
 }
@@ -524,6 +542,10 @@ int ddgx_pcg_solve(ddgx_op op, double alpha, double shift, ddgx_pc pc, const voi
     set_error("ddgx_pcg_solve: preconditioner built for a different mesh/order");
     return DDGX_ERR_INVALID;
   }
+  Claim claim;
+  if (!claim.take(&op->op.ws_busy)) return busy("ddgx_pcg_solve", kOpState);
+  if (pc != nullptr && pc->pc.type == 1 && !claim.take(&pc->pc.mg->ws_busy))
+    return busy("ddgx_pcg_solve", kMgState);
   Work* w = nullptr;
   const int rc = default_work(op->op, false, &w);
   if (rc != DDGX_OK) return rc;
@@ -545,6 +567,9 @@ int ddgx_pcg_solve_noalloc(ddgx_op op, double alpha, double shift, ddgx_pc pc, c
     set_error("ddgx_pcg_solve_noalloc: preconditioner built for a different mesh/order");
     return DDGX_ERR_INVALID;
   }
+  Claim claim;
+  if (pc != nullptr && pc->pc.type == 1 && !claim.take(&pc->pc.mg->ws_busy))
+    return busy("ddgx_pcg_solve_noalloc", kMgState);
   return pcg_solve_noalloc(op->op, alpha, shift, pc ? &pc->pc : nullptr, b, x, *cfg, info,
                            (hipStream_t)stream);
 }
@@ -570,6 +595,11 @@ int ddgx_ir_solve(ddgx_op op64, ddgx_op op32, double alpha, double shift, ddgx_p
     set_error("ddgx_ir_solve: bad refinement settings");
     return DDGX_ERR_INVALID;
   }
+  Claim claim;
+  if (!claim.take(&op64->op.ws_busy) || !claim.take(&op32->op.ws_busy))
+    return busy("ddgx_ir_solve", kOpState);
+  if (pc32 != nullptr && pc32->pc.type == 1 && !claim.take(&pc32->pc.mg->ws_busy))
+    return busy("ddgx_ir_solve", kMgState);
   Work *w64 = nullptr, *w32 = nullptr;
   int rc = default_work(op64->op, false, &w64);
   if (rc == DDGX_OK) rc = default_work(op32->op, true, &w32);
@@ -599,6 +629,8 @@ int ddgx_dot(ddgx_op op, const void* a, const void* b, double* out, void* stream
     set_error("ddgx_dot: null argument");
     return DDGX_ERR_INVALID;
   }
+  Claim claim;
+  if (!claim.take(&op->op.ws_busy)) return busy("ddgx_dot", kOpState);
   Work* w = nullptr;
   const int rc = default_work(op->op, false, &w);
   if (rc != DDGX_OK) return rc;
@@ -683,6 +715,8 @@ int ddgx_solver_pcg(ddgx_solver sv, double alpha, double shift, const void* b, v
     set_error("ddgx_solver_pcg: maxiter must be >= 0");
     return DDGX_ERR_INVALID;
   }
+  Claim claim;
+  if (!claim.take(&sv->busy)) return busy("ddgx_solver_pcg", "this solver context");
   return pcg_solve(*sv->op, *sv->w, alpha, shift, sv->pc, sv->mgw, b, x, *cfg, info,
                    (hipStream_t)stream);
 }
@@ -701,6 +735,8 @@ int ddgx_solver_ir(ddgx_solver sv, double alpha, double shift, const void* b, vo
     set_error("ddgx_solver_ir: bad refinement settings");
     return DDGX_ERR_INVALID;
   }
+  Claim claim;
+  if (!claim.take(&sv->busy)) return busy("ddgx_solver_ir", "this solver context");
   return ir_solve(*sv->op, *sv->w, *sv->op32, *sv->w32, alpha, shift, sv->pc, sv->mgw, b, x,
                   *cfg, info, (hipStream_t)stream);
 }
@@ -745,6 +781,8 @@ int ddgx_batch_pcg(ddgx_batch bt, const double* alpha, const void* b, void* x,
     set_error("ddgx_batch_pcg: maxiter must be >= 0");
     return DDGX_ERR_INVALID;
   }
+  Claim claim;
+  if (!claim.take(&bt->busy)) return busy("ddgx_batch_pcg", "this batch context");
   DDGX_HIP(hipSetDevice(bt->b->op->device));
   return pcg_batch(*bt->b, alpha, b, x, *cfg, info, (hipStream_t)stream);
 }

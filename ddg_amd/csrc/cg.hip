@@ -1007,6 +1007,7 @@ static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
 
 static std::mutex g_scratch_mu;
 static void* g_scratch_ptr[64] = {};
+static int g_scratch_busy[64] = {};  // a solve holds this GPU's array (Claim)
 
 // Device address of this GPU's copy of `scratch`, looked up once per device
 // with that device current.  The array's name is only a host stand-in; the
@@ -1050,6 +1051,12 @@ int pcg_solve_noalloc(const Op& op, double alpha, double shift, const PC* pc, co
   }
   void* base = nullptr;
   CHECK(scratch_base(op.device, &base));
+  Claim claim;
+  if (!claim.take(&g_scratch_busy[op.device])) {
+    set_error("ddgx_pcg_solve_noalloc: this GPU's fixed solve workspace is in use by a "
+              "concurrent call");
+    return DDGX_ERR_BUSY;
+  }
   unsigned char* q = (unsigned char*)base;
   Work w;
   w.K = op.K;

@@ -290,6 +290,7 @@ struct Op {
   int halo_cap = 0;      // HALO_SMALL or HALO_MAX: halo entries stored per tile
   int grid_vec = 0;      // grid for vector kernels / reductions
   struct Work* ws = nullptr;  // default Krylov state (ddgx_pcg_solve / ddgx_ir_solve / ddgx_dot)
+  int ws_busy = 0;            // ws claimed by a running call (Claim below)
   // A batch replica (op_replicate): nseg disjoint copies of a source operator
   // of K / nseg elements each, sharing the source's table (not owned).
   int nseg = 1;
@@ -378,6 +379,7 @@ struct MG {
   double* Ainv = nullptr;       // coarsest level dense inverse (float64), n x n
   int nc = 0;                   // unknowns of the coarsest level (of one segment)
   MGWork* ws = nullptr;         // default cycle buffers (ddgx_pc_apply, ddgx_pcg_solve)
+  int ws_busy = 0;              // ws claimed by a running call (Claim below)
   // A batch replica (mg_replicate): every level is nseg disjoint copies and
   // the coarse solve applies Ainv to each segment; the level operators are
   // then owned by the hierarchy.
@@ -389,6 +391,32 @@ void mgwork_free(MGWork* w);
 
 void set_error(const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
+
+// Claims shared buffers (a handle's default Krylov state or cycle buffers, a
+// GPU's fixed solve workspace) for the duration of one call: take() sets the
+// flag unless another call holds it, and the destructor releases every flag
+// taken.  Calls that return only after their device work ends (the solves,
+// ddgx_dot) hold the claim over all of that work, so two such calls on one
+// set of buffers cannot overlap: the second is refused instead of racing.
+class Claim {
+ public:
+  Claim() = default;
+  Claim(const Claim&) = delete;
+  Claim& operator=(const Claim&) = delete;
+  ~Claim() {
+    for (int i = 0; i < n_; ++i) __atomic_store_n(held_[i], 0, __ATOMIC_RELEASE);
+  }
+  bool take(int* flag) {
+    if (n_ == kMax || __atomic_exchange_n(flag, 1, __ATOMIC_ACQUIRE) != 0) return false;
+    held_[n_++] = flag;
+    return true;
+  }
+
+ private:
+  static constexpr int kMax = 4;
+  int* held_[kMax] = {};
+  int n_ = 0;
+};
 
 #define DDGX_HIP(call)                                   \
   do {                                                   \

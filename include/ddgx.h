@@ -30,7 +30,8 @@ enum {
   DDGX_ERR_INVALID = 1,      /* bad argument, shape or handle            */
   DDGX_ERR_UNSUPPORTED = 2,  /* polynomial order / dtype not built       */
   DDGX_ERR_HIP = 3,          /* a HIP runtime call failed                */
-  DDGX_ERR_NOMEM = 4         /* device allocation failed                 */
+  DDGX_ERR_NOMEM = 4,        /* device allocation failed                 */
+  DDGX_ERR_BUSY = 5          /* shared buffers held by a concurrent call */
 };
 
 enum { DDGX_F64 = 0, DDGX_F32 = 1 };
@@ -187,8 +188,10 @@ int ddgx_pcg_solve(ddgx_op op, double alpha, double shift, ddgx_pc pc,
  * with a float32 multigrid, twice as many in float32.  Larger solves are
  * refused with DDGX_ERR_INVALID before any device work.  Every call on a GPU
  * shares that GPU's array, and a multigrid preconditioner's cycle buffers are
- * shared with ddgx_pcg_solve and ddgx_pc_apply: calls that overlap on one GPU
- * race.  No graph capture of CG chunks on this path. */
+ * shared with ddgx_pcg_solve and ddgx_pc_apply: a call made while another
+ * solve holds either is refused with DDGX_ERR_BUSY; a ddgx_pc_apply cycle
+ * still running on another stream is not seen and races.  No graph capture
+ * of CG chunks on this path. */
 int ddgx_pcg_solve_noalloc(ddgx_op op, double alpha, double shift, ddgx_pc pc,
                            const void* b, void* x, const ddgx_cg_cfg* cfg,
                            ddgx_cg_info* info, void* stream);
@@ -250,14 +253,19 @@ int ddgx_dot(ddgx_op op, const void* a, const void* b, double* out, void* stream
  * reference, whose JAX solves are functional and hold no state; it exists so
  * that several solves on the same resident operators and preconditioner can
  * run at once on different streams (the request loop in ddg_amd/serve.py).
- * ddgx_pcg_solve / ddgx_ir_solve / ddgx_pc_apply use one default context per
- * handle instead, so their concurrent calls on one handle race.
+ * ddgx_pcg_solve / ddgx_ir_solve / ddgx_dot use one default context per
+ * handle instead (and a multigrid's default cycle buffers), held until the
+ * call returns: a concurrent call on the same handle is refused with
+ * DDGX_ERR_BUSY.  ddgx_pc_apply holds the cycle buffers only while it
+ * enqueues the cycle, which runs after it returns, so its device work on one
+ * stream still races with a solve or another ddgx_pc_apply on another.
  *   op32 == NULL: a PCG solver on op (pc of op's dtype, a float32 multigrid
  *                 for a float64 op, or NULL);
  *   op32 != NULL: a refinement solver (op float64, op32 its float32 twin, pc
  *                 float32 or NULL).
  * The operators and preconditioner must outlive the solver; one solver runs
- * one solve at a time.  On a non-NULL stream a solver replays captured
+ * one solve at a time (a second concurrent call is refused with
+ * DDGX_ERR_BUSY).  On a non-NULL stream a solver replays captured
  * chunks of check_every CG iterations as HIP graphs; an odd check_every is
  * raised by one there (a chunk starts on an even iteration). */
 typedef struct ddgx_solver_s* ddgx_solver;

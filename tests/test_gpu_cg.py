@@ -127,3 +127,41 @@ def test_direct_lu_path(dev):
     _say("direct lu")
     u = D.solve_sipg_poisson_2d(pm, f, bc_types=bc)
     assert rel(u, S.solve_sipg_poisson_2d(om, f, bc_types=bc)) < 1e-10
+
+
+def test_default_context_refuses_concurrent_calls(dev):
+    """ddgx_pcg_solve holds the operator's default solve state until it
+    returns: a dot or a solve on the same handle from another thread meanwhile
+    is refused (DDGX_ERR_BUSY) instead of racing on it, and runs afterwards."""
+    import threading
+    import time
+    from ddg_amd._lib import DdgxBusyError
+    pm, _ = unit_square(96, 96, 4)
+    op = SipgOperator(pm, pmesh.dirichlet_x_neumann_y(pm))
+    b = torch.ones(pm.K * pm.Np, dtype=torch.float64, device=dev)
+    started, refused, ran = threading.Event(), [], []
+
+    def probe():
+        started.wait()
+        time.sleep(0.05)
+        for name, fn in (("dot", lambda: op.dot(b, b)), ("pcg", lambda: op.pcg(b, maxiter=1))):
+            try:
+                fn()
+                ran.append(name)
+            except DdgxBusyError as e:
+                refused.append(str(e))
+
+    th = threading.Thread(target=probe)
+    th.start()
+    _say(f"busy guard K={pm.K}")
+    t0 = time.perf_counter()
+    started.set()
+    # tol 1e-30 is never met: the solve runs all maxiter iterations (> 0.1 s)
+    _, info = op.pcg(b, tol=1e-30, maxiter=20000, check_every=64)
+    dt = time.perf_counter() - t0
+    th.join()
+    _say(f"solve {dt:.3f} s, {info['iterations']} iterations, refused {len(refused)}, ran {ran}")
+    assert dt > 0.2 and info["iterations"] == 20000
+    assert ran == [] and len(refused) == 2
+    assert all("in use by a concurrent call" in m for m in refused)
+    assert op.dot(b, b) == float(pm.K * pm.Np)

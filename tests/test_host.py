@@ -89,6 +89,22 @@ def test_library_exports_every_header_symbol():
     assert set(names) == set(_lib.EXPORTS)
 
 
+def test_error_codes_map_to_python_exceptions():
+    """Every DDGX_ERR_* code of the header raises its Python exception."""
+    import re
+    from ddg_amd import _lib
+    text = (ROOT / "include" / "ddgx.h").read_text()
+    codes = {m.group(1): int(m.group(2)) for m in re.finditer(r"DDGX_(OK|ERR_\w+) = (\d+)", text)}
+    assert codes == {"OK": 0, "ERR_INVALID": 1, "ERR_UNSUPPORTED": 2, "ERR_HIP": 3,
+                     "ERR_NOMEM": 4, "ERR_BUSY": 5}
+    _lib.check(0)
+    kinds = {1: ValueError, 2: NotImplementedError, 3: _lib.DdgxError, 4: _lib.DdgxError,
+             5: _lib.DdgxBusyError}
+    for rc, kind in kinds.items():
+        with pytest.raises(kind):
+            _lib.check(rc, "call")
+
+
 def test_table_size_matches_python():
     from ddg_amd import _lib
     L = _lib.lib()

@@ -203,6 +203,35 @@ def time_apply(op, u, y, steps, warmup, **kw):
     return e0.elapsed_time(e1) / 1e3 / (n_rep * reps), wall
 
 
+def shard_block(n, N, steps, warmup, parts=8):
+    """The float64 and float32 applies on one GPU's share of the 2 n^2 mesh
+    split into ``parts`` contiguous element ranges of the forest order: the
+    first range, a compact patch (the subtree of one root triangle).  Its cut
+    faces are tagged Dirichlet: in a sharded apply they would read the
+    neighbour GPU's traces from the same LDS halo, at the same cost."""
+    import torch
+    from ddg_amd import mesh as pm
+    from ddg_amd.operator import SipgOperator
+    lv = int(round(np.log2(n))) - 1
+    VX, VY, EToV = pm.forest_rectangle_hierarchy(2, lv)[-1]
+    VX, VY = pm.jitter_vertices(VX, VY, 1.0 / n, 0.1, 0)
+    sub = EToV[:len(EToV) // parts]
+    used, inv = np.unique(sub, return_inverse=True)
+    m = pm.build_mesh_2d(VX[used], VY[used], inv.reshape(sub.shape).astype(EToV.dtype), N)
+    bc = pm.all_dirichlet(m)
+    out = {"workload": f"1/{parts} of the 2x{n}x{n} N={N} forest mesh (first element range), "
+                       "cut faces Dirichlet", "K": m.K, "tiles": -(-m.K // 64)}
+    for dt, s, name in ((torch.float64, 8, "f64"), (torch.float32, 4, "f32")):
+        op = SipgOperator(m, bc, dtype=dt)
+        u = torch.randn(m.K * m.Np, dtype=torch.float64, device="cuda").to(dt)
+        t, _ = time_apply(op, u, torch.empty_like(u), steps, warmup)
+        out[name] = {"us": round(t * 1e6, 3), "GDOF/s": round(m.K * m.Np / t / 1e9, 2),
+                     "frac": round(apply_bytes(m.K, m.Np, s) / t / 1e9 / HBM_PEAK_GBS, 4)}
+        print(f"[bench] shard apply {name}: K={m.K} {t * 1e6:.2f} us", file=sys.stderr,
+              flush=True)
+    return out
+
+
 def cpu_baseline(m, bc, budget_s=12.0):
     """Oracle apply (numpy float64) on the same mesh, timed on host cores."""
     from oracle import sipg as S
@@ -710,6 +739,7 @@ def main():
     t32, _ = time_apply(op32, u32, torch.empty_like(u32), args.steps, args.warmup)
     print(f"[bench] apply f32: {t32 * 1e6:.2f} us", file=sys.stderr, flush=True)
     del op, op32
+    shard = shard_block(args.n, 4, args.steps, args.warmup) if args.order == "forest" else None
     copy = copy_bandwidth(args.steps, args.warmup)
     print(f"[bench] device copy: {copy:.0f} GB/s", file=sys.stderr, flush=True)
 
@@ -793,6 +823,7 @@ def main():
                "config0": c0, "config3": c3},
         "serve": serve,
         "curved": curved,
+        "shard": shard,
     }
     print(json.dumps(out), flush=True)
 

@@ -129,39 +129,63 @@ def test_direct_lu_path(dev):
     assert rel(u, S.solve_sipg_poisson_2d(om, f, bc_types=bc)) < 1e-10
 
 
-def test_default_context_refuses_concurrent_calls(dev):
-    """ddgx_pcg_solve holds the operator's default solve state until it
-    returns: a dot or a solve on the same handle from another thread meanwhile
-    is refused (DDGX_ERR_BUSY) instead of racing on it, and runs afterwards."""
+@pytest.mark.parametrize("where", ["default", "context"])
+def test_concurrent_calls_on_busy_state_are_refused(dev, where):
+    """A solve holds its Krylov state until it returns: on the operator's
+    default context (ddgx_pcg_solve) a dot or a solve on the same handle from
+    another thread meanwhile, on a solver context a second solve on it, is
+    refused (DDGX_ERR_BUSY) instead of racing on that state, and runs after."""
     import threading
     import time
+    from ddg_amd import _lib
     from ddg_amd._lib import DdgxBusyError
+    from ddg_amd.serve import SolverContext
     pm, _ = unit_square(96, 96, 4)
     op = SipgOperator(pm, pmesh.dirichlet_x_neumann_y(pm))
     b = torch.ones(pm.K * pm.Np, dtype=torch.float64, device=dev)
     started, refused, ran = threading.Event(), [], []
+    # tol 1e-30 is never met: a solve runs all maxiter iterations (> 0.2 s)
+    long_cfg = _lib.CGConfig(1e-30, 0.0, 20000, 64)
+    if where == "default":
+        probes = (("dot", lambda: op.dot(b, b)), ("pcg", lambda: op.pcg(b, maxiter=1)))
+
+        def long_solve():
+            _, info = op.pcg(b, tol=1e-30, maxiter=20000, check_every=64)
+            return info["iterations"]
+    else:
+        ctx = SolverContext(op)
+        side = torch.cuda.Stream()
+        x1, x2 = torch.zeros_like(b), torch.zeros_like(b)
+        probes = (("ctx.pcg", lambda: ctx.pcg(b, x2, 0.0, _lib.CGConfig(1e-9, 0.0, 1, 16),
+                                              side.cuda_stream)),)
+
+        def long_solve():
+            return ctx.pcg(b, x1, 0.0, long_cfg, side.cuda_stream).iterations
 
     def probe():
         started.wait()
         time.sleep(0.05)
-        for name, fn in (("dot", lambda: op.dot(b, b)), ("pcg", lambda: op.pcg(b, maxiter=1))):
+        for name, fn in probes:
             try:
                 fn()
                 ran.append(name)
             except DdgxBusyError as e:
                 refused.append(str(e))
 
+    torch.cuda.synchronize()  # b and the solution vectors exist before the side stream runs
     th = threading.Thread(target=probe)
     th.start()
-    _say(f"busy guard K={pm.K}")
+    _say(f"busy guard {where} K={pm.K}")
     t0 = time.perf_counter()
     started.set()
-    # tol 1e-30 is never met: the solve runs all maxiter iterations (> 0.1 s)
-    _, info = op.pcg(b, tol=1e-30, maxiter=20000, check_every=64)
+    iterations = long_solve()
     dt = time.perf_counter() - t0
     th.join()
-    _say(f"solve {dt:.3f} s, {info['iterations']} iterations, refused {len(refused)}, ran {ran}")
-    assert dt > 0.2 and info["iterations"] == 20000
-    assert ran == [] and len(refused) == 2
+    torch.cuda.synchronize()
+    _say(f"solve {dt:.3f} s, {iterations} iterations, refused {len(refused)}, ran {ran}")
+    assert dt > 0.2 and iterations == 20000
+    assert ran == [] and len(refused) == len(probes)
     assert all("in use by a concurrent call" in m for m in refused)
+    for _, fn in probes:
+        fn()
     assert op.dot(b, b) == float(pm.K * pm.Np)

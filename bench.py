@@ -579,7 +579,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the config-1 solves by float64 CG without a preconditioner "
                          "and with block-Jacobi (about 15 s)")
-    ap.add_argument("--serve-requests", type=int, default=2000,
+    ap.add_argument("--serve-requests", type=int, default=10000,
                     help="requests in the serving stream (0: skip the serving leg)")
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--serve-max-batch", type=int, default=8,

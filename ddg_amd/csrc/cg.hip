@@ -33,16 +33,7 @@ namespace ddgx {
 
 // what finalize_kernel forms from its two sums (the per-iteration alpha
 // and beta are formed by the update kernels themselves)
-enum { F_INIT_BB = 0, F_INIT_GAMMA = 1, F_SUM = 4 };
-
-__device__ inline double block_sum_256(double v, double* s_red) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double t = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
-  __syncthreads();
-  return t;
-}
+enum { F_INIT_BB = 0, F_INIT_GAMMA = 1, F_SUM = 4, F_STOP = 5 };
 
 // partial[b] = sum a*b ; partial[G + b] = sum c*d (if c != nullptr)
 template <typename T>
@@ -121,35 +112,6 @@ __device__ T cg_alpha(const double* __restrict__ apart, int napart, const CGScal
   return (T)(sc->gq[q] / block_sum_partials(apart, napart, s_red));
 }
 
-// Both sums of partial[0..nparts) and partial[nparts..2 nparts) by a
-// 256-thread block, in one fixed order: each thread adds entries i, i + 256,
-// ... in that order (eight loaded before they are added, so the loads
-// overlap), then block_sum_256.
-__device__ void block_sum2_partials(const double* __restrict__ partial, int nparts,
-                                    double* s_red, double& r0, double& r1) {
-  double s0 = 0.0, s1 = 0.0;
-  int i = threadIdx.x;
-  for (; i + 7 * 256 < nparts; i += 8 * 256) {
-    double a[8], b[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      a[u] = partial[i + u * 256];
-      b[u] = partial[nparts + i + u * 256];
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      s0 += a[u];
-      s1 += b[u];
-    }
-  }
-  for (; i < nparts; i += 256) {
-    s0 += partial[i];
-    s1 += partial[nparts + i];
-  }
-  r0 = block_sum_256(s0, s_red);
-  r1 = block_sum_256(s1, s_red);
-}
-
 // The CG scalars after the <r,z>, <r,r> reduction of an iteration: beta,
 // the new gamma (also into slot gq[q ^ 1]), the residual and the stop test.
 // A zero gamma = <r, M r> with the residual still above the threshold is a
@@ -184,11 +146,23 @@ finalize_kernel(const double* __restrict__ partial, int nparts, CGScalars* sc, i
       break;
     }
     case F_INIT_GAMMA:
+      // gq[1] too: with the direction update in the apply, iteration 0 forms
+      // beta = gamma0 / gq[1] = 1 on a zero p_old, so p0 = z0
       sc->gamma = s0;
       sc->gq[0] = s0;
+      sc->gq[1] = s0;
       sc->rr = sc->use_pc ? s1 : s0;
       sc->rs = sc->rr;
       sc->k = 0;
+      sc->kq[0] = sc->kq[1] = 0;
+      sc->done = !(sc->rs > sc->atol2 && sc->k < sc->maxiter && s0 != 0.0);
+      break;
+    case F_STOP:
+      // the stop test on the last update's partials (the direction-updating
+      // apply makes it at the start of the next iteration): at the end of a
+      // chunk, so the host reads current scalars
+      sc->rr = sc->use_pc ? s1 : s0;
+      sc->rs = sc->rr;
       sc->done = !(sc->rs > sc->atol2 && sc->k < sc->maxiter && s0 != 0.0);
       break;
     default:  // F_SUM
@@ -203,6 +177,13 @@ __global__ void __launch_bounds__(256)
 residual_kernel(const T* __restrict__ b, const T* __restrict__ Ax, T* __restrict__ r, size_t n) {
   const size_t stride = (size_t)gridDim.x * 256;
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) r[i] = b[i] - Ax[i];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+zero_kernel(T* __restrict__ a, size_t n) {
+  const size_t stride = (size_t)gridDim.x * 256;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) a[i] = T(0);
 }
 
 template <typename T>
@@ -519,8 +500,8 @@ static size_t dsize(int dtype) { return dtype == DDGX_F64 ? 8 : 4; }
 void work_free(Work* w) {
   if (w == nullptr) return;
   if (w->chunk) (void)hipGraphExecDestroy(w->chunk);
-  void* ptrs[] = {w->r,    w->p,    w->z,    w->Ap,  w->partials,
-                  w->ir_b, w->ir_x, w->pc_r, w->pc_z, w->scal};
+  void* ptrs[] = {w->r,    w->p,    w->z,    w->Ap,   w->partials, w->ir_b,
+                  w->ir_x, w->pc_r, w->pc_z, w->p2,  w->scal};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (w->scal_host) (void)hipHostFree(w->scal_host);
@@ -557,6 +538,17 @@ int work_create(const Op& op, bool refine, Work** out) {
   w->bytes = 4 * n + np + sizeof(CGScalars) +
              (refine ? 2 * (size_t)op.K * op.Np * sizeof(float) : 0);
   *out = w;
+  return DDGX_OK;
+}
+
+// The second direction vector (first use of the direction update inside the
+// apply).
+static int work_alloc_p2(Work& w) {
+  if (w.p2 != nullptr) return DDGX_OK;
+  const size_t nb = (size_t)w.K * w.Np * dsize(w.dtype);
+  DDGX_HIP(hipSetDevice(w.device));
+  DDGX_HIP(hipMalloc(&w.p2, nb));
+  w.bytes += nb;
   return DDGX_OK;
 }
 
@@ -607,9 +599,10 @@ static int finalize(Work& w, int nparts, int mode, hipStream_t s) {
 }
 
 template <typename T, int N, bool FULL>
-static int update_bj_f(const Op& op, Work& w, void* x, const void* binv, int q, hipStream_t s) {
+static int update_bj_f(const Op& op, Work& w, void* x, const void* binv, int q, hipStream_t s,
+                       const void* p) {
   hipLaunchKernelGGL((update_bj_kernel<T, N, FULL>), dim3(op.grid_vec), dim3(256), 0, s, (T*)x,
-                     (T*)w.r, (const T*)w.p, (const T*)w.Ap, (const T*)binv, (T*)w.z,
+                     (T*)w.r, (const T*)p, (const T*)w.Ap, (const T*)binv, (T*)w.z,
                      (const CGScalars*)w.scal, (int)op.K, w.partials,
                      (const double*)(w.partials + 2 * w.npart), (int)op.grid_apply, q);
   DDGX_HIP(hipGetLastError());
@@ -618,21 +611,21 @@ static int update_bj_f(const Op& op, Work& w, void* x, const void* binv, int q, 
 
 template <typename T, int N>
 static int update_bj(const Op& op, Work& w, void* x, const void* binv, int q, hipStream_t s,
-                     bool full) {
-  return full ? update_bj_f<T, N, true>(op, w, x, binv, q, s)
-              : update_bj_f<T, N, false>(op, w, x, binv, q, s);
+                     bool full, const void* p) {
+  return full ? update_bj_f<T, N, true>(op, w, x, binv, q, s, p)
+              : update_bj_f<T, N, false>(op, w, x, binv, q, s, p);
 }
 
 template <typename T>
 static int update_bj_dispatch(const Op& op, Work& w, void* x, const void* binv, int q,
-                              hipStream_t s, bool full) {
+                              hipStream_t s, bool full, const void* p) {
   switch (op.N) {
-    case 1: return update_bj<T, 1>(op, w, x, binv, q, s, full);
-    case 2: return update_bj<T, 2>(op, w, x, binv, q, s, full);
-    case 3: return update_bj<T, 3>(op, w, x, binv, q, s, full);
-    case 4: return update_bj<T, 4>(op, w, x, binv, q, s, full);
-    case 5: return update_bj<T, 5>(op, w, x, binv, q, s, full);
-    case 6: return update_bj<T, 6>(op, w, x, binv, q, s, full);
+    case 1: return update_bj<T, 1>(op, w, x, binv, q, s, full, p);
+    case 2: return update_bj<T, 2>(op, w, x, binv, q, s, full, p);
+    case 3: return update_bj<T, 3>(op, w, x, binv, q, s, full, p);
+    case 4: return update_bj<T, 4>(op, w, x, binv, q, s, full, p);
+    case 5: return update_bj<T, 5>(op, w, x, binv, q, s, full, p);
+    case 6: return update_bj<T, 6>(op, w, x, binv, q, s, full, p);
   }
   set_error("polynomial order not built");
   return DDGX_ERR_UNSUPPORTED;
@@ -655,11 +648,13 @@ static int check_work(const Op& op, const Work& w) {
 // Record `every` calls of `iteration` on stream s into a graph and make it
 // the context's chunk (updating the existing executable graph in place when
 // the topology is unchanged).
-template <typename F>
-static int capture_chunk(hipGraphExec_t& chunk, int every, F&& iteration, hipStream_t s) {
+template <typename F, typename G>
+static int capture_chunk(hipGraphExec_t& chunk, int every, F&& iteration, G&& tail,
+                         hipStream_t s) {
   DDGX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   int rc = DDGX_OK;
   for (int c = 0; c < every && rc == DDGX_OK; ++c) rc = iteration();
+  if (rc == DDGX_OK) rc = tail();
   hipGraph_t g = nullptr;
   const hipError_t ec = hipStreamEndCapture(s, &g);
   if (rc != DDGX_OK) {
@@ -735,8 +730,18 @@ static int pcg_t(const Op& op, Work& w, double alpha, double shift, const PC* pc
   }
   if (bj) CHECK(launch_bj_apply(op.N, op.K, op.dtype, pc->binv, w.r, w.z, s, pc->full));
   else if (has_pc) CHECK(precondition(nullptr));
-  hipLaunchKernelGGL((copy_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (const T*)z,
-                     (T*)w.p, n);
+  // Without a multigrid, the direction update p = z + beta p_old runs inside
+  // the next apply (one launch fewer per iteration): iteration i reads p_old
+  // from pb[i & 1] and writes p to the other, and iteration 0 starts from a
+  // zero p_old with beta = 1.  Otherwise p0 = z0 and a separate update.
+  const bool fused = (bj || !has_pc) && !w.fixed && apply_pupdate_ok(op);
+  if (fused) CHECK(work_alloc_p2(w));
+  void* const pb[2] = {w.p, w.p2};
+  if (fused)
+    hipLaunchKernelGGL((zero_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (T*)w.p, n);
+  else
+    hipLaunchKernelGGL((copy_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (const T*)z,
+                       (T*)w.p, n);
   DDGX_HIP(hipGetLastError());
   CHECK(dot2<T>(op, w, w.r, z, w.r, w.r, nullptr, s));
   CHECK(finalize(w, op.grid_vec, F_INIT_GAMMA, s));
@@ -748,12 +753,29 @@ static int pcg_t(const Op& op, Work& w, double alpha, double shift, const PC* pc
   auto iteration = [&]() -> int {
     const int q = iter_no & 1;
     ++iter_no;
+    if (fused) {
+      // p = z + beta p_old, Ap = A p, partial <p, Ap>; then x, r, z and the
+      // partials of <r, z>, <r, r> the next iteration's beta needs
+      void* const pn = pb[q ^ 1];
+      CHECK(launch_apply_pupdate(op, z, pb[q], pn, w.Ap, alpha, shift, w.partials + 2 * w.npart,
+                                 w.scal, w.partials, op.grid_vec, q, s));
+      if (bj) {
+        CHECK(update_bj_dispatch<T>(op, w, x, pc->binv, q, s, pc->full, pn));
+      } else {
+        hipLaunchKernelGGL((update_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (T*)x,
+                           (T*)w.r, (const T*)pn, (const T*)w.Ap, (const CGScalars*)w.scal, n,
+                           w.partials, nullptr, (const double*)(w.partials + 2 * w.npart),
+                           (int)op.grid_apply, q, 0);
+        DDGX_HIP(hipGetLastError());
+      }
+      return DDGX_OK;
+    }
     // Ap = A p, partial <p, Ap>
     // (the update kernels form alpha from these partials themselves)
     CHECK(launch_apply(op, w.p, w.Ap, nullptr, alpha, shift, M_DOT, w.partials + 2 * w.npart,
                        done, s));
     if (bj) {
-      CHECK(update_bj_dispatch<T>(op, w, x, pc->binv, q, s, pc->full));
+      CHECK(update_bj_dispatch<T>(op, w, x, pc->binv, q, s, pc->full, w.p));
     } else {
       // with a float32 multigrid the update also writes the scaled float32
       // residual, and the conversion back is fused with the dot products
@@ -780,6 +802,11 @@ static int pcg_t(const Op& op, Work& w, double alpha, double shift, const PC* pc
     DDGX_HIP(hipGetLastError());
     return DDGX_OK;
   };
+  // end of a chunk of iterations: with the direction update in the apply,
+  // the stop test of the last update's residual (else the p update made it)
+  auto chunk_end = [&]() -> int {
+    return fused ? finalize(w, op.grid_vec, F_STOP, s) : DDGX_OK;
+  };
   int every = cfg.check_every > 0 ? cfg.check_every : 16;
   // A solver context on a non-null stream replays a captured chunk of
   // `every` iterations as one graph launch (captured again when the vectors,
@@ -792,7 +819,7 @@ static int pcg_t(const Op& op, Work& w, double alpha, double shift, const PC* pc
     const bool same = w.chunk && w.g_x == x && w.g_pc == (const void*)pc &&
                       w.g_mgw == (const void*)mgw && w.g_alpha == alpha &&
                       w.g_shift == shift && w.g_every == every && w.g_stream == s;
-    if (!same) CHECK(capture_chunk(w.chunk, every, iteration, s));
+    if (!same) CHECK(capture_chunk(w.chunk, every, iteration, chunk_end, s));
     w.g_x = x;
     w.g_pc = pc;
     w.g_mgw = mgw;
@@ -811,6 +838,7 @@ static int pcg_t(const Op& op, Work& w, double alpha, double shift, const PC* pc
       launched += every;
     } else {
       for (int c = 0; c < every && launched < cfg.maxiter; ++c, ++launched) CHECK(iteration());
+      CHECK(chunk_end());
     }
   }
   const CGScalars& h = *w.scal_host;
@@ -959,7 +987,7 @@ int pcg_batch(Batch& bt, const double* alpha, const void* b, void* x, const ddgx
   if (graph) {
     const bool same = bt.chunk && bt.g_x == x && bt.g_a == a_apply && bt.g_every == every &&
                       bt.g_stream == s;
-    if (!same) CHECK(capture_chunk(bt.chunk, every, iteration, s));
+    if (!same) CHECK(capture_chunk(bt.chunk, every, iteration, [] { return DDGX_OK; }, s));
     bt.g_x = x;
     bt.g_a = a_apply;
     bt.g_every = every;
@@ -1079,6 +1107,7 @@ int pcg_solve_noalloc(const Op& op, double alpha, double shift, const PC* pc, co
   CGScalars host{};          // host side of the scalars: lives until the solve returns
   w.scal_host = &host;
   w.use_graph = false;       // no graph instantiation (it allocates)
+  w.fixed = true;            // nor a second direction vector
   return pcg_solve(op, w, alpha, shift, pc, nullptr, b, x, cfg, info, s);
 }
 

@@ -174,10 +174,12 @@ constexpr int CODE_NONE = -2;
 // level's first smoothing sweep, launched through launch_resid_restrict;
 // M_PROLONG_OUT, with M_SMOOTH: the sweep's result is also prolongated into
 // the next-finer level's iterate, p-transfers, launched through
-// launch_smooth's prolong_to)
+// launch_smooth's prolong_to; M_PUPDATE, with M_DOT: CG's direction update
+// p = z + beta p_old runs inside the apply, launched through
+// launch_apply_pupdate)
 enum : int {
   M_HELM = 1, M_UDIR = 2, M_DOT = 8, M_RESID = 16, M_SMOOTH = 32, M_PROLONG = 64,
-  M_RESTRICT = 128, M_PROLONG_OUT = 256
+  M_RESTRICT = 128, M_PROLONG_OUT = 256, M_PUPDATE = 512
 };
 
 // Device scalars of one CG solve
@@ -188,7 +190,51 @@ struct CGScalars {
   // gamma of iteration i at gq[i & 1]: the kernel that forms the next gamma
   // writes the other slot while its blocks still read this one
   double gq[2];
+  // iterations begun, by the parity of the iteration about to begin (the
+  // apply that forms the direction reads kq[i & 1] and its block 0 writes
+  // kq[(i + 1) & 1] and k)
+  int kq[2];
 };
+
+// Sum over a 256-thread block (every thread gets the sum; s_red: 4 doubles
+// of LDS); the fixed order makes the bits independent of timing.
+__device__ inline double block_sum_256(double v, double* s_red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
+  __syncthreads();
+  return t;
+}
+
+// Both sums of partial[0..nparts) and partial[nparts..2 nparts) by a
+// 256-thread block, in one fixed order: each thread adds entries i, i + 256,
+// ... in that order (eight loaded before they are added, so the loads
+// overlap), then block_sum_256.
+__device__ inline void block_sum2_partials(const double* __restrict__ partial, int nparts,
+                                           double* s_red, double& r0, double& r1) {
+  double s0 = 0.0, s1 = 0.0;
+  int i = threadIdx.x;
+  for (; i + 7 * 256 < nparts; i += 8 * 256) {
+    double a[8], b[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      a[u] = partial[i + u * 256];
+      b[u] = partial[nparts + i + u * 256];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      s0 += a[u];
+      s1 += b[u];
+    }
+  }
+  for (; i < nparts; i += 256) {
+    s0 += partial[i];
+    s1 += partial[nparts + i];
+  }
+  r0 = block_sum_256(s0, s_red);
+  r1 = block_sum_256(s1, s_red);
+}
 
 // Per-element data of an operator is stored in tile order: tile position
 // k (0..K-1, 64 per tile) holds element perm[k].  Vectors (u, y, ...) keep
@@ -313,6 +359,11 @@ struct Work {
   void* ir_x = nullptr;
   void* pc_r = nullptr;        // float64 solve with a float32 multigrid: residual and
   void* pc_z = nullptr;        // preconditioned residual in float32 (allocated on first use)
+  // second direction vector: with the direction update inside the apply,
+  // iteration i reads p_old from one and writes p to the other (by parity);
+  // allocated on first use unless the state lives in fixed memory
+  void* p2 = nullptr;
+  bool fixed = false;
   CGScalars* scal = nullptr;
   CGScalars* scal_host = nullptr;  // pinned
   size_t bytes = 0;            // device bytes held
@@ -428,6 +479,16 @@ class Claim {
 int launch_apply(const Op& op, const void* u, void* y, const void* udir, double alpha,
                  double shift, int mode, double* partial, const int* skip, hipStream_t s,
                  const void* bvec = nullptr);
+// A p with CG's direction update in the same launch (affine float64
+// operators of order 2..4, apply_pupdate_ok): every workgroup forms gamma = <r, z> (and <r, r>) from the update's
+// partials rz (2 x nrz), beta = gamma / gq[q ^ 1] and the stop test, then
+// applies p = z + beta p_old (tile and halo), writes p for its tile and the
+// partial <p, A p>; block 0 records gamma in gq[q], the residual and the
+// iteration count (kq), or the stop flag.
+bool apply_pupdate_ok(const Op& op);
+int launch_apply_pupdate(const Op& op, const void* z, const void* p_old, void* p, void* Ap,
+                         double alpha, double shift, double* partial, CGScalars* sc,
+                         const double* rz, int nrz, int q, hipStream_t s);
 // One fused block-Jacobi sweep x_out = x_in + omega Binv (b - (A + alpha M) x_in)
 // (x_out must not alias x_in; Binv packed, element order, the operator's dtype).
 bool smooth_fused(const Op& op);

@@ -28,7 +28,8 @@ static int launch_t(const Op& op, const void* u, void* y, const void* udir, doub
                     double shift, double* partial, const int* skip, hipStream_t s,
                     const void* bvec, const void* binv = nullptr, double omega = 0.0,
                     const void* pxc = nullptr, const void* pmat = nullptr, int npt = 0,
-                    void* xco = nullptr) {
+                    void* xco = nullptr, CGScalars* sc = nullptr, const double* rzp = nullptr,
+                    int nrz = 0, int q = 0) {
   // one workgroup per 64-element tile; in dot mode each writes one partial
   const int* perm = op.perm_id ? nullptr : op.perm;
   if constexpr (UseValuApply<T, N>::value)
@@ -42,13 +43,13 @@ static int launch_t(const Op& op, const void* u, void* y, const void* udir, doub
                        dim3(256), 0, s, (const T*)op.frag, (const T*)op.rec, op.fcode, perm,
                        op.hpos, (const T*)op.hab, (const T*)u, (T*)y, (const T*)udir, (T)alpha,
                        (T)shift, (int)op.K, partial, skip, (const T*)bvec, (const T*)binv,
-                       (T)omega, (const T*)pmat, npt, (T*)xco);
+                       (T)omega, (const T*)pmat, npt, (T*)xco, sc, rzp, nrz, q);
   else
     hipLaunchKernelGGL((sipg_apply_mfma_kernel<T, N, MODE, HALO_MAX>), dim3(op.grid_apply),
                        dim3(256), 0, s, (const T*)op.frag, (const T*)op.rec, op.fcode, perm,
                        op.hpos, (const T*)op.hab, (const T*)u, (T*)y, (const T*)udir, (T)alpha,
                        (T)shift, (int)op.K, partial, skip, (const T*)bvec, (const T*)binv,
-                       (T)omega, (const T*)pmat, npt, (T*)xco);
+                       (T)omega, (const T*)pmat, npt, (T*)xco, sc, rzp, nrz, q);
   DDGX_HIP(hipGetLastError());
   return DDGX_OK;
 }
@@ -66,6 +67,42 @@ static int launch_mode(const Op& op, const void* u, void* y, const void* udir, d
       set_error("unsupported apply mode " + std::to_string(mode));
       return DDGX_ERR_UNSUPPORTED;
   }
+}
+
+// Float64, orders 2-4: there the apply holds the direction update's extra
+// values (z and p_old of the tile and the halo) without register spills at
+// its four waves per SIMD; float32 and orders 5-6 would spill.
+bool apply_pupdate_ok(const Op& op) {
+  return op.cv == nullptr && op.dtype == DDGX_F64 && op.N >= 2 && op.N <= 4 && op.nseg == 1;
+}
+
+template <typename T, int N>
+static int pupdate_n(const Op& op, const void* z, const void* p_old, void* p, void* Ap,
+                     double alpha, double shift, double* partial, CGScalars* sc,
+                     const double* rz, int nrz, int q, hipStream_t s) {
+  if (alpha != 0.0 || shift != 0.0)
+    return launch_t<T, N, M_DOT | M_PUPDATE | M_HELM>(op, z, Ap, nullptr, alpha, shift, partial,
+                                                      nullptr, s, p_old, nullptr, 0.0, nullptr,
+                                                      nullptr, 0, p, sc, rz, nrz, q);
+  return launch_t<T, N, M_DOT | M_PUPDATE>(op, z, Ap, nullptr, alpha, shift, partial, nullptr, s,
+                                           p_old, nullptr, 0.0, nullptr, nullptr, 0, p, sc, rz,
+                                           nrz, q);
+}
+
+int launch_apply_pupdate(const Op& op, const void* z, const void* p_old, void* p, void* Ap,
+                         double alpha, double shift, double* partial, CGScalars* sc,
+                         const double* rz, int nrz, int q, hipStream_t s) {
+  if (!apply_pupdate_ok(op)) {
+    set_error("the direction update inside the apply needs an affine float64 operator of "
+              "order 2..4");
+    return DDGX_ERR_UNSUPPORTED;
+  }
+#define PCASE(T, n) \
+  case n: return pupdate_n<T, n>(op, z, p_old, p, Ap, alpha, shift, partial, sc, rz, nrz, q, s);
+  switch (op.N) { PCASE(double, 2) PCASE(double, 3) PCASE(double, 4) }
+#undef PCASE
+  set_error("polynomial order not built");
+  return DDGX_ERR_UNSUPPORTED;
 }
 
 int launch_apply(const Op& op, const void* u, void* y, const void* udir, double alpha,

@@ -347,7 +347,8 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
                        T alpha, T shift, int K, double* __restrict__ partial,
                        const int* __restrict__ skip, const T* __restrict__ bvec,
                        const T* __restrict__ binv, T omega, const T* __restrict__ pmat,
-                       int npt, T* __restrict__ xco) {
+                       int npt, T* __restrict__ xco, CGScalars* __restrict__ sc,
+                       const double* __restrict__ rzp, int nrz, int q) {
   using Q = MS<N>;
   using MF = MfmaT<T>;
   using v4 = typename MF::v4;
@@ -372,6 +373,10 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
   // p-level's iterate, xco[k] += pmat x_new[k] (pmat: npt x Np)
   constexpr bool PROLONG_OUT = MODE & M_PROLONG_OUT;
   static_assert(!PROLONG_OUT || SMOOTH, "prolongation of a sweep's result");
+  // with DOT: CG's direction update first, p = u + beta bvec (u = z, bvec =
+  // p_old), p written to xco for the tile (the prologue below)
+  constexpr bool PUPD = MODE & M_PUPDATE;
+  static_assert(!PUPD || (DOT && !UDIR && !RESID && !SMOOTH), "direction update of a CG apply");
   // the transfer's other level: np_lo = npt coarse nodes (RESTRICT)
   const int np_lo = npt;
   constexpr bool ROWS = SMOOTH || RESTRICT;  // epilogue on element rows in LDS
@@ -413,7 +418,9 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
   __shared__ T s_a1[FLDS ? (A2OWN ? NA1 + NA2 : NA1) * 64 : 1];
   T* s_a2 = A2OWN ? s_a1 + NA1 * 64 : s_x;
 
-  if (skip != nullptr && *skip) return;
+  if constexpr (!PUPD) {
+    if (skip != nullptr && *skip) return;
+  }
 
   const int tid = threadIdx.x;
   // the wave index is uniform: held in a scalar register, so the staging
@@ -462,7 +469,7 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
       const int n = nn[b][r];
       const unsigned go = (unsigned)(kg * Np + (n < Np ? n : Np - 1)) * (unsigned)sizeof(T);
       U[b][r] = ld_off(u, go);
-      if constexpr (NEEDB) BV[b][r] = ld_off(bvec, go);
+      if constexpr (NEEDB || PUPD) BV[b][r] = ld_off(bvec, go);
     }
   // halo: the element ids of the tile's out-of-tile neighbours, 16 per
   // wave, listed from entry 0 (-1 past the end); a wave past the list skips
@@ -509,6 +516,7 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
   const int hk = el < HC ? hkv : -1;
   const bool hwave = __ballot(hk >= 0) != 0;  // uniform per wave
   T UH[NB][4] = {};
+  T PH[NB][4] = {};  // the halo's p_old (direction update only)
   if (hwave) {
     const int hg = hk >= 0 ? hk : 0;
 #pragma unroll
@@ -516,8 +524,53 @@ sipg_apply_mfma_kernel(const T* __restrict__ frag, const T* __restrict__ rec,
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int n = nn[b][r];
-        UH[b][r] = ld_off(u, (unsigned)(hg * Np + (n < Np ? n : Np - 1)) * (unsigned)sizeof(T));
+        const unsigned ho = (unsigned)(hg * Np + (n < Np ? n : Np - 1)) * (unsigned)sizeof(T);
+        UH[b][r] = ld_off(u, ho);
+        if constexpr (PUPD) PH[b][r] = ld_off(bvec, ho);
       }
+  }
+  if constexpr (PUPD) {
+    // CG's direction update, with the tile's and the halo's loads in flight:
+    // gamma = <r, z> and <r, r> from the update's partials, in the order every
+    // other reduction of the solve uses.  Every workgroup takes the same stop
+    // decision from them, so none depends on what block 0 writes here.
+    __shared__ double s_pr[4];
+    double s0, s1;
+    block_sum2_partials(rzp, nrz, s_pr, s0, s1);
+    const double rs = sc->use_pc ? s1 : s0, g_old = sc->gq[q ^ 1];
+    const int kb = sc->kq[q];
+    const bool stop = sc->done || !(rs > sc->atol2 && kb < sc->maxiter && s0 != 0.0);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->rr = rs;
+      sc->rs = rs;
+      if (stop) {
+        sc->done = 1;
+      } else {
+        sc->gamma = s0;
+        sc->gq[q] = s0;
+        sc->beta = s0 / g_old;
+        sc->kq[q ^ 1] = kb + 1;
+        sc->k = kb + 1;
+      }
+    }
+    if (stop) return;
+    const T beta = (T)(s0 / g_old);
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        U[b][r] += beta * BV[b][r];
+        UH[b][r] += beta * PH[b][r];
+      }
+    // the tile's new direction (padding positions repeat node Np - 1)
+    if (valid) {
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (nn[b][r] < Np)
+            st_off(xco, (unsigned)(kg * Np + nn[b][r]) * (unsigned)sizeof(T), U[b][r]);
+    }
   }
   // records and codes past the tile's end are the clamped copies of its
   // last element's, read only by the lanes past the end

@@ -95,13 +95,16 @@ def test_graph_mixed_preconditioned_pcg_equals_eager(problem):
         torch.cuda.synchronize()
 
 
-@pytest.mark.parametrize("case", ["odd_chunk", "block_jacobi", "maxiter_13"])
+@pytest.mark.parametrize("case", ["odd_chunk", "block_jacobi", "maxiter_13", "bj_odd_chunk",
+                                  "bj_maxiter_13", "plain_maxiter_13"])
 def test_graph_chunk_edge_cases(problem, case):
     """An odd check_every (raised to an even chunk under capture), the
     block-Jacobi update fused into the captured chunk, and a maxiter that is
     not a multiple of the chunk (the captured chunk runs past it; the
     kernels after the stop flag return at once): replays equal the eager
-    solve bit for bit."""
+    solve bit for bit.  The bj_ and plain_ cases run the float64 iteration whose
+    direction update is inside the apply (two launches per iteration and a
+    stop test per chunk) under the same edge cases."""
     pm, op64, op32, mg64, mg32, b = problem
     every, tol, maxiter, pc = 8, 1e-11, 500, mg64
     if case == "odd_chunk":
@@ -109,6 +112,12 @@ def test_graph_chunk_edge_cases(problem, case):
     elif case == "block_jacobi":
         pc = op64.block_jacobi()
         tol, maxiter = 1e-9, 5000
+    elif case == "bj_odd_chunk":
+        pc, every, tol, maxiter = op64.block_jacobi(), 7, 1e-9, 5000
+    elif case == "bj_maxiter_13":
+        pc, tol, maxiter = op64.block_jacobi(), 1e-30, 13
+    elif case == "plain_maxiter_13":
+        pc, tol, maxiter = None, 1e-30, 13
     else:
         tol, maxiter = 1e-30, 13
     _say(f"graph edge case {case}: every={every} maxiter={maxiter}")
@@ -126,7 +135,7 @@ def test_graph_chunk_edge_cases(problem, case):
         assert torch.equal(x, x_e), f"replay {rep} differs from the eager solve"
         op64.apply(x)
         torch.cuda.synchronize()
-    if case == "maxiter_13":
+    if case.endswith("maxiter_13"):
         assert info_e["iterations"] == 13 and not info_e["converged"]
     else:
         assert info_e["converged"]

@@ -730,11 +730,12 @@ static int pcg_t(const Op& op, Work& w, double alpha, double shift, const PC* pc
   }
   if (bj) CHECK(launch_bj_apply(op.N, op.K, op.dtype, pc->binv, w.r, w.z, s, pc->full));
   else if (has_pc) CHECK(precondition(nullptr));
-  // Without a multigrid, the direction update p = z + beta p_old runs inside
-  // the next apply (one launch fewer per iteration): iteration i reads p_old
-  // from pb[i & 1] and writes p to the other, and iteration 0 starts from a
-  // zero p_old with beta = 1.  Otherwise p0 = z0 and a separate update.
-  const bool fused = (bj || !has_pc) && !w.fixed && apply_pupdate_ok(op);
+  // The direction update p = z + beta p_old runs inside the next apply (one
+  // launch fewer per iteration): iteration i reads p_old from pb[i & 1] and
+  // writes p to the other, and iteration 0 starts from a zero p_old with
+  // beta = 1.  Where the apply cannot (float32, orders 1 and 5-6, curved
+  // operators, the fixed-memory state) p0 = z0 and a separate update.
+  const bool fused = !w.fixed && apply_pupdate_ok(op);
   if (fused) CHECK(work_alloc_p2(w));
   void* const pb[2] = {w.p, w.p2};
   if (fused)
@@ -764,9 +765,19 @@ static int pcg_t(const Op& op, Work& w, double alpha, double shift, const PC* pc
       } else {
         hipLaunchKernelGGL((update_kernel<T>), dim3(op.grid_vec), dim3(256), 0, s, (T*)x,
                            (T*)w.r, (const T*)pn, (const T*)w.Ap, (const CGScalars*)w.scal, n,
-                           w.partials, nullptr, (const double*)(w.partials + 2 * w.npart),
-                           (int)op.grid_apply, q, 0);
+                           has_pc ? nullptr : w.partials, mixed ? (float*)w.pc_r : nullptr,
+                           (const double*)(w.partials + 2 * w.npart), (int)op.grid_apply, q, 0);
         DDGX_HIP(hipGetLastError());
+        if (mixed) {
+          CHECK(mg_vcycle(*pc->mg, *mgw, w.pc_r, w.pc_z, done, s));
+          hipLaunchKernelGGL(pc_up_dot_kernel, dim3(op.grid_vec), dim3(256), 0, s,
+                             (const float*)w.pc_z, (double*)w.z, (const double*)w.r,
+                             (const CGScalars*)w.scal, n, w.partials, done, 0);
+          DDGX_HIP(hipGetLastError());
+        } else if (has_pc) {
+          CHECK(precondition(done));
+          CHECK(dot2<T>(op, w, w.r, w.z, w.r, w.r, done, s));
+        }
       }
       return DDGX_OK;
     }
